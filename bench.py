@@ -1,0 +1,210 @@
+"""Benchmark: bfps vorticity-equation step + tracer loop on one MI355X.
+
+Workload (precision form): 512^3 periodic box, FP32 FFT / nonlinear
+arithmetic with FP64 integrating factors, reductions and tracer positions,
+nu = 0.0012, E(k) ~ k^4 exp(-2 (k/4)^2) initial field (total energy 1/2),
+linear band forcing 0.5 on 2 <= |k| <= 4, dt = 0.5/N, 2^22 tracers with
+6-neighbour smoothness-2 spline interpolation and AB4.  One step =
+NSVEparticles::step (tracer completeLoop with u(t), then the IF-SSP-RK3
+vorticity step).  Velocity-gradient sampling of the tracers (9 components)
+is timed separately.  The same workload in FP64 is reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fp64]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+
+
+def step_bytes(n, es, with_tracer_u):
+    """Algorithmic HBM bytes of one fused vorticity step: 141 field volumes
+    (N^3 elements of size es) + 3 for the real velocity handed to the
+    tracers."""
+    return (141 + (3 if with_tracer_u else 0)) * n ** 3 * es
+
+
+# algorithmic bytes per launch of each kernel, in field volumes N^3 * es
+KERNEL_VOLUMES = {
+    "inv_z_bs": 9,         # read omega (3 complex planes), write u, omega (6)
+    "inv_y": 12,           # 6 planes read + written
+    "rows_cross": 9,       # read 6, write 3 (+3 real velocity in stage 1)
+    "fwd_z": 6,            # 3 planes read + written
+    "fwd_y_update1": 9,    # read N (3), v0 (3), write v1 (3)
+    "fwd_y_update2": 12,   # read N, v0, v1, write v2
+    "fwd_y_update3": 12,   # read N, v0, v2, write v0
+}
+
+
+def tracer_bytes(np_, nsteps=4):
+    """positions read+write (48 B), history read (8 (nsteps-1) 3 B) and
+    write (24 B) per tracer."""
+    return np_ * (48 + 24 * (nsteps - 1) + 24)
+
+
+def make_case(B, ic, n, precision, npart, nb, m, seed=1):
+    fs = B.vorticity_equation("bench", n, precision=precision)
+    fs.nu, fs.famplitude, fs.fk0, fs.fk1, fs.forcing_type = 0.0012, 0.5, 2.0, 4.0, "linear"
+    w = ic.solenoidal_vorticity(n, seed=seed, k0=4.0,
+                                dtype=__import__("numpy").complex64 if precision == "single" else complex)
+    fs.io_checkpoint(read=True, data=w)
+    del w
+    ps = None
+    if npart:
+        ps = B.particles_system(fs, npart, nb, m, 4, step_idx=1)
+        ps.put(ic.uniform_positions(npart, seed=seed + 1))
+    return fs, ps
+
+
+def run_precision(B, ic, args, precision, steps, warmup):
+    n = args.n
+    fs, ps = make_case(B, ic, n, precision, args.np, args.nb, args.m)
+    dt = 0.5 / n
+    es = 4 if precision == "single" else 8
+    fs._push_physics()
+    for _ in range(warmup):
+        B.NSVEparticles_step(fs, dt)
+    fs.synchronize()
+    fs.profile(True)
+    t0 = time.perf_counter()
+    ms = fs.time_steps(steps, dt, with_particles=True)
+    wall = time.perf_counter() - t0
+    prof = fs.profile_report()
+    fs.profile(False)
+    # gradient sampling (velocity gradient, 9 components, at all tracers)
+    fs.synchronize()
+    t1 = time.perf_counter()
+    reps = 2
+    for _ in range(reps):
+        fs.gradient_rspace()
+        g = ps.sample("gradient") if ps is not None else None
+    fs.synchronize()
+    grad_s = (time.perf_counter() - t1) / reps
+    del g
+    mem = fs.device_bytes()
+    res = {
+        "ms_per_step": ms / steps,
+        "wall_ms_per_step": 1e3 * wall / steps,
+        "profile": prof,
+        "grad_sample_s": grad_s,
+        "device_bytes": mem,
+        "es": es,
+    }
+    fs.close()
+    return res
+
+
+def roofline(prof, n, es, steps):
+    """dominant kernel by total time; achieved = algorithmic bytes per launch
+    / average launch duration."""
+    best = None
+    for name, (ms, cnt) in prof.items():
+        if best is None or ms > best[1]:
+            best = (name, ms, cnt)
+    name, ms, cnt = best
+    vol = KERNEL_VOLUMES.get(name)
+    avg_s = ms / cnt / 1e3
+    if vol is None:
+        return {"kernel": name, "avg_ms": ms / cnt}
+    bytes_per_launch = vol * n ** 3 * es
+    ach = bytes_per_launch / avg_s
+    return {"bound": "hbm", "kernel": name, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": ach / HBM_PEAK, "traffic": None, "avg_ms": ms / cnt, "bytes_per_launch": bytes_per_launch}
+
+
+def cpu_baseline(ic, n, threads):
+    """The oracle (CPU restatement, C++/OpenMP) on one full step of the same
+    512^3 FP32 field workload (no tracers), on `threads` host threads."""
+    from oracle import oracle as O
+    O.build()
+    O.set_threads(threads)
+    s = O.Solver(n, precision="single")
+    s.set_physics(0.0012, "linear", 0.5, 2.0, 4.0)
+    s.set_cfield("vorticity", ic.solenoidal_vorticity(n, seed=1, k0=4.0))
+    s.io_projection()
+    t0 = time.perf_counter()
+    s.step(0.5 / n)
+    dt = time.perf_counter() - t0
+    del s
+    return {"value": n ** 3 / dt, "unit": "grid-pt*steps/s", "cores": threads, "kind": "port",
+            "sample": "1 step of the %d^3 FP32 vorticity step (no tracers) by the CPU oracle, %.1f s" % (n, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--np", type=int, default=2 ** 22)
+    ap.add_argument("--nb", type=int, default=6)
+    ap.add_argument("--m", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import bfps_amd as B
+    from bfps_amd import ic
+
+    r32 = run_precision(B, ic, args, "single", args.steps, args.warmup)
+    n = args.n
+    ms = r32["ms_per_step"]
+    value = n ** 3 / (ms / 1e3)
+    tb = tracer_bytes(args.np)
+    alg = step_bytes(n, 4, args.np > 0) + tb
+    out = {
+        "metric": "grid-pt*steps/s + particle interps/s, FP32 vs FP64 at 512^3, % HBM peak",
+        "value": value,
+        "unit": "grid-pt*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 factors, reductions, tracer positions)",
+        "data": "synthetic: seeded solenoidal field E(k)~k^4 exp(-2(k/4)^2), seeded uniform tracers",
+        "config": {"workload": "512^3 precision form: FP32 vorticity step + 2^22 tracers n6 m2 AB4",
+                   "n": n, "nu": 0.0012, "dt": 0.5 / n, "tracers": args.np, "neighbours": args.nb,
+                   "smoothness": args.m},
+        "particle_interps_per_s": args.np / (ms / 1e3),
+        "gradient_sample_interps_per_s": args.np / r32["grad_sample_s"] if args.np else None,
+        "hbm_frac_step": alg / (ms / 1e3) / HBM_PEAK,
+        "algorithmic_bytes_per_step": alg,
+        "roofline": roofline(r32["profile"], n, 4, args.steps),
+        "kernel_ms": {k: v[0] / v[1] for k, v in r32["profile"].items()},
+        "wall_ms_per_step": r32["wall_ms_per_step"],
+        "device_gb": r32["device_bytes"] / 1e9,
+    }
+    if not args.no_fp64:
+        r64 = run_precision(B, ic, args, "double", max(2, args.steps // 2), 1)
+        ms64 = r64["ms_per_step"]
+        out["fp64"] = {"value": n ** 3 / (ms64 / 1e3), "ms_per_step": ms64,
+                       "particle_interps_per_s": args.np / (ms64 / 1e3),
+                       "hbm_frac_step": (step_bytes(n, 8, args.np > 0) + tb) / (ms64 / 1e3) / HBM_PEAK,
+                       "roofline": roofline(r64["profile"], n, 8, max(2, args.steps // 2)),
+                       "kernel_ms": {k: v[0] / v[1] for k, v in r64["profile"].items()}}
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(ic, n, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,273 @@
+"""bfps_amd -- MI355X (gfx950) backend for bfps's vorticity-equation step and
+tracer loop.
+
+Python mirror of the reference's class surface over the C ABI
+(include/bfpsh.h):
+
+* ``vorticity_equation``  <- ``vorticity_equation<rnumber, FFTW>``
+  (bfps/cpp/vorticity_equation.hpp:46-134)
+* ``kspace``              <- ``kspace<FFTW, SMOOTH>`` metadata
+  (bfps/cpp/kspace.hpp:43-195)
+* ``particles_system``    <- ``abstract_particles_system`` built by
+  ``particles_system_builder`` (bfps/cpp/particles/abstract_particles_system.hpp:12-55)
+* ``NSVEparticles_step``  <- ``NSVEparticles::step`` (bfps/cpp/full_code/NSVEparticles.cpp:33-41)
+
+Host arrays use the reference's layouts (complex ``[ny][nz][nx//2+1][3]``,
+real ``[nz][ny][nx][ncomp]``).  Everything runs on the GPU through
+libbfps_hip.so; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import capi
+
+__all__ = ["vorticity_equation", "kspace", "particles_system", "NSVEparticles_step", "capi"]
+
+
+class kspace:
+    """Fourier-space geometry of a context (bfps/cpp/kspace.cpp:34-167)."""
+
+    def __init__(self, ctx, nx, ny, nz, dkx, dky, dkz):
+        lib = capi.load()
+        info = np.zeros(4)
+        capi.check(ctx, lib.bfpsh_kspace_info(ctx, info.ctypes.data_as(capi._DP), None, None), "kspace_info")
+        self.kM, self.kM2, self.dk = float(info[0]), float(info[1]), float(info[2])
+        self.nshells = int(info[3])
+        self.kshell = np.zeros(self.nshells)
+        self.nshell = np.zeros(self.nshells, dtype=np.int64)
+        capi.check(ctx, lib.bfpsh_kspace_info(ctx, info.ctypes.data_as(capi._DP),
+                                              self.kshell.ctypes.data_as(capi._DP),
+                                              self.nshell.ctypes.data_as(capi._LP)), "kspace_info")
+        self.dkx, self.dky, self.dkz = dkx, dky, dkz
+        self.dk2 = self.dk ** 2
+        self.kx = np.arange(nx // 2 + 1) * dkx
+        self.ky = np.where(np.arange(ny) <= ny // 2, np.arange(ny), np.arange(ny) - ny) * dky
+        self.kz = np.where(np.arange(nz) <= nz // 2, np.arange(nz), np.arange(nz) - nz) * dkz
+
+
+class vorticity_equation:
+    """GPU-resident ``vorticity_equation<rnumber, HIP>``.
+
+    precision: 'single' (rnumber = float) or 'double'.
+    dealias:   'SMOOTH' (the reference solver's fixed choice) or 'TWO_THIRDS'.
+    """
+
+    def __init__(self, name, nx, ny=None, nz=None, dkx=1.0, dky=1.0, dkz=1.0,
+                 precision="single", dealias="SMOOTH", device=0):
+        ny = nx if ny is None else ny
+        nz = nx if nz is None else nz
+        self.name = name
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.precision = precision
+        self.rdtype = np.float32 if precision == "single" else np.float64
+        self.cdtype = np.complex64 if precision == "single" else np.complex128
+        lib = capi.load()
+        h = ctypes.c_void_p()
+        rc = lib.bfpsh_create(ctypes.byref(h), nx, ny, nz, dkx, dky, dkz,
+                              capi.FP32 if precision == "single" else capi.FP64,
+                              capi.SMOOTH if dealias == "SMOOTH" else capi.TWO_THIRDS, device)
+        self._h = h
+        if rc != 0:
+            msg = lib.bfpsh_last_error(h)
+            lib.bfpsh_destroy(h)
+            self._h = None
+            raise capi.BfpshError("bfpsh_create failed (%d): %s" % (rc, msg.decode() if msg else ""))
+        self.kk = kspace(h, nx, ny, nz, dkx, dky, dkz)
+        # physical parameters, initialised as the reference constructor does
+        # (bfps/cpp/vorticity_equation.cpp:149-155)
+        self.nu = 0.1
+        self.fmode = 1
+        self.famplitude = 1.0
+        self.fk0 = 2.0
+        self.fk1 = 4.0
+        self.forcing_type = "linear"
+        self.particles = None
+
+    # ---------------------------------------------------------------- basics
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            capi.load().bfpsh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _push_physics(self):
+        rc = capi.load().bfpsh_set_physics(self._h, float(self.nu), capi.FORCING[self.forcing_type],
+                                           float(self.famplitude), float(self.fk0), float(self.fk1),
+                                           int(self.fmode))
+        capi.check(self._h, rc, "set_physics")
+
+    @property
+    def iteration(self):
+        it = ctypes.c_int()
+        capi.check(self._h, capi.load().bfpsh_get_iteration(self._h, ctypes.byref(it)), "get_iteration")
+        return it.value
+
+    @iteration.setter
+    def iteration(self, v):
+        capi.check(self._h, capi.load().bfpsh_set_iteration(self._h, int(v)), "set_iteration")
+
+    def cshape(self, nc=3):
+        return (self.ny, self.nz, self.nx // 2 + 1, nc)
+
+    # ---------------------------------------------------------------- fields
+    def get_cfield(self, which=capi.CVORTICITY):
+        out = np.zeros(self.cshape(), dtype=self.cdtype)
+        capi.check(self._h, capi.load().bfpsh_get_cfield(self._h, which, out.ctypes.data), "get_cfield")
+        return out
+
+    @property
+    def cvorticity(self):
+        return self.get_cfield(capi.CVORTICITY)
+
+    @cvorticity.setter
+    def cvorticity(self, data):
+        a = np.ascontiguousarray(np.asarray(data).reshape(self.cshape()), dtype=self.cdtype)
+        capi.check(self._h, capi.load().bfpsh_put_cfield(self._h, capi.CVORTICITY, a.ctypes.data), "put_cfield")
+
+    def io_checkpoint(self, read=True, data=None):
+        """read=True: load `data` as cvorticity and apply the reference's read
+        projections (symmetrize, low_pass(kM), force_divfree;
+        bfps/cpp/vorticity_equation.hpp:107-128).  read=False: return the
+        current cvorticity as it would be written."""
+        if read:
+            self.cvorticity = data
+            capi.check(self._h, capi.load().bfpsh_load_projection(self._h), "load_projection")
+            return None
+        return self.cvorticity
+
+    def compute_velocity(self):
+        """cvelocity = Biot-Savart(cvorticity) + symmetrize (spectral)."""
+        return self.get_cfield(capi.CVELOCITY)
+
+    def velocity_rspace(self):
+        capi.check(self._h, capi.load().bfpsh_velocity_rspace(self._h), "velocity_rspace")
+
+    def gradient_rspace(self):
+        capi.check(self._h, capi.load().bfpsh_gradient_rspace(self._h), "gradient_rspace")
+
+    def get_rfield(self, which=capi.RVELOCITY):
+        nc = 3 if which == capi.RVELOCITY else 9
+        out = np.zeros((self.nz, self.ny, self.nx, nc), dtype=self.rdtype)
+        capi.check(self._h, capi.load().bfpsh_get_rfield(self._h, which, out.ctypes.data), "get_rfield")
+        return out
+
+    def cospectrum(self, a="velocity", b="velocity"):
+        sel = {"vorticity": capi.CVORTICITY, "velocity": capi.CVELOCITY}
+        out = np.zeros((self.kk.nshells, 3, 3))
+        capi.check(self._h, capi.load().bfpsh_cospectrum(self._h, sel[a], sel[b], out.ctypes.data_as(capi._DP)),
+                   "cospectrum")
+        return out
+
+    # ---------------------------------------------------------------- stepping
+    def step(self, dt):
+        self._push_physics()
+        capi.check(self._h, capi.load().bfpsh_step(self._h, float(dt)), "step")
+
+    def synchronize(self):
+        capi.check(self._h, capi.load().bfpsh_synchronize(self._h), "synchronize")
+
+    def time_steps(self, nsteps, dt, with_particles=False):
+        self._push_physics()
+        ms = ctypes.c_double()
+        capi.check(self._h, capi.load().bfpsh_time_steps(self._h, int(nsteps), float(dt), int(bool(with_particles)),
+                                                         ctypes.byref(ms)), "time_steps")
+        return ms.value
+
+    def profile(self, enable=True):
+        capi.check(self._h, capi.load().bfpsh_profile_enable(self._h, int(enable)), "profile_enable")
+
+    def profile_report(self):
+        out = {}
+        lib = capi.load()
+        i = 0
+        buf = ctypes.create_string_buffer(128)
+        while True:
+            ms = ctypes.c_double()
+            cnt = ctypes.c_int()
+            if lib.bfpsh_profile_get(self._h, i, buf, 128, ctypes.byref(ms), ctypes.byref(cnt)) != 0:
+                break
+            out[buf.value.decode()] = (ms.value, cnt.value)
+            i += 1
+        return out
+
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        capi.check(self._h, capi.load().bfpsh_device_bytes(self._h, ctypes.byref(b)), "device_bytes")
+        return b.value
+
+
+class particles_system:
+    """Tracers of one context (particles_system on one rank).
+
+    neighbours n in [1, 10], smoothness m in [0, 2]
+    (particles_generic_interp<double, n, m>), integration_steps = AB order
+    (1..6), step_idx as passed by NSVEparticles (iteration + 1)."""
+
+    def __init__(self, fs, nparticles, neighbours, smoothness, integration_steps=4, step_idx=1):
+        self.fs = fs
+        self.np = int(nparticles)
+        self.nsteps = int(integration_steps)
+        rc = capi.load().bfpsh_particles_create(fs.handle, self.np, int(neighbours), int(smoothness),
+                                                self.nsteps, int(step_idx))
+        capi.check(fs.handle, rc, "particles_create")
+        fs.particles = self
+
+    def put(self, pos, rhs=None, idx=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.np, 3)
+        r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64).reshape(self.nsteps, self.np, 3)
+        i = None if idx is None else np.ascontiguousarray(idx, dtype=np.int64)
+        rc = capi.load().bfpsh_particles_put(self.fs.handle, pos.ctypes.data_as(capi._DP),
+                                             None if r is None else r.ctypes.data_as(capi._DP),
+                                             None if i is None else i.ctypes.data_as(capi._LP))
+        capi.check(self.fs.handle, rc, "particles_put")
+
+    def get(self):
+        pos = np.zeros((self.np, 3))
+        rhs = np.zeros((self.nsteps, self.np, 3))
+        idx = np.zeros(self.np, dtype=np.int64)
+        si = ctypes.c_int()
+        rc = capi.load().bfpsh_particles_get(self.fs.handle, pos.ctypes.data_as(capi._DP),
+                                             rhs.ctypes.data_as(capi._DP), idx.ctypes.data_as(capi._LP),
+                                             ctypes.byref(si))
+        capi.check(self.fs.handle, rc, "particles_get")
+        return pos, rhs, idx, si.value
+
+    def getParticlesPositions(self):
+        return self.get()[0]
+
+    def getParticlesRhs(self):
+        return self.get()[1]
+
+    def getParticlesIndexes(self):
+        return self.get()[2]
+
+    def get_step_idx(self):
+        return self.get()[3]
+
+    def completeLoop(self, dt):
+        capi.check(self.fs.handle, capi.load().bfpsh_particles_complete_loop(self.fs.handle, float(dt)),
+                   "particles_complete_loop")
+
+    def sample(self, which="velocity"):
+        sel = capi.RVELOCITY if which == "velocity" else capi.RGRADIENT
+        nc = 3 if which == "velocity" else 9
+        out = np.zeros((self.np, nc))
+        rc = capi.load().bfpsh_particles_sample(self.fs.handle, sel, out.ctypes.data_as(capi._DP))
+        capi.check(self.fs.handle, rc, "particles_sample")
+        return out
+
+
+def NSVEparticles_step(fs, dt):
+    """NSVEparticles::step: tracers advance with u(t), then the fluid step."""
+    fs._push_physics()
+    capi.check(fs.handle, capi.load().bfpsh_nsve_particles_step(fs.handle, float(dt)), "nsve_particles_step")

@@ -1,0 +1,87 @@
+"""Build libbfps_hip.so (gfx950) in-tree with hipcc.
+
+The launch sequences are instantiated once per (precision, N) pair from
+csrc/impl_inst.hip; the objects compile in parallel and link with the C ABI
+unit csrc/bfpsh.hip into bfps_amd/libbfps_hip.so.
+
+    python -m bfps_amd.build [-j JOBS] [--sizes 32,64,...]
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libbfps_hip.so")
+SIZES = (16, 32, 48, 64, 128, 256, 512, 1024)
+ARCH = os.environ.get("BFPSH_ARCH", "gfx950")
+# packed FP32 instructions are disabled on purpose (see DESIGN.md notes on gfx950)
+FLAGS = ["-O3", "-std=c++17", "--offload-arch=" + ARCH, "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _hipcc():
+    for p in ("/opt/rocm/bin/hipcc", "hipcc"):
+        if os.path.sep not in p or os.path.exists(p):
+            return p
+    return "hipcc"
+
+
+def _compile(args):
+    src, obj, defs = args
+    cmd = [_hipcc()] + FLAGS + defs + ["-c", src, "-o", obj]
+    t0 = time.time()
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    errs = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    return obj, r.returncode, errs, time.time() - t0
+
+
+def sources():
+    out = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC))
+           if f.endswith((".hip", ".hpp", ".h"))]
+    return out
+
+
+def build(jobs=None, sizes=SIZES, verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    tasks = [(os.path.join(CSRC, "bfpsh.hip"), os.path.join(BUILD, "bfpsh.o"), [])]
+    for tn, t in (("f32", "float"), ("f64", "double")):
+        for n in sizes:
+            tasks.append((os.path.join(CSRC, "impl_inst.hip"),
+                          os.path.join(BUILD, "impl_%s_%d.o" % (tn, n)),
+                          ["-DBFPSH_T=%s" % t, "-DBFPSH_TN=%s" % tn, "-DBFPSH_N=%d" % n]))
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    t0 = time.time()
+    failed = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for obj, rc, errs, dt in ex.map(_compile, tasks):
+            if verbose:
+                print("  %-28s %5.1fs%s" % (os.path.basename(obj), dt, "" if rc == 0 else "  FAILED"))
+            if rc != 0:
+                failed.append((obj, errs))
+            elif errs and verbose:
+                print(errs)
+    if failed:
+        for obj, errs in failed:
+            print("---", obj, "\n", errs, file=sys.stderr)
+        raise RuntimeError("HIP compilation failed")
+    tmp = LIB + ".tmp"
+    objs = [t[1] for t in tasks]
+    subprocess.check_call([_hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC"] + objs + ["-o", tmp])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built %s in %.1fs" % (LIB, time.time() - t0))
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", "--jobs", type=int, default=None)
+    ap.add_argument("--sizes", default=None)
+    a = ap.parse_args()
+    sz = tuple(int(s) for s in a.sizes.split(",")) if a.sizes else SIZES
+    build(a.jobs, sz)

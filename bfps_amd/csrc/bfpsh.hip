@@ -1,0 +1,488 @@
+// C ABI implementation of the MI355X backend (see include/bfpsh.h).
+#include "ctx.hpp"
+#include "interp_tables.h"
+
+namespace {
+
+const OpsTable* ops_for(const bfpsh_ctx* ctx) {
+    switch (ctx->n) {
+#define X(N) case N: return ctx->prec == BFPSH_FP32 ? bfpsh_ops_f32_##N() : bfpsh_ops_f64_##N();
+        BFPSH_FOR_SIZES(X)
+#undef X
+    }
+    return nullptr;
+}
+
+bool supported_size(int n) {
+    switch (n) {
+#define X(N) case N: return true;
+        BFPSH_FOR_SIZES(X)
+#undef X
+    }
+    return false;
+}
+
+template <typename T>
+void build_twiddles(std::vector<unsigned char>& out, int n) {
+    out.resize(size_t(n) * 2 * sizeof(T));
+    T* p = reinterpret_cast<T*>(out.data());
+    for (int t = 0; t < n; t++) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * t / n;
+        p[2 * t] = T(std::cos(a));
+        p[2 * t + 1] = T(std::sin(a));
+    }
+}
+
+}  // namespace
+
+// host [ky][kz][kx=N/2+1][3]{re,im} <-> device planar [c][ky][kz][kx<N/2]
+template <typename T>
+static void host_to_planar(const T* h, T* p, int n) {
+    const int nxh = n / 2 + 1, nxs = n / 2;
+    const size_t ps = size_t(n) * n * nxs;
+    for (size_t yz = 0; yz < size_t(n) * n; yz++)
+        for (int x = 0; x < nxs; x++)
+            for (int c = 0; c < 3; c++) {
+                p[(c * ps + yz * nxs + x) * 2 + 0] = h[((yz * nxh + x) * 3 + c) * 2 + 0];
+                p[(c * ps + yz * nxs + x) * 2 + 1] = h[((yz * nxh + x) * 3 + c) * 2 + 1];
+            }
+}
+template <typename T>
+static void planar_to_host(const T* p, T* h, int n) {
+    const int nxh = n / 2 + 1, nxs = n / 2;
+    const size_t ps = size_t(n) * n * nxs;
+    for (size_t yz = 0; yz < size_t(n) * n; yz++)
+        for (int x = 0; x < nxh; x++)
+            for (int c = 0; c < 3; c++) {
+                T re = 0, im = 0;
+                if (x < nxs) {
+                    re = p[(c * ps + yz * nxs + x) * 2 + 0];
+                    im = p[(c * ps + yz * nxs + x) * 2 + 1];
+                }
+                h[((yz * nxh + x) * 3 + c) * 2 + 0] = re;
+                h[((yz * nxh + x) * 3 + c) * 2 + 1] = im;
+            }
+}
+
+#define CTX_CHECK()                                  \
+    do {                                             \
+        if (!ctx) return BFPSH_EINVAL;               \
+        hipError_t e_ = hipSetDevice(ctx->device);   \
+        if (e_ != hipSuccess) return fail(ctx, BFPSH_EHIP, hipGetErrorString(e_)); \
+    } while (0)
+
+#define PREC_CALL(fn, ...) (ops_for(ctx)->fn(__VA_ARGS__))
+
+extern "C" {
+
+int bfpsh_create(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, double dkz, int precision,
+                 int dealias, int device) {
+    if (!out) return BFPSH_EINVAL;
+    *out = nullptr;
+    bfpsh_ctx* ctx = new bfpsh_ctx();
+    if (nx != ny || ny != nz || !supported_size(nx)) {
+        ctx->err = "unsupported grid (cubic N in {16,32,48,64,128,256,512,1024} required)";
+        *out = ctx;
+        return BFPSH_EINVAL;
+    }
+    if (precision != BFPSH_FP32 && precision != BFPSH_FP64) { *out = ctx; return fail(ctx, BFPSH_EINVAL, "bad precision"); }
+    if (dealias != BFPSH_SMOOTH && dealias != BFPSH_TWO_THIRDS) { *out = ctx; return fail(ctx, BFPSH_EINVAL, "bad dealias type"); }
+    if (!(dkx > 0 && dky > 0 && dkz > 0)) { *out = ctx; return fail(ctx, BFPSH_EINVAL, "dk must be positive"); }
+    *out = ctx;
+    ctx->n = nx;
+    ctx->prec = precision;
+    ctx->dealias = dealias;
+    ctx->device = device;
+    ctx->dkx = dkx;
+    ctx->dky = dky;
+    ctx->dkz = dkz;
+    ctx->es = precision == BFPSH_FP32 ? 4 : 8;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // kspace geometry (bfps/cpp/kspace.cpp:56-166), cubic grid
+    const int n = nx, nxh = n / 2 + 1;
+    double kMx, kMy, kMz;
+    if (dealias == BFPSH_TWO_THIRDS) {
+        kMx = dkx * (int(2 * (nxh - 1) / 3) - 1);
+        kMy = dky * (int(n / 3) - 1);
+        kMz = dkz * (int(n / 3) - 1);
+    } else {
+        kMx = dkx * (nxh - 2);
+        kMy = dky * (int(n / 2) - 1);
+        kMz = dkz * (int(n / 2) - 1);
+    }
+    ctx->kM = std::max(kMx, std::max(kMy, kMz));
+    ctx->kM2 = ctx->kM * ctx->kM;
+    ctx->dk = std::min(dkx, std::min(dky, dkz));
+    ctx->nshells = int(ctx->kM / ctx->dk) + 2;
+    ctx->kshell.assign(ctx->nshells, 0.0);
+    ctx->nshell.assign(ctx->nshells, 0);
+    for (int y = 0; y < n; y++)
+        for (int z = 0; z < n; z++)
+            for (int x = 0; x < nxh; x++) {
+                const double kx = x * dkx;
+                const double ky = (y <= n / 2 ? y : y - n) * dky;
+                const double kz = (z <= n / 2 ? z : z - n) * dkz;
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                if (k2 < ctx->kM2) {
+                    const double kn = std::sqrt(k2);
+                    ctx->kshell[int(kn / ctx->dk)] += (x == 0 ? 1 : 2) * kn;
+                    ctx->nshell[int(kn / ctx->dk)] += (x == 0 ? 1 : 2);
+                }
+            }
+    for (int s = 0; s < ctx->nshells; s++)
+        if (ctx->nshell[s]) ctx->kshell[s] /= ctx->nshell[s];
+    // device state
+    ctx->ps = size_t(n) * n * (n / 2);
+    const size_t vbytes = 3 * ctx->ps * 2 * ctx->es;
+    for (void** p : {&ctx->v0, &ctx->v1, &ctx->v2}) {
+        HIPCHK(hipMalloc(p, vbytes));
+        HIPCHK(hipMemsetAsync(*p, 0, vbytes, ctx->stream));
+        ctx->bytes += vbytes;
+    }
+    std::vector<unsigned char> tw;
+    if (precision == BFPSH_FP32) build_twiddles<float>(tw, n); else build_twiddles<double>(tw, n);
+    HIPCHK(hipMalloc(&ctx->tw, tw.size()));
+    HIPCHK(hipMemcpy(ctx->tw, tw.data(), tw.size(), hipMemcpyHostToDevice));
+    ctx->bytes += tw.size();
+    int r = ensure_work(ctx, 6);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BFPSH_OK;
+}
+
+int bfpsh_destroy(bfpsh_ctx* ctx) {
+    if (!ctx) return BFPSH_OK;
+    if (ctx->stream) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (void* p : {ctx->v0, ctx->v1, ctx->v2, ctx->work, ctx->ru, ctx->rgrad, ctx->tw,
+                        (void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab})
+            if (p) (void)hipFree(p);
+        resolve_profile(ctx);
+        for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return BFPSH_OK;
+}
+
+const char* bfpsh_last_error(const bfpsh_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int bfpsh_set_physics(bfpsh_ctx* ctx, double nu, int forcing, double famplitude, double fk0, double fk1, int fmode) {
+    if (!ctx) return BFPSH_EINVAL;
+    if (forcing < 0 || forcing > 2) return fail(ctx, BFPSH_EINVAL, "bad forcing type");
+    if (forcing == BFPSH_FORCING_KOLMOGOROV && (fmode < 0 || fmode >= ctx->n))
+        return fail(ctx, BFPSH_EINVAL, "fmode out of range");
+    ctx->sp.nu = nu;
+    ctx->sp.forcing = forcing;
+    ctx->sp.famp = famplitude;
+    ctx->sp.fk0 = fk0;
+    ctx->sp.fk1 = fk1;
+    ctx->sp.fmode = fmode;
+    return BFPSH_OK;
+}
+
+int bfpsh_kspace_info(bfpsh_ctx* ctx, double* out4, double* kshell, int64_t* nshell) {
+    if (!ctx || !out4) return BFPSH_EINVAL;
+    out4[0] = ctx->kM;
+    out4[1] = ctx->kM2;
+    out4[2] = ctx->dk;
+    out4[3] = ctx->nshells;
+    if (kshell) std::copy(ctx->kshell.begin(), ctx->kshell.end(), kshell);
+    if (nshell) std::copy(ctx->nshell.begin(), ctx->nshell.end(), nshell);
+    return BFPSH_OK;
+}
+
+int bfpsh_put_cfield(bfpsh_ctx* ctx, int which, const void* host) {
+    CTX_CHECK();
+    if (!host || which != BFPSH_CVORTICITY) return fail(ctx, BFPSH_EINVAL, "only the vorticity can be put");
+    const size_t bytes = 3 * ctx->ps * 2 * ctx->es;
+    std::vector<unsigned char> tmp(bytes);
+    if (ctx->prec == BFPSH_FP32) host_to_planar<float>((const float*)host, (float*)tmp.data(), ctx->n);
+    else host_to_planar<double>((const double*)host, (double*)tmp.data(), ctx->n);
+    HIPCHK(hipMemcpyAsync(ctx->v0, tmp.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->ru_valid = false;
+    return BFPSH_OK;
+}
+
+int bfpsh_get_cfield(bfpsh_ctx* ctx, int which, void* host) {
+    CTX_CHECK();
+    if (!host) return fail(ctx, BFPSH_EINVAL, "null buffer");
+    const void* src;
+    if (which == BFPSH_CVORTICITY) {
+        src = ctx->v0;
+    } else if (which == BFPSH_CVELOCITY) {
+        int r = PREC_CALL(spectral_velocity, ctx);
+        if (r) return r;
+        src = ctx->work;
+    } else {
+        return fail(ctx, BFPSH_EINVAL, "bad field selector");
+    }
+    const size_t bytes = 3 * ctx->ps * 2 * ctx->es;
+    std::vector<unsigned char> tmp(bytes);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->prec == BFPSH_FP32) planar_to_host<float>((const float*)tmp.data(), (float*)host, ctx->n);
+    else planar_to_host<double>((const double*)tmp.data(), (double*)host, ctx->n);
+    return BFPSH_OK;
+}
+
+int bfpsh_get_rfield(bfpsh_ctx* ctx, int which, void* host) {
+    CTX_CHECK();
+    if (!host) return fail(ctx, BFPSH_EINVAL, "null buffer");
+    const void* src = which == BFPSH_RVELOCITY ? ctx->ru : (which == BFPSH_RGRADIENT ? ctx->rgrad : nullptr);
+    const int nc = which == BFPSH_RVELOCITY ? 3 : 9;
+    if (!src) return fail(ctx, BFPSH_ESTATE, "real field not computed");
+    const size_t n = ctx->n, n3 = n * n * n;
+    std::vector<unsigned char> tmp(nc * n3 * ctx->es);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // device [c][y][z][x] -> host [z][y][x][c]
+    for (size_t y = 0; y < n; y++)
+        for (size_t z = 0; z < n; z++)
+            for (size_t x = 0; x < n; x++)
+                for (int c = 0; c < nc; c++) {
+                    const size_t di = c * n3 + (y * n + z) * n + x;
+                    const size_t hi = ((z * n + y) * n + x) * nc + c;
+                    if (ctx->prec == BFPSH_FP32) ((float*)host)[hi] = ((const float*)tmp.data())[di];
+                    else ((double*)host)[hi] = ((const double*)tmp.data())[di];
+                }
+    return BFPSH_OK;
+}
+
+int bfpsh_load_projection(bfpsh_ctx* ctx) {
+    CTX_CHECK();
+    return PREC_CALL(load_projection, ctx);
+}
+
+int bfpsh_step(bfpsh_ctx* ctx, double dt) {
+    CTX_CHECK();
+    return PREC_CALL(step, ctx, dt, false);
+}
+
+int bfpsh_velocity_rspace(bfpsh_ctx* ctx) {
+    CTX_CHECK();
+    return PREC_CALL(velocity_rspace, ctx);
+}
+
+int bfpsh_gradient_rspace(bfpsh_ctx* ctx) {
+    CTX_CHECK();
+    return PREC_CALL(gradient_rspace, ctx);
+}
+
+int bfpsh_cospectrum(bfpsh_ctx* ctx, int a, int b, double* out) {
+    CTX_CHECK();
+    if (!out) return fail(ctx, BFPSH_EINVAL, "null output");
+    // velocity needs its own buffer when both operands are used
+    const bool need_u = a == BFPSH_CVELOCITY || b == BFPSH_CVELOCITY;
+    if ((a != BFPSH_CVORTICITY && a != BFPSH_CVELOCITY) || (b != BFPSH_CVORTICITY && b != BFPSH_CVELOCITY))
+        return fail(ctx, BFPSH_EINVAL, "bad field selector");
+    if (need_u) {
+        int r = PREC_CALL(spectral_velocity, ctx);
+        if (r) return r;
+    }
+    const void* pa = a == BFPSH_CVORTICITY ? ctx->v0 : ctx->work;
+    const void* pb = b == BFPSH_CVORTICITY ? ctx->v0 : ctx->work;
+    return PREC_CALL(cospectrum, ctx, pa, pb, out);
+}
+
+int bfpsh_particles_create(bfpsh_ctx* ctx, int64_t np, int n, int m, int nsteps, int step_idx) {
+    CTX_CHECK();
+    if (np < 0 || n < 1 || n > BFPS_INTERP_NMAX || m < 0 || m > BFPS_INTERP_MMAX || nsteps < 1 || nsteps > 6)
+        return fail(ctx, BFPSH_EINVAL, "bad tracer parameters (n in [1,10], m in [0,2], nsteps in [1,6])");
+    if (2 * n + 2 > ctx->n) return fail(ctx, BFPSH_EINVAL, "interpolation stencil wider than the grid");
+    for (void* p : {(void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab})
+        if (p) HIPCHK(hipFree(p));
+    ctx->d_pos = ctx->d_rhs = ctx->d_tab = nullptr;
+    ctx->np = np;
+    ctx->pn = n;
+    ctx->pm = m;
+    ctx->nsteps = nsteps;
+    ctx->step_idx = step_idx;
+    ctx->head = 0;
+    ctx->head_zero = false;
+    ctx->add_head = false;
+    const int k = (n - 1) * (BFPS_INTERP_MMAX + 1) + m;
+    ctx->deg = bfps_interp_degree[k];
+    const size_t tabn = size_t(3) * (2 * n + 2) * (ctx->deg + 1);
+    HIPCHK(hipMalloc((void**)&ctx->d_tab, tabn * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->d_tab, bfps_interp_coef + bfps_interp_offset[k], tabn * sizeof(double), hipMemcpyHostToDevice));
+    if (np > 0) {
+        HIPCHK(hipMalloc((void**)&ctx->d_pos, np * 3 * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&ctx->d_rhs, size_t(nsteps) * np * 3 * sizeof(double)));
+        HIPCHK(hipMemsetAsync(ctx->d_pos, 0, np * 3 * sizeof(double), ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->d_rhs, 0, size_t(nsteps) * np * 3 * sizeof(double), ctx->stream));
+    }
+    ctx->idx.resize(np);
+    std::iota(ctx->idx.begin(), ctx->idx.end(), int64_t(0));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BFPSH_OK;
+}
+
+int bfpsh_particles_put(bfpsh_ctx* ctx, const double* pos, const double* rhs, const int64_t* idx) {
+    CTX_CHECK();
+    if (!pos) return fail(ctx, BFPSH_EINVAL, "null positions");
+    if (ctx->np == 0) return BFPSH_OK;
+    const size_t pb = ctx->np * 3 * sizeof(double);
+    HIPCHK(hipMemcpyAsync(ctx->d_pos, pos, pb, hipMemcpyHostToDevice, ctx->stream));
+    ctx->head = 0;
+    ctx->add_head = false;
+    if (rhs) {
+        HIPCHK(hipMemcpyAsync(ctx->d_rhs, rhs, pb * ctx->nsteps, hipMemcpyHostToDevice, ctx->stream));
+        for (int64_t i = 0; i < ctx->np * 3; i++)
+            if (rhs[i] != 0.0) { ctx->add_head = true; break; }
+    } else {
+        HIPCHK(hipMemsetAsync(ctx->d_rhs, 0, pb * ctx->nsteps, ctx->stream));
+    }
+    ctx->head_zero = !ctx->add_head;
+    if (idx) std::copy(idx, idx + ctx->np, ctx->idx.begin());
+    else std::iota(ctx->idx.begin(), ctx->idx.end(), int64_t(0));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BFPSH_OK;
+}
+
+int bfpsh_particles_get(bfpsh_ctx* ctx, double* pos, double* rhs, int64_t* idx, int* step_idx) {
+    CTX_CHECK();
+    if (step_idx) *step_idx = ctx->step_idx;
+    if (ctx->np == 0) return BFPSH_OK;
+    const int64_t np = ctx->np;
+    std::vector<int64_t> order(np);
+    std::iota(order.begin(), order.end(), int64_t(0));
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ctx->idx[a] < ctx->idx[b]; });
+    std::vector<double> tmp(np * 3);
+    if (pos) {
+        HIPCHK(hipMemcpy(tmp.data(), ctx->d_pos, np * 3 * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < np; i++)
+            for (int d = 0; d < 3; d++) pos[i * 3 + d] = tmp[order[i] * 3 + d];
+    }
+    if (rhs) {
+        for (int h = 0; h < ctx->nsteps; h++) {
+            const int slot = (ctx->head + h) % ctx->nsteps;
+            if (h == 0 && ctx->head_zero) {
+                std::fill(rhs, rhs + np * 3, 0.0);
+                continue;
+            }
+            HIPCHK(hipMemcpy(tmp.data(), ctx->d_rhs + size_t(slot) * np * 3, np * 3 * sizeof(double), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < np; i++)
+                for (int d = 0; d < 3; d++) rhs[(size_t(h) * np + i) * 3 + d] = tmp[order[i] * 3 + d];
+        }
+    }
+    if (idx)
+        for (int64_t i = 0; i < np; i++) idx[i] = ctx->idx[order[i]];
+    return BFPSH_OK;
+}
+
+int bfpsh_particles_complete_loop(bfpsh_ctx* ctx, double dt) {
+    CTX_CHECK();
+    if (ctx->nsteps == 0) return fail(ctx, BFPSH_ESTATE, "no tracers");
+    if (!ctx->ru_valid) {
+        int r = PREC_CALL(velocity_rspace, ctx);
+        if (r) return r;
+    }
+    return PREC_CALL(particles_loop, ctx, dt);
+}
+
+int bfpsh_particles_sample(bfpsh_ctx* ctx, int which, double* out) {
+    CTX_CHECK();
+    if (!out) return fail(ctx, BFPSH_EINVAL, "null output");
+    if (which != BFPSH_RVELOCITY && which != BFPSH_RGRADIENT) return fail(ctx, BFPSH_EINVAL, "bad field selector");
+    if ((which == BFPSH_RVELOCITY && !ctx->ru) || (which == BFPSH_RGRADIENT && !ctx->rgrad))
+        return fail(ctx, BFPSH_ESTATE, "real field not computed");
+    if (ctx->np == 0) return BFPSH_OK;
+    const int nc = which == BFPSH_RVELOCITY ? 3 : 9;
+    double* d_out = nullptr;
+    HIPCHK(hipMalloc((void**)&d_out, ctx->np * nc * sizeof(double)));
+    int r = PREC_CALL(sample, ctx, which, d_out);
+    if (r) { (void)hipFree(d_out); return r; }
+    std::vector<double> tmp(ctx->np * nc);
+    HIPCHK(hipMemcpy(tmp.data(), d_out, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipFree(d_out));
+    std::vector<int64_t> order(ctx->np);
+    std::iota(order.begin(), order.end(), int64_t(0));
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ctx->idx[a] < ctx->idx[b]; });
+    for (int64_t i = 0; i < ctx->np; i++)
+        for (int c = 0; c < nc; c++) out[i * nc + c] = tmp[order[i] * nc + c];
+    return BFPSH_OK;
+}
+
+int bfpsh_nsve_particles_step(bfpsh_ctx* ctx, double dt) {
+    CTX_CHECK();
+    return PREC_CALL(step, ctx, dt, ctx->nsteps > 0 && ctx->np > 0);
+}
+
+int bfpsh_synchronize(bfpsh_ctx* ctx) {
+    CTX_CHECK();
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BFPSH_OK;
+}
+
+int bfpsh_get_iteration(bfpsh_ctx* ctx, int* it) {
+    if (!ctx || !it) return BFPSH_EINVAL;
+    *it = ctx->iteration;
+    return BFPSH_OK;
+}
+int bfpsh_set_iteration(bfpsh_ctx* ctx, int it) {
+    if (!ctx) return BFPSH_EINVAL;
+    ctx->iteration = it;
+    return BFPSH_OK;
+}
+
+int bfpsh_time_steps(bfpsh_ctx* ctx, int nsteps, double dt, int with_particles, double* ms) {
+    CTX_CHECK();
+    if (!ms || nsteps < 0) return fail(ctx, BFPSH_EINVAL, "bad arguments");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, ctx->stream));
+    const bool part = with_particles && ctx->nsteps > 0 && ctx->np > 0;
+    for (int i = 0; i < nsteps; i++) {
+        int r = PREC_CALL(step, ctx, dt, part);
+        if (r) return r;
+    }
+    HIPCHK(hipEventRecord(b, ctx->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    *ms = t;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return BFPSH_OK;
+}
+
+int bfpsh_profile_enable(bfpsh_ctx* ctx, int enable) {
+    if (!ctx) return BFPSH_EINVAL;
+    int r = resolve_profile(ctx);
+    if (r) return r;
+    ctx->prof = enable != 0;
+    if (enable) {
+        ctx->prof_data.clear();
+        ctx->prof_names.clear();
+    }
+    return BFPSH_OK;
+}
+
+int bfpsh_profile_get(bfpsh_ctx* ctx, int i, char* name, int namelen, double* total_ms, int* count) {
+    if (!ctx) return BFPSH_EINVAL;
+    int r = resolve_profile(ctx);
+    if (r) return r;
+    if (i < 0 || i >= int(ctx->prof_names.size())) return BFPSH_EINVAL;
+    const std::string& s = ctx->prof_names[i];
+    if (name && namelen > 0) {
+        std::strncpy(name, s.c_str(), namelen - 1);
+        name[namelen - 1] = 0;
+    }
+    const ProfEntry& pe = ctx->prof_data[s];
+    if (total_ms) *total_ms = pe.ms;
+    if (count) *count = pe.count;
+    return BFPSH_OK;
+}
+
+int bfpsh_device_bytes(bfpsh_ctx* ctx, size_t* bytes) {
+    if (!ctx || !bytes) return BFPSH_EINVAL;
+    *bytes = ctx->bytes;
+    return BFPSH_OK;
+}
+
+}  // extern "C"

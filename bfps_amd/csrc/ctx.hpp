@@ -1,0 +1,192 @@
+// Context state and launch helpers shared by the C ABI and the per-size
+// kernel instantiations.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/bfpsh.h"
+#include "kernels.hpp"
+
+using namespace bfpsh;
+
+struct ProfEntry {
+    double ms = 0;
+    int count = 0;
+};
+
+struct bfpsh_ctx {
+    int n = 0, prec = 0, dealias = 1, device = 0;
+    double dkx = 1, dky = 1, dkz = 1;
+    double kM = 0, kM2 = 0, dk = 1;
+    int nshells = 0;
+    std::vector<double> kshell;
+    std::vector<int64_t> nshell;
+    hipStream_t stream = nullptr;
+    // device buffers (element size es = sizeof(rnumber))
+    size_t es = 4;
+    size_t ps = 0;                 // complex elements per spectral plane
+    void* v0 = nullptr;            // cvorticity (3 planes)
+    void* v1 = nullptr;            // v[1]
+    void* v2 = nullptr;            // v[2]
+    void* work = nullptr;          // 6 spectral planes (9 when gradients are used)
+    int work_planes = 0;
+    void* ru = nullptr;            // real velocity, 3 planes [y][z][x]
+    void* rgrad = nullptr;         // real gradient, 9 planes
+    void* tw = nullptr;            // twiddles exp(-2 pi i t / N), N complex
+    bool ru_valid = false;
+    StepParams sp{0.1, 0.0, 1.0, 2.0, 4.0, 1, 1};
+    int iteration = 0;
+    std::string err;
+    // tracers
+    int64_t np = 0;
+    int pn = 0, pm = 0, nsteps = 0, step_idx = 0, head = 0, deg = 0;
+    bool head_zero = false, add_head = false;
+    double* d_pos = nullptr;
+    double* d_rhs = nullptr;
+    double* d_tab = nullptr;
+    std::vector<int64_t> idx;
+    // profiling
+    bool prof = false;
+    std::vector<std::string> prof_names;
+    std::map<std::string, ProfEntry> prof_data;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> ev_pool;
+    size_t bytes = 0;
+};
+
+inline int fail(bfpsh_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(ctx, BFPSH_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// launch wrapper with optional per-kernel event timing: events are recorded
+// around every launch on the context stream without synchronising; they are
+// resolved (and recycled) by resolve_profile().
+template <class F>
+inline int launch(bfpsh_ctx* ctx, const char* name, F&& f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->prof) {
+        if (ctx->ev_pool.size() < 2) {
+            for (int i = 0; i < 64; i++) {
+                hipEvent_t e;
+                HIPCHK(hipEventCreate(&e));
+                ctx->ev_pool.push_back(e);
+            }
+        }
+        a = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        b = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        HIPCHK(hipEventRecord(a, ctx->stream));
+    }
+    f();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, BFPSH_EHIP, std::string("launch ") + name + ": " + hipGetErrorString(e));
+    if (ctx->prof) {
+        HIPCHK(hipEventRecord(b, ctx->stream));
+        ctx->pending.push_back({name, a, b});
+    }
+    return BFPSH_OK;
+}
+
+inline int resolve_profile(bfpsh_ctx* ctx) {
+    if (ctx->pending.empty()) return BFPSH_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (auto& p : ctx->pending) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+        ProfEntry& pe = ctx->prof_data[p.name];
+        if (pe.count == 0) ctx->prof_names.push_back(p.name);
+        pe.ms += ms;
+        pe.count += 1;
+        ctx->ev_pool.push_back(p.a);
+        ctx->ev_pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+    return BFPSH_OK;
+}
+
+#define LAUNCH(name, ...)                                     \
+    do {                                                      \
+        int r_ = launch(ctx, name, [&]() { __VA_ARGS__; });   \
+        if (r_) return r_;                                    \
+    } while (0)
+
+inline Geom geom(const bfpsh_ctx* ctx) {
+    Geom g;
+    g.n = ctx->n;
+    g.dkx = ctx->dkx;
+    g.dky = ctx->dky;
+    g.dkz = ctx->dkz;
+    g.kM2 = ctx->kM2;
+    g.smooth = ctx->dealias == BFPSH_SMOOTH;
+    return g;
+}
+
+inline PartGeom pgeom(const bfpsh_ctx* ctx) {
+    PartGeom p;
+    p.n = ctx->n;
+    const double dkv[3] = {ctx->dkx, ctx->dky, ctx->dkz};
+    for (int d = 0; d < 3; d++) {
+        // particles_system_builder.hpp:175-189
+        p.width[d] = 4 * std::acos(0.0) / dkv[d];
+        p.h[d] = p.width[d] / double(ctx->n);
+    }
+    return p;
+}
+
+inline int ensure_work(bfpsh_ctx* ctx, int planes) {
+    if (ctx->work_planes >= planes) return BFPSH_OK;
+    if (ctx->work) {
+        HIPCHK(hipFree(ctx->work));
+        ctx->bytes -= ctx->work_planes * ctx->ps * 2 * ctx->es;
+    }
+    HIPCHK(hipMalloc(&ctx->work, planes * ctx->ps * 2 * ctx->es));
+    ctx->bytes += planes * ctx->ps * 2 * ctx->es;
+    ctx->work_planes = planes;
+    return BFPSH_OK;
+}
+
+inline int ensure_real(bfpsh_ctx* ctx, void** buf, int planes) {
+    if (*buf) return BFPSH_OK;
+    const size_t sz = size_t(planes) * ctx->n * ctx->n * ctx->n * ctx->es;
+    HIPCHK(hipMalloc(buf, sz));
+    HIPCHK(hipMemsetAsync(*buf, 0, sz, ctx->stream));
+    ctx->bytes += sz;
+    return BFPSH_OK;
+}
+
+
+// per-(precision, N) entry points, filled by impl_inst.hip
+struct OpsTable {
+    int (*step)(bfpsh_ctx*, double, bool);
+    int (*load_projection)(bfpsh_ctx*);
+    int (*velocity_rspace)(bfpsh_ctx*);
+    int (*gradient_rspace)(bfpsh_ctx*);
+    int (*spectral_velocity)(bfpsh_ctx*);
+    int (*cospectrum)(bfpsh_ctx*, const void*, const void*, double*);
+    int (*particles_loop)(bfpsh_ctx*, double);
+    int (*sample)(bfpsh_ctx*, int, double*);
+};
+
+#define BFPSH_FOR_SIZES(X) X(16) X(32) X(48) X(64) X(128) X(256) X(512) X(1024)
+#define BFPSH_DECL_OPS(N) const OpsTable* bfpsh_ops_f32_##N(); const OpsTable* bfpsh_ops_f64_##N();
+BFPSH_FOR_SIZES(BFPSH_DECL_OPS)
+#undef BFPSH_DECL_OPS

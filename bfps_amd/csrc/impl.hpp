@@ -1,0 +1,190 @@
+// Launch sequences of the vorticity step, transforms and tracer loop for one
+// (precision, N) pair.  Instantiated once per pair by impl_inst.hip.
+#pragma once
+#include "ctx.hpp"
+
+// ------------------------------------------------------------------ per-(T, N) launchers
+template <typename T, int N>
+struct Impl {
+    using CF = Cfg<T, N>;
+    static constexpr int B = CF::B, NT = CF::NT, NXH = CF::NXH;
+    typedef C2<T> CT;
+
+    static int stage(bfpsh_ctx* ctx, int s, double dt, T* ru) {
+        const Geom g = geom(ctx);
+        StepParams sp = ctx->sp;
+        sp.dt = dt;
+        CT* v0 = (CT*)ctx->v0;
+        CT* v1 = (CT*)ctx->v1;
+        CT* v2 = (CT*)ctx->v2;
+        CT* w = (CT*)ctx->work;
+        const CT* tw = (const CT*)ctx->tw;
+        const CT* src = s == 1 ? v0 : (s == 2 ? v1 : v2);
+        const dim3 tiles(NXH / B, N);
+        LAUNCH("inv_z_bs", (k_inv_z_bs<T, N, 0><<<tiles, NT, 0, ctx->stream>>>(src, w, tw, g)));
+        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 6), NT, 0, ctx->stream>>>(
+                             w, size_t(N) * NXH, size_t(NXH), tw)));
+        LAUNCH("rows_cross", (k_rows_cross<T, N><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(
+                                  w, ru, tw, T(double(N) * N * N))));
+        LAUNCH("fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, 3), NT, 0, ctx->stream>>>(
+                             w, size_t(NXH), size_t(N) * NXH, tw)));
+        if (s == 1)
+            LAUNCH("fwd_y_update1", (k_fwd_y_update<T, N, 1><<<tiles, NT, 0, ctx->stream>>>(w, v0, v1, v1, tw, g, sp)));
+        else if (s == 2)
+            LAUNCH("fwd_y_update2", (k_fwd_y_update<T, N, 2><<<tiles, NT, 0, ctx->stream>>>(w, v0, v1, v2, tw, g, sp)));
+        else
+            LAUNCH("fwd_y_update3", (k_fwd_y_update<T, N, 3><<<tiles, NT, 0, ctx->stream>>>(w, v0, v2, v0, tw, g, sp)));
+        return BFPSH_OK;
+    }
+
+    static int step(bfpsh_ctx* ctx, double dt, bool particles) {
+        int r = ensure_work(ctx, 6);
+        if (r) return r;
+        T* ru = nullptr;
+        if (particles) {
+            r = ensure_real(ctx, &ctx->ru, 3);
+            if (r) return r;
+            ru = (T*)ctx->ru;
+        }
+        r = stage(ctx, 1, dt, ru);
+        if (r) return r;
+        if (particles) {
+            r = particles_loop(ctx, dt);
+            if (r) return r;
+        }
+        r = stage(ctx, 2, dt, nullptr);
+        if (r) return r;
+        r = stage(ctx, 3, dt, nullptr);
+        if (r) return r;
+        r = symmetrize(ctx, ctx->v0, 3);
+        if (r) return r;
+        ctx->ru_valid = false;
+        ctx->iteration++;
+        return BFPSH_OK;
+    }
+
+    static int symmetrize(bfpsh_ctx* ctx, void* v, int nc) {
+        const int nthreads = 256;
+        const int nblocks = (N * N + nthreads - 1) / nthreads;
+        LAUNCH("symmetrize", (k_symmetrize<T><<<nblocks, nthreads, 0, ctx->stream>>>((CT*)v, N, nc)));
+        return BFPSH_OK;
+    }
+
+    static int load_projection(bfpsh_ctx* ctx) {
+        int r = symmetrize(ctx, ctx->v0, 3);
+        if (r) return r;
+        const size_t ps = size_t(N) * N * NXH;
+        LAUNCH("lowpass_divfree", (k_lowpass_divfree<T><<<(ps + 255) / 256, 256, 0, ctx->stream>>>((CT*)ctx->v0, geom(ctx))));
+        ctx->ru_valid = false;
+        return BFPSH_OK;
+    }
+
+    static int velocity_rspace(bfpsh_ctx* ctx) {
+        int r = ensure_work(ctx, 6);
+        if (r) return r;
+        r = ensure_real(ctx, &ctx->ru, 3);
+        if (r) return r;
+        CT* w = (CT*)ctx->work;
+        const CT* tw = (const CT*)ctx->tw;
+        LAUNCH("inv_z_bs_u", (k_inv_z_bs<T, N, 1><<<dim3(NXH / B, N), NT, 0, ctx->stream>>>((const CT*)ctx->v0, w, tw, geom(ctx))));
+        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 3), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw)));
+        LAUNCH("rows_c2r3", (k_rows_c2r<T, N, 3><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->ru, tw)));
+        ctx->ru_valid = true;
+        return BFPSH_OK;
+    }
+
+    static int gradient_rspace(bfpsh_ctx* ctx) {
+        int r = ensure_work(ctx, 9);
+        if (r) return r;
+        r = ensure_real(ctx, &ctx->rgrad, 9);
+        if (r) return r;
+        CT* w = (CT*)ctx->work;
+        const CT* tw = (const CT*)ctx->tw;
+        LAUNCH("inv_z_bs_grad", (k_inv_z_bs<T, N, 2><<<dim3(NXH / B, N), NT, 0, ctx->stream>>>((const CT*)ctx->v0, w, tw, geom(ctx))));
+        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 9), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw)));
+        LAUNCH("rows_c2r9", (k_rows_c2r<T, N, 9><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->rgrad, tw)));
+        return BFPSH_OK;
+    }
+
+    // cvelocity = symmetrize(BS(cvorticity)) into work planes 0..2
+    static int spectral_velocity(bfpsh_ctx* ctx) {
+        int r = ensure_work(ctx, 6);
+        if (r) return r;
+        const size_t ps = size_t(N) * N * NXH;
+        LAUNCH("biot_savart", (k_biot_savart<T><<<(ps + 255) / 256, 256, 0, ctx->stream>>>((const CT*)ctx->v0, (CT*)ctx->work, geom(ctx))));
+        return symmetrize(ctx, ctx->work, 3);
+    }
+
+    static int cospectrum(bfpsh_ctx* ctx, const void* a, const void* b, double* out) {
+        double* d_out = nullptr;
+        HIPCHK(hipMallocAsync((void**)&d_out, ctx->nshells * 9 * sizeof(double), ctx->stream));
+        HIPCHK(hipMemsetAsync(d_out, 0, ctx->nshells * 9 * sizeof(double), ctx->stream));
+        const size_t lds = ctx->nshells * 9 * sizeof(double);
+        if (lds > 64 * 1024) return fail(ctx, BFPSH_EINVAL, "too many shells for the cospectrum kernel");
+        LAUNCH("cospectrum", (k_cospectrum<T><<<1024, 256, lds, ctx->stream>>>((const CT*)a, (const CT*)b, d_out, geom(ctx), ctx->dk, ctx->nshells)));
+        HIPCHK(hipMemcpyAsync(out, d_out, ctx->nshells * 9 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipFreeAsync(d_out, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return BFPSH_OK;
+    }
+
+    template <int NB>
+    static int interp_ab_nb(bfpsh_ctx* ctx, double dt, int order) {
+        const int nthreads = 256;
+        const int64_t nblocks = (ctx->np + nthreads - 1) / nthreads;
+        const size_t lds = size_t(3) * (2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
+        LAUNCH("interp_ab", (k_interp_ab<T, NB><<<dim3(unsigned(nblocks)), nthreads, lds, ctx->stream>>>(
+                                (const T*)ctx->ru, ctx->d_pos, ctx->d_rhs, ctx->np, ctx->nsteps, ctx->head,
+                                order, ctx->add_head ? 1 : 0, dt, ctx->d_tab, ctx->deg, pgeom(ctx))));
+        return BFPSH_OK;
+    }
+    template <int NB, int NC>
+    static int sample_nb(bfpsh_ctx* ctx, const void* f, double* d_out) {
+        const int nthreads = 256;
+        const int64_t nblocks = (ctx->np + nthreads - 1) / nthreads;
+        const size_t lds = size_t(3) * (2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
+        LAUNCH("sample", (k_sample<T, NB, NC><<<dim3(unsigned(nblocks)), nthreads, lds, ctx->stream>>>(
+                             (const T*)f, ctx->d_pos, d_out, ctx->np, ctx->d_tab, ctx->deg, pgeom(ctx))));
+        return BFPSH_OK;
+    }
+
+    static int particles_loop(bfpsh_ctx* ctx, double dt) {
+        if (ctx->np == 0) return BFPSH_OK;
+        const int order = std::min(ctx->step_idx, ctx->nsteps);
+        int r;
+        switch (ctx->pn) {
+        case 1: r = interp_ab_nb<1>(ctx, dt, order); break;
+        case 2: r = interp_ab_nb<2>(ctx, dt, order); break;
+        case 3: r = interp_ab_nb<3>(ctx, dt, order); break;
+        case 4: r = interp_ab_nb<4>(ctx, dt, order); break;
+        case 5: r = interp_ab_nb<5>(ctx, dt, order); break;
+        case 6: r = interp_ab_nb<6>(ctx, dt, order); break;
+        case 7: r = interp_ab_nb<7>(ctx, dt, order); break;
+        case 8: r = interp_ab_nb<8>(ctx, dt, order); break;
+        case 9: r = interp_ab_nb<9>(ctx, dt, order); break;
+        default: r = interp_ab_nb<10>(ctx, dt, order); break;
+        }
+        if (r) return r;
+        // inc_step_idx + shift_rhs_vectors: the oldest slot becomes the (zero) head
+        ctx->step_idx += 1;
+        ctx->head = (ctx->head + ctx->nsteps - 1) % ctx->nsteps;
+        ctx->head_zero = true;
+        ctx->add_head = false;
+        return BFPSH_OK;
+    }
+
+    static int sample(bfpsh_ctx* ctx, int which, double* d_out) {
+        const void* f = which == BFPSH_RVELOCITY ? ctx->ru : ctx->rgrad;
+        const bool three = which == BFPSH_RVELOCITY;
+        switch (ctx->pn) {
+#define BFPSH_SAMPLE_CASE(k) \
+        case k: return three ? sample_nb<k, 3>(ctx, f, d_out) : sample_nb<k, 9>(ctx, f, d_out);
+        BFPSH_SAMPLE_CASE(1) BFPSH_SAMPLE_CASE(2) BFPSH_SAMPLE_CASE(3) BFPSH_SAMPLE_CASE(4)
+        BFPSH_SAMPLE_CASE(5) BFPSH_SAMPLE_CASE(6) BFPSH_SAMPLE_CASE(7) BFPSH_SAMPLE_CASE(8)
+        BFPSH_SAMPLE_CASE(9) BFPSH_SAMPLE_CASE(10)
+#undef BFPSH_SAMPLE_CASE
+        }
+        return fail(ctx, BFPSH_EINVAL, "bad neighbour count");
+    }
+};
+

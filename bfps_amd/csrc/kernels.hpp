@@ -1,0 +1,664 @@
+// HIP kernels of the vorticity-equation step and the tracer loop (gfx950).
+//
+// Device data layout (one GPU holds the whole periodic box, cubic N^3):
+//   spectral plane  [ky][kz][kx],  kx = 0 .. N/2-1 stored (NXH = N/2)
+//   real plane      [y][z][x],     x  = 0 .. N-1
+// A vector field is 3 (or 6, 9) consecutive planes of PS = N*N*NXH complex
+// (spectral) or N^3 real values.  The kx = N/2 plane of the reference's
+// half spectrum is not stored: it lies outside the dealiasing sphere
+// (kx^2 >= (N/2)^2 > kM^2), the reference's io_checkpoint read zeroes it
+// (low_pass, bfps/cpp/kspace.cpp:235-253) and the time step never writes it
+// (bfps/cpp/vorticity_equation.cpp:372), so it is identically zero.
+//
+// Inverse 3D transform = z pass (within ky slab) -> y pass -> x (c2r) pass;
+// forward = x (r2c) -> z -> y.  The real-space array therefore comes out
+// as [y][z][x]; the x passes work on pairs of rows (z, z+1) packed as the
+// real and imaginary parts of one complex sequence.
+#pragma once
+#include "fft.hpp"
+
+namespace bfpsh {
+
+// ------------------------------------------------------------ configuration
+template <typename T, int N>
+struct Cfg {
+    static constexpr int NXH = N / 2;
+    static constexpr int RMAX = 16;
+    static constexpr int bcap() {
+        int b = int(128 / sizeof(C2<T>));
+        if (b > NXH) b = NXH;
+        while (NXH % b) b /= 2;
+        return b;
+    }
+    static constexpr int B = bcap();  // kx columns per strided tile
+    static constexpr int NT = (N * B / 8) < 64 ? 64 : ((N * B / 8) > 1024 ? 1024 : (N * B / 8));
+    static constexpr int E = (N * B + NT - 1) / NT;  // tile elements per thread
+    static constexpr int RP = (N * int(sizeof(C2<T>)) >= 8192) ? 1 : 2;  // row pairs per row block
+    static constexpr int NT_ROW = N >= 256 ? 256 : (N >= 64 ? 128 : 64);
+    static_assert(NT % B == 0, "tile threads must cover whole columns");
+};
+
+// ------------------------------------------------------------ parameters
+struct Geom {
+    int n;
+    double dkx, dky, dkz;
+    double kM2;
+    int smooth;  // 1: smooth filter exp(-36 (k^2/kM^2)^18), 0: two-thirds low pass
+};
+struct StepParams {
+    double nu, dt, famp, fk0, fk1;
+    int forcing;  // 0 none, 1 linear, 2 Kolmogorov
+    int fmode;
+};
+
+BFPSH_DEV double kwave(int i, int n, double dk) { return (i <= n / 2 ? i : i - n) * dk; }
+
+// ------------------------------------------------------------ K1 + first inverse pass
+// MODE 0: outputs u_hat (3), omega_hat (3)        [nonlinear term]
+// MODE 1: outputs u_hat (3)                        [velocity in real space]
+// MODE 2: outputs i k_j u_hat_i (9), k^2 < kM^2    [velocity gradient]
+// u_hat = i k x omega_hat / k^2 for 0 < k^2 <= kM^2, else 0
+// (compute_velocity, bfps/cpp/vorticity_equation.cpp:206-239, with the
+// symmetrize of field.cpp:900-978 applied to its kx = 0 plane by reading the
+// conjugated mirror mode; compute_gradient, bfps/cpp/field.cpp:1077-1097).
+template <int MODE> struct BsOut { static constexpr int NF = MODE == 0 ? 6 : (MODE == 1 ? 3 : 9); };
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__((Cfg<T, N>::NT))
+k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __restrict__ tw, Geom g) {
+    using CF = Cfg<T, N>;
+    constexpr int B = CF::B, NT = CF::NT, E = CF::E, NXH = CF::NXH;
+    constexpr size_t PS = size_t(N) * N * NXH;
+    constexpr int NF = BsOut<MODE>::NF;
+    __shared__ C2<T> lds[N * B];
+    const int tid = threadIdx.x;
+    const int kx0 = blockIdx.x * B;
+    const int ky = blockIdx.y;
+    const int b = tid % B;
+    const int kxi = kx0 + b;
+    const double kx = kxi * g.dkx, kyv = kwave(ky, N, g.dky);
+    C2<T> wr[3][E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int n = tid / B + i * (NT / B);
+        if (n < N) {
+            const size_t off = (size_t(ky) * N + n) * NXH + kxi;
+#pragma unroll
+            for (int c = 0; c < 3; c++) wr[c][i] = w[c * PS + off];
+        }
+    }
+    for (int f = 0; f < NF; f++) {
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int n = tid / B + i * (NT / B);
+            if (n < N) {
+                const double kz = kwave(n, N, g.dkz);
+                const double k2 = kx * kx + kyv * kyv + kz * kz;
+                C2<T> val;
+                if (MODE == 0 && f >= 3) {
+                    val = wr[f - 3][i];
+                } else {
+                    const int cu = MODE == 2 ? f % 3 : f;
+                    const bool keep = MODE == 2 ? (k2 < g.kM2) : true;
+                    C2<T> a0 = wr[0][i], a1 = wr[1][i], a2 = wr[2][i];
+                    // symmetrized kx = 0 plane: lower half takes conj(mirror)
+                    if (kxi == 0 && ((ky > N / 2) || (ky == 0 && n > N / 2))) {
+                        const size_t moff = (size_t((N - ky) % N) * N + (N - n) % N) * NXH;
+                        a0 = cconj<T>(w[moff]);
+                        a1 = cconj<T>(w[PS + moff]);
+                        a2 = cconj<T>(w[2 * PS + moff]);
+                    }
+                    C2<T> u = mk<T>(T(0), T(0));
+                    if (k2 <= g.kM2 && k2 > 0) {
+                        if (cu == 0) u = mk<T>(T(-(kyv * a2.y - kz * a1.y) / k2), T((kyv * a2.x - kz * a1.x) / k2));
+                        if (cu == 1) u = mk<T>(T(-(kz * a0.y - kx * a2.y) / k2), T((kz * a0.x - kx * a2.x) / k2));
+                        if (cu == 2) u = mk<T>(T(-(kx * a1.y - kyv * a0.y) / k2), T((kx * a1.x - kyv * a0.x) / k2));
+                    }
+                    if (MODE == 2) {
+                        const int j = f / 3;
+                        const double kj = j == 0 ? kx : (j == 1 ? kyv : kz);
+                        val = keep ? mk<T>(T(-kj * u.y), T(kj * u.x)) : mk<T>(T(0), T(0));
+                    } else {
+                        val = u;
+                    }
+                }
+                lds[n * B + b] = val;
+            }
+        }
+        __syncthreads();
+        fft_tile<T, N, B, B, NT, CF::RMAX, +1>(lds, tw, tid);
+        C2<T>* o = out + size_t(f) * PS;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int n = tid / B + i * (NT / B);
+            if (n < N) o[(size_t(ky) * N + n) * NXH + kxi] = lds[n * B + b];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ generic strided pass (in place)
+// element (n, b) of tile (blockIdx.x, blockIdx.y) in plane blockIdx.z at
+//   base + blockIdx.z * PS + blockIdx.y * ostride + n * estride + kx0 + b
+template <typename T, int N, int SIGN>
+__global__ void __launch_bounds__((Cfg<T, N>::NT))
+k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __restrict__ tw) {
+    using CF = Cfg<T, N>;
+    constexpr int B = CF::B, NT = CF::NT, E = CF::E, NXH = CF::NXH;
+    constexpr size_t PS = size_t(N) * N * NXH;
+    __shared__ C2<T> lds[N * B];
+    const int tid = threadIdx.x;
+    const int b = tid % B;
+    C2<T>* base = data + blockIdx.z * PS + blockIdx.y * ostride + blockIdx.x * B + b;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int n = tid / B + i * (NT / B);
+        if (n < N) lds[n * B + b] = base[n * estride];
+    }
+    __syncthreads();
+    fft_tile<T, N, B, B, NT, CF::RMAX, SIGN>(lds, tw, tid);
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int n = tid / B + i * (NT / B);
+        if (n < N) base[n * estride] = lds[n * B + b];
+    }
+}
+
+// ------------------------------------------------------------ x passes on row pairs
+// Row pair q of row block: rows (y, z = 2q) and (y, z = 2q+1) of a plane.
+// Inverse: Z[k] = A[k] + i B[k] (k < N/2; k = 0 real parts only, as a c2r
+// ignores the imaginary part of the zero mode), Z[N/2] = 0,
+// Z[N-k] = conj(A[k]) + i conj(B[k]); z = IDFT(Z); row z = Re z, row z+1 = Im z.
+// Forward: Z = DFT(a + i b); A[k] = (Z[k] + conj Z[N-k]) / 2,
+// B[k] = (Z[k] - conj Z[N-k]) / (2 i).
+template <typename T, int N, int LS>
+BFPSH_DEV void rows_unpack_inverse(C2<T>* lds, int col, const C2<T>* __restrict__ ra,
+                                   const C2<T>* __restrict__ rb, int tid, int nt) {
+    constexpr int H = N / 2;
+    for (int k = tid; k < H; k += nt) {
+        C2<T> A = ra[k], Bv = rb[k];
+        if (k == 0) {
+            lds[col] = mk<T>(A.x, Bv.x);
+            lds[H * LS + col] = mk<T>(T(0), T(0));
+        } else {
+            lds[k * LS + col] = mk<T>(A.x - Bv.y, A.y + Bv.x);
+            lds[(N - k) * LS + col] = mk<T>(A.x + Bv.y, Bv.x - A.y);
+        }
+    }
+}
+template <typename T, int N, int LS>
+BFPSH_DEV void rows_pack_forward(const C2<T>* lds, int col, C2<T>* __restrict__ ra, C2<T>* __restrict__ rb,
+                                 int tid, int nt) {
+    constexpr int H = N / 2;
+    for (int k = tid; k < H; k += nt) {
+        const C2<T> Z = lds[k * LS + col];
+        const C2<T> M = lds[((N - k) % N) * LS + col];
+        ra[k] = mk<T>(T(0.5) * (Z.x + M.x), T(0.5) * (Z.y - M.y));
+        rb[k] = mk<T>(T(0.5) * (Z.y + M.y), T(-0.5) * (Z.x - M.x));
+    }
+}
+
+// u x omega / N^3 in real space between the inverse and forward x passes
+// (omega_nonlin, bfps/cpp/vorticity_equation.cpp:297-323).
+// in:  planes 0..2 = u_hat, 3..5 = omega_hat after the z and y inverse passes
+// out: planes 0..2 = x-transformed (u x omega)/N^3 (in place); optional real
+//      velocity ru (3 real planes [y][z][x]) for the tracers.
+template <typename T, int N>
+__global__ void __launch_bounds__((Cfg<T, N>::NT_ROW))
+k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restrict__ tw, T npoints) {
+    using CF = Cfg<T, N>;
+    constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = 6 * RP;
+    constexpr size_t PS = size_t(N) * N * NXH;
+    __shared__ C2<T> lds[N * LS];
+    const int tid = threadIdx.x;
+    const int q0 = blockIdx.x * RP;  // first row pair; rows (y, z) flattened: row = y*N + z
+    for (int c = 0; c < 6; c++)
+        for (int p = 0; p < RP; p++) {
+            const size_t row = size_t(q0 + p) * 2;
+            rows_unpack_inverse<T, N, LS>(lds, c * RP + p, data + c * PS + row * NXH,
+                                          data + c * PS + (row + 1) * NXH, tid, NT);
+        }
+    __syncthreads();
+    fft_tile<T, N, LS, LS, NT, CF::RMAX, +1>(lds, tw, tid);
+    for (int e = tid; e < N * RP; e += NT) {
+        const int nn = e / RP, p = e % RP;
+        C2<T> u[3], w[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            u[c] = lds[nn * LS + c * RP + p];
+            w[c] = lds[nn * LS + (3 + c) * RP + p];
+        }
+        if (ru) {
+            const size_t row = size_t(q0 + p) * 2;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                ru[c * size_t(N) * N * N + row * N + nn] = u[c].x;
+                ru[c * size_t(N) * N * N + (row + 1) * N + nn] = u[c].y;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            const T re = u[c1].x * w[c2].x - u[c2].x * w[c1].x;
+            const T im = u[c1].y * w[c2].y - u[c2].y * w[c1].y;
+            lds[nn * LS + c * RP + p] = mk<T>(re / npoints, im / npoints);
+        }
+    }
+    __syncthreads();
+    fft_tile<T, N, 3 * RP, LS, NT, CF::RMAX, -1>(lds, tw, tid);
+    for (int c = 0; c < 3; c++)
+        for (int p = 0; p < RP; p++) {
+            const size_t row = size_t(q0 + p) * 2;
+            rows_pack_forward<T, N, LS>(lds, c * RP + p, data + c * PS + row * NXH,
+                                        data + c * PS + (row + 1) * NXH, tid, NT);
+        }
+}
+
+// c2r x pass of NC planes into real planes out ([y][z][x] each)
+template <typename T, int N, int NC>
+__global__ void __launch_bounds__((Cfg<T, N>::NT_ROW))
+k_rows_c2r(const C2<T>* __restrict__ data, T* __restrict__ out, const C2<T>* __restrict__ tw) {
+    using CF = Cfg<T, N>;
+    constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = 3 * RP;
+    constexpr size_t PS = size_t(N) * N * NXH;
+    __shared__ C2<T> lds[N * LS];
+    const int tid = threadIdx.x;
+    const int q0 = blockIdx.x * RP;
+    for (int c0 = 0; c0 < NC; c0 += 3) {
+        for (int c = 0; c < 3; c++)
+            for (int p = 0; p < RP; p++) {
+                const size_t row = size_t(q0 + p) * 2;
+                rows_unpack_inverse<T, N, LS>(lds, c * RP + p, data + (c0 + c) * PS + row * NXH,
+                                              data + (c0 + c) * PS + (row + 1) * NXH, tid, NT);
+            }
+        __syncthreads();
+        fft_tile<T, N, LS, LS, NT, CF::RMAX, +1>(lds, tw, tid);
+        for (int e = tid; e < N * LS; e += NT) {
+            const int nn = e / LS, col = e % LS;
+            const int c = col / RP, p = col % RP;
+            const size_t row = size_t(q0 + p) * 2;
+            const C2<T> v = lds[nn * LS + col];
+            out[(c0 + c) * size_t(N) * N * N + row * N + nn] = v.x;
+            out[(c0 + c) * size_t(N) * N * N + (row + 1) * N + nn] = v.y;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ last forward pass + update
+// STAGE 1..3 of the integrating-factor SSP-RK3 step
+// (vorticity_equation::step, bfps/cpp/vorticity_equation.cpp:358-439), fused
+// with the tail of omega_nonlin (:323-355): dealias (kspace.cpp:461-485),
+// curl, forcing (:241-288), force_divfree (kspace.cpp:487-519).
+// data: planes 0..2 hold the output of the z forward pass ([y][kz][kx]).
+// v0 = omega(t); vp = previous stage (stage 2: v1, stage 3: v2); dst receives
+// the new stage (stage 3 writes v0 in place and applies the final projection).
+template <typename T, int N, int STAGE>
+__global__ void __launch_bounds__((Cfg<T, N>::NT))
+k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __restrict__ vp,
+               C2<T>* dst, const C2<T>* __restrict__ tw, Geom g, StepParams sp) {
+    using CF = Cfg<T, N>;
+    constexpr int B = CF::B, NT = CF::NT, E = CF::E, NXH = CF::NXH;
+    constexpr size_t PS = size_t(N) * N * NXH;
+    __shared__ C2<T> lds[N * B];
+    const int tid = threadIdx.x;
+    const int b = tid % B;
+    const int kxi = blockIdx.x * B + b;
+    const int kz = blockIdx.y;
+    C2<T> res[3][E];
+    for (int c = 0; c < 3; c++) {
+        const C2<T>* src = data + c * PS + size_t(kz) * NXH + kxi;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int n = tid / B + i * (NT / B);
+            if (n < N) lds[n * B + b] = src[size_t(n) * N * NXH];
+        }
+        __syncthreads();
+        fft_tile<T, N, B, B, NT, CF::RMAX, -1>(lds, tw, tid);
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int n = tid / B + i * (NT / B);
+            if (n < N) res[c][i] = lds[n * B + b];
+        }
+        __syncthreads();
+    }
+    const double kx = kxi * g.dkx, kzv = kwave(kz, N, g.dkz);
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int ky = tid / B + i * (NT / B);
+        if (ky >= N) continue;
+        const double kyv = kwave(ky, N, g.dky);
+        const double k2 = kx * kx + kyv * kyv + kzv * kzv;
+        const size_t off = (size_t(ky) * N + kz) * NXH + kxi;
+        // dealias
+        C2<T> a[3];
+        if (g.smooth) {
+            const double r = k2 / g.kM2;
+            const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4, r16 = r8 * r8;
+            const double flt = exp(-36.0 * (r16 * r2));
+#pragma unroll
+            for (int c = 0; c < 3; c++) a[c] = mk<T>(T(res[c][i].x * flt), T(res[c][i].y * flt));
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++) a[c] = k2 >= g.kM2 ? mk<T>(T(0), T(0)) : res[c][i];
+        }
+        // curl: i k x a
+        C2<T> cu[3];
+        cu[0] = mk<T>(T(-(kyv * a[2].y - kzv * a[1].y)), T(kyv * a[2].x - kzv * a[1].x));
+        cu[1] = mk<T>(T(-(kzv * a[0].y - kx * a[2].y)), T(kzv * a[0].x - kx * a[2].x));
+        cu[2] = mk<T>(T(-(kx * a[1].y - kyv * a[0].y)), T(kx * a[1].x - kyv * a[0].x));
+        // forcing with the stage's source vorticity
+        const C2<T>* srcv = STAGE == 1 ? v0 : vp;
+        if (sp.forcing == 1) {
+            const double kn = sqrt(k2);
+            if (sp.fk0 <= kn && sp.fk1 >= kn) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const C2<T> s = srcv[c * PS + off];
+                    cu[c] = mk<T>(T(cu[c].x + sp.famp * double(s.x) * 1.0), T(cu[c].y + sp.famp * double(s.y) * 1.0));
+                }
+            }
+        } else if (sp.forcing == 2) {
+            if (kxi == 0 && kz == 0 && (ky == sp.fmode || ky == N - sp.fmode))
+                cu[2].x = T(cu[2].x - sp.famp * 1.0 / 2);
+        }
+        // force_divfree
+        if (k2 > 0) {
+            const T t0 = T((kx * cu[0].x + kyv * cu[1].x + kzv * cu[2].x) / k2);
+            const T t1 = T((kx * cu[0].y + kyv * cu[1].y + kzv * cu[2].y) / k2);
+            cu[0] = mk<T>(T(cu[0].x - t0 * kx), T(cu[0].y - t1 * kx));
+            cu[1] = mk<T>(T(cu[1].x - t0 * kyv), T(cu[1].y - t1 * kyv));
+            cu[2] = mk<T>(T(cu[2].x - t0 * kzv), T(cu[2].y - t1 * kzv));
+        } else {
+            cu[0] = cu[1] = cu[2] = mk<T>(T(0), T(0));
+        }
+        // integrating-factor RK stage
+        const double dt = sp.dt;
+        if (STAGE == 1) {
+            if (k2 <= g.kM2) {
+                const double f0 = exp(-sp.nu * k2 * dt);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const C2<T> w0 = v0[c * PS + off];
+                    dst[c * PS + off] = mk<T>(T((w0.x + dt * cu[c].x) * f0), T((w0.y + dt * cu[c].y) * f0));
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) dst[c * PS + off] = mk<T>(T(0), T(0));
+            }
+        } else if (STAGE == 2) {
+            if (k2 <= g.kM2) {
+                const double f0 = exp(-sp.nu * k2 * dt / 2);
+                const double f1 = exp(sp.nu * k2 * dt / 2);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const C2<T> w0 = v0[c * PS + off];
+                    const C2<T> w1 = vp[c * PS + off];
+                    dst[c * PS + off] = mk<T>(T((3 * w0.x * f0 + (w1.x + dt * cu[c].x) * f1) * 0.25),
+                                              T((3 * w0.y * f0 + (w1.y + dt * cu[c].y) * f1) * 0.25));
+                }
+            }
+        } else {
+            C2<T> nv[3];
+            if (k2 <= g.kM2) {
+                const double f0 = exp(-sp.nu * k2 * dt * 0.5);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const C2<T> w0 = v0[c * PS + off];
+                    const C2<T> w2 = vp[c * PS + off];
+                    nv[c] = mk<T>(T((w0.x * f0 + 2 * (w2.x + dt * cu[c].x)) * f0 / 3),
+                                  T((w0.y * f0 + 2 * (w2.y + dt * cu[c].y)) * f0 / 3));
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; c++) nv[c] = v0[c * PS + off];
+            }
+            // final force_divfree of the new vorticity
+            if (k2 > 0) {
+                const T t0 = T((kx * nv[0].x + kyv * nv[1].x + kzv * nv[2].x) / k2);
+                const T t1 = T((kx * nv[0].y + kyv * nv[1].y + kzv * nv[2].y) / k2);
+                nv[0] = mk<T>(T(nv[0].x - t0 * kx), T(nv[0].y - t1 * kx));
+                nv[1] = mk<T>(T(nv[1].x - t0 * kyv), T(nv[1].y - t1 * kyv));
+                nv[2] = mk<T>(T(nv[2].x - t0 * kzv), T(nv[2].y - t1 * kzv));
+            } else {
+                nv[0] = nv[1] = nv[2] = mk<T>(T(0), T(0));
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) dst[c * PS + off] = nv[c];
+        }
+    }
+}
+
+// ------------------------------------------------------------ small spectral kernels
+// field::symmetrize on the kx = 0 plane (bfps/cpp/field.cpp:900-978):
+// Im(0,0,0) = 0; (0, kz) = conj(0, N-kz) for N/2 < kz < N;
+// (ky, kz) = conj(N-ky, (N-kz) mod N) for N/2 < ky < N.
+template <typename T>
+__global__ void k_symmetrize(C2<T>* v, int n, int nc) {
+    const int nxh = n / 2;
+    const size_t ps = size_t(n) * n * nxh;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    const int ky = idx / n, kz = idx % n;
+    for (int c = 0; c < nc; c++) {
+        C2<T>* p = v + c * ps;
+        if (ky == 0 && kz == 0) {
+            p[0].y = T(0);
+        } else if ((ky > n / 2) || (ky == 0 && kz > n / 2)) {
+            const C2<T> m = p[(size_t((n - ky) % n) * n + (n - kz) % n) * nxh];
+            p[(size_t(ky) * n + kz) * nxh] = cconj<T>(m);
+        }
+    }
+}
+
+// low_pass(kM) (k^2 >= kM^2 -> 0) followed by force_divfree
+// (io_checkpoint read projections, bfps/cpp/vorticity_equation.hpp:118-127)
+template <typename T>
+__global__ void k_lowpass_divfree(C2<T>* v, Geom g) {
+    const int n = g.n, nxh = n / 2;
+    const size_t ps = size_t(n) * n * nxh;
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= ps) return;
+    const int kxi = idx % nxh, kz = (idx / nxh) % n, ky = idx / (size_t(nxh) * n);
+    const double kx = kxi * g.dkx, kyv = kwave(ky, n, g.dky), kzv = kwave(kz, n, g.dkz);
+    const double k2 = kx * kx + kyv * kyv + kzv * kzv;
+    C2<T> a[3];
+    for (int c = 0; c < 3; c++) a[c] = v[c * ps + idx];
+    if (k2 >= g.kM2) a[0] = a[1] = a[2] = mk<T>(T(0), T(0));
+    if (k2 > 0) {
+        const T t0 = T((kx * a[0].x + kyv * a[1].x + kzv * a[2].x) / k2);
+        const T t1 = T((kx * a[0].y + kyv * a[1].y + kzv * a[2].y) / k2);
+        a[0] = mk<T>(T(a[0].x - t0 * kx), T(a[0].y - t1 * kx));
+        a[1] = mk<T>(T(a[1].x - t0 * kyv), T(a[1].y - t1 * kyv));
+        a[2] = mk<T>(T(a[2].x - t0 * kzv), T(a[2].y - t1 * kzv));
+    } else {
+        a[0] = a[1] = a[2] = mk<T>(T(0), T(0));
+    }
+    for (int c = 0; c < 3; c++) v[c * ps + idx] = a[c];
+}
+
+// Biot-Savart into a spectral field (compute_velocity without symmetrize;
+// the caller symmetrizes), used for host-visible cvelocity.
+template <typename T>
+__global__ void k_biot_savart(const C2<T>* __restrict__ w, C2<T>* __restrict__ u, Geom g) {
+    const int n = g.n, nxh = n / 2;
+    const size_t ps = size_t(n) * n * nxh;
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= ps) return;
+    const int kxi = idx % nxh, kz = (idx / nxh) % n, ky = idx / (size_t(nxh) * n);
+    const double kx = kxi * g.dkx, kyv = kwave(ky, n, g.dky), kzv = kwave(kz, n, g.dkz);
+    const double k2 = kx * kx + kyv * kyv + kzv * kzv;
+    const C2<T> a0 = w[idx], a1 = w[ps + idx], a2 = w[2 * ps + idx];
+    if (k2 <= g.kM2 && k2 > 0) {
+        u[idx] = mk<T>(T(-(kyv * a2.y - kzv * a1.y) / k2), T((kyv * a2.x - kzv * a1.x) / k2));
+        u[ps + idx] = mk<T>(T(-(kzv * a0.y - kx * a2.y) / k2), T((kzv * a0.x - kx * a2.x) / k2));
+        u[2 * ps + idx] = mk<T>(T(-(kx * a1.y - kyv * a0.y) / k2), T((kx * a1.x - kyv * a0.x) / k2));
+    } else {
+        u[idx] = u[ps + idx] = u[2 * ps + idx] = mk<T>(T(0), T(0));
+    }
+}
+
+// kspace::cospectrum (bfps/cpp/kspace.cpp:521-598): FP64 shell sums
+// S[s][i][j] += nxmodes (Re a_i Re b_j + Im a_i Im b_j), s = int(|k| / dk),
+// k^2 <= kM^2, nxmodes = 1 at kx = 0 else 2.
+template <typename T>
+__global__ void k_cospectrum(const C2<T>* __restrict__ a, const C2<T>* __restrict__ bb, double* out,
+                             Geom g, double dk, int nshells) {
+    extern __shared__ double acc[];
+    const int n = g.n, nxh = n / 2;
+    const size_t ps = size_t(n) * n * nxh;
+    for (int i = threadIdx.x; i < nshells * 9; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+    for (size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < ps; idx += size_t(gridDim.x) * blockDim.x) {
+        const int kxi = idx % nxh, kz = (idx / nxh) % n, ky = idx / (size_t(nxh) * n);
+        const double kx = kxi * g.dkx, kyv = kwave(ky, n, g.dky), kzv = kwave(kz, n, g.dkz);
+        const double k2 = kx * kx + kyv * kyv + kzv * kzv;
+        if (k2 > g.kM2) continue;
+        const double nxm = kxi == 0 ? 1.0 : 2.0;
+        const int s = int(sqrt(k2) / dk) * 9;
+        C2<T> av[3], bv[3];
+        for (int c = 0; c < 3; c++) { av[c] = a[c * ps + idx]; bv[c] = bb[c * ps + idx]; }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                atomicAdd(&acc[s + i * 3 + j], nxm * (double(av[i].x) * double(bv[j].x) + double(av[i].y) * double(bv[j].y)));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nshells * 9; i += blockDim.x)
+        if (acc[i] != 0.0) atomicAdd(&out[i], acc[i]);
+}
+
+// ------------------------------------------------------------ tracers
+// Interpolation (particles_field_computer::apply_computation,
+// bfps/cpp/particles/particles_field_computer.hpp:65-157, one rank):
+// per axis, cell = floor(pos / h) mod N, x = position inside the cell in
+// [0,1) (get_norm_pos_in_cell, :55-63), weights beta_j(x), j = 0..2n+1 for
+// nodes cell-n .. cell+n+1 (wrapped); sum over z, x, y of
+// f(x,y,z) * ((bz * by) * bx) in double.  Field planes are [y][z][x].
+struct PartGeom {
+    int n;            // grid size
+    double width[3];  // box widths 2 pi / dk
+    double h[3];      // grid spacing width / N
+};
+
+template <int NB>
+BFPSH_DEV void interp_setup(const double* pos, const double* __restrict__ tab, int deg, const PartGeom& pg,
+                            double (&bw)[3][2 * NB + 2], int (&cell)[3]) {
+    constexpr int W = 2 * NB + 2;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double s = pos[d];
+        const double rep = floor(s / pg.width[d]);
+        const double inbox = s - rep * pg.width[d];
+        const double ci = floor(inbox / pg.h[d]);
+        const double xr = (inbox - ci * pg.h[d]) / pg.h[d];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            double acc = 0.0;
+            for (int q = deg; q >= 0; q--) acc = acc * xr + tab[j * (deg + 1) + q];
+            bw[d][j] = acc;
+        }
+        const int lvl = int(floor(s / pg.h[d]));
+        cell[d] = ((lvl % pg.n) + pg.n) % pg.n;
+    }
+}
+
+template <typename T, int NB, int NC>
+BFPSH_DEV void interp_gather(const T* __restrict__ f, const PartGeom& pg, const double (&bw)[3][2 * NB + 2],
+                             const int (&cell)[3], double (&acc)[NC]) {
+    constexpr int W = 2 * NB + 2;
+    const int n = pg.n;
+    const size_t ps = size_t(n) * n * n;
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    for (int jz = 0; jz < W; jz++) {
+        int zz = cell[2] - NB + jz;
+        zz = zz < 0 ? zz + n : (zz >= n ? zz - n : zz);
+        for (int jx = 0; jx < W; jx++) {
+            int xx = cell[0] - NB + jx;
+            xx = xx < 0 ? xx + n : (xx >= n ? xx - n : xx);
+            for (int jy = 0; jy < W; jy++) {
+                int yy = cell[1] - NB + jy;
+                yy = yy < 0 ? yy + n : (yy >= n ? yy - n : yy);
+                const double coef = bw[2][jz] * bw[1][jy] * bw[0][jx];
+                const size_t ri = (size_t(yy) * n + zz) * n + xx;
+#pragma unroll
+                for (int c = 0; c < NC; c++) acc[c] += double(f[c * ps + ri]) * coef;
+            }
+        }
+    }
+}
+
+// interpolate velocity into the head of the rhs ring, then Adams-Bashforth
+// (particles_adams_bashforth.hpp:41-131) with order = min(step_idx, nsteps)
+// (particles_system.hpp:173-178).  rhs slot of history index i is
+// (head + i) mod nsteps.  add_head: the head slot held a caller-supplied
+// value to accumulate onto (first loop after a put only).
+template <typename T, int NB>
+__global__ void k_interp_ab(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ rhs,
+                            int64_t np, int nsteps, int head, int order, int add_head, double dt,
+                            const double* __restrict__ tab, int deg, PartGeom pg) {
+    constexpr int W = 2 * NB + 2;
+    extern __shared__ double stab[];
+    const int tabsize = 3 * W * (deg + 1);
+    for (int i = threadIdx.x; i < tabsize; i += blockDim.x) stab[i] = tab[i];
+    __syncthreads();
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    double x[3] = {pos[p * 3 + 0], pos[p * 3 + 1], pos[p * 3 + 2]};
+    double bw[3][W];
+    int cell[3];
+    interp_setup<NB>(x, stab, deg, pg, bw, cell);
+    double r0[3];
+    interp_gather<T, NB, 3>(f, pg, bw, cell, r0);
+    double* h0 = rhs + size_t(head) * np * 3 + p * 3;
+    if (add_head) {
+        r0[0] += h0[0];
+        r0[1] += h0[1];
+        r0[2] += h0[2];
+    }
+    h0[0] = r0[0];
+    h0[1] = r0[1];
+    h0[2] = r0[2];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double r[6];
+        r[0] = r0[d];
+        for (int i = 1; i < order; i++) r[i] = rhs[size_t((head + i) % nsteps) * np * 3 + p * 3 + d];
+        double incr;
+        switch (order) {
+        case 1: incr = dt * r[0]; break;
+        case 2: incr = dt * (3. * r[0] - r[1]) / 2.; break;
+        case 3: incr = dt * (23. * r[0] - 16. * r[1] + 5. * r[2]) / 12.; break;
+        case 4: incr = dt * (55. * r[0] - 59. * r[1] + 37. * r[2] - 9. * r[3]) / 24.; break;
+        case 5: incr = dt * (1901. * r[0] - 2774. * r[1] + 2616. * r[2] - 1274. * r[3] + 251. * r[4]) / 720.; break;
+        default: incr = dt * (4277. * r[0] - 7923. * r[1] + 9982. * r[2] - 7298. * r[3] + 2877. * r[4] - 475. * r[5]) / 1440.; break;
+        }
+        x[d] += incr;
+    }
+    pos[p * 3 + 0] = x[0];
+    pos[p * 3 + 1] = x[1];
+    pos[p * 3 + 2] = x[2];
+}
+
+// sample an NC-component real field at the tracer positions
+template <typename T, int NB, int NC>
+__global__ void k_sample(const T* __restrict__ f, const double* __restrict__ pos, double* __restrict__ out,
+                         int64_t np, const double* __restrict__ tab, int deg, PartGeom pg) {
+    constexpr int W = 2 * NB + 2;
+    extern __shared__ double stab[];
+    const int tabsize = 3 * W * (deg + 1);
+    for (int i = threadIdx.x; i < tabsize; i += blockDim.x) stab[i] = tab[i];
+    __syncthreads();
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    double x[3] = {pos[p * 3 + 0], pos[p * 3 + 1], pos[p * 3 + 2]};
+    double bw[3][W];
+    int cell[3];
+    interp_setup<NB>(x, stab, deg, pg, bw, cell);
+    double acc[NC];
+    interp_gather<T, NB, NC>(f, pg, bw, cell, acc);
+#pragma unroll
+    for (int c = 0; c < NC; c++) out[p * NC + c] = acc[c];
+}
+
+}  // namespace bfpsh

@@ -1,0 +1,145 @@
+/* bfpsh -- C ABI of the MI355X (gfx950) backend for bfps's vorticity-equation
+ * time step and tracer loop.
+ *
+ * One context holds one periodic box entirely in device memory (cubic N^3,
+ * N in {16, 32, 48, 64, 128, 256, 512, 1024}) plus an optional tracer set.
+ * All calls are ordered on the context's HIP stream and are not re-entrant
+ * per context; every function returns 0 on success and a non-zero
+ * BFPSH_E* code otherwise, with a message in bfpsh_last_error().
+ * No exceptions cross this boundary.
+ *
+ * Host-side array layouts are the reference's (bfps/cpp/field.cpp:85-101):
+ *   complex field: rnumber {re, im} [ny][nz][nx/2+1][ncomp]  (FFTW transposed)
+ *   real field:    rnumber          [nz][ny][nx][ncomp]      (padding dropped)
+ *   tracers:       double positions [np][3], history [nsteps][np][3]
+ * where rnumber is float (BFPSH_FP32) or double (BFPSH_FP64), as
+ * vorticity_equation<rnumber, FFTW>.
+ */
+#ifndef BFPSH_H
+#define BFPSH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bfpsh_ctx bfpsh_ctx;
+
+/* precision: field storage type (rnumber of the reference) */
+enum { BFPSH_FP32 = 0, BFPSH_FP64 = 1 };
+/* dealiasing: kspace_dealias_type (bfps/cpp/kspace.hpp:40) */
+enum { BFPSH_TWO_THIRDS = 0, BFPSH_SMOOTH = 1 };
+/* forcing_type strings of vorticity_equation::add_forcing
+ * (bfps/cpp/vorticity_equation.cpp:241-288) */
+enum { BFPSH_FORCING_NONE = 0, BFPSH_FORCING_LINEAR = 1, BFPSH_FORCING_KOLMOGOROV = 2 };
+/* field selectors */
+enum {
+    BFPSH_CVORTICITY = 0, /* vorticity_equation::cvorticity (spectral) */
+    BFPSH_CVELOCITY = 1,  /* Biot-Savart of cvorticity (compute_velocity) */
+    BFPSH_RVELOCITY = 5,  /* real-space velocity (3 comps) */
+    BFPSH_RGRADIENT = 6   /* real-space velocity gradient (9 comps, j*3+i = d_j u_i) */
+};
+/* error codes */
+enum {
+    BFPSH_OK = 0,
+    BFPSH_EINVAL = 1,  /* bad argument or unsupported size */
+    BFPSH_EHIP = 2,    /* HIP runtime error */
+    BFPSH_ESTATE = 3,  /* call not valid in the current state */
+    BFPSH_ENOMEM = 4
+};
+
+/* Replaces the constructor vorticity_equation<rnumber, FFTW>(name, nx, ny, nz,
+ * dkx, dky, dkz, plan_rigor) (bfps/cpp/vorticity_equation.cpp:110-156) and
+ * the kspace<FFTW, SMOOTH> it builds (bfps/cpp/kspace.cpp:34-167).  The
+ * reference hard-codes SMOOTH; `dealias` also allows TWO_THIRDS.
+ * device: HIP device ordinal.  All fields start at zero. */
+int bfpsh_create(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, double dkz,
+                 int precision, int dealias, int device);
+/* Replaces ~vorticity_equation (bfps/cpp/vorticity_equation.cpp:158-169). */
+int bfpsh_destroy(bfpsh_ctx* ctx);
+const char* bfpsh_last_error(const bfpsh_ctx* ctx);
+
+/* Sets the public physics members that NSVE::initialize writes
+ * (bfps/cpp/full_code/NSVE.cpp:46-52): nu, forcing_type, famplitude, fk0,
+ * fk1, fmode. */
+int bfpsh_set_physics(bfpsh_ctx* ctx, double nu, int forcing, double famplitude, double fk0,
+                      double fk1, int fmode);
+/* kspace metadata (bfps/cpp/kspace.cpp:97-166): out[0] = kM, out[1] = kM2,
+ * out[2] = dk, out[3] = nshells; kshell/nshell (may be NULL) sized nshells. */
+int bfpsh_kspace_info(bfpsh_ctx* ctx, double* out4, double* kshell, int64_t* nshell);
+
+/* Replaces `*cvorticity = host_data` / field::io read of vorticity/complex
+ * (bfps/cpp/field.hpp:193-200, field.cpp:336-341) without the projection. */
+int bfpsh_put_cfield(bfpsh_ctx* ctx, int which, const void* host);
+/* Replaces reading field::get_cdata() of cvorticity / cvelocity
+ * (bfps/cpp/field.hpp:140-146).  The kx = nx/2 plane is returned as zero. */
+int bfpsh_get_cfield(bfpsh_ctx* ctx, int which, void* host);
+/* Real-space field (BFPSH_RVELOCITY or BFPSH_RGRADIENT) as computed by the
+ * last bfpsh_velocity_rspace / bfpsh_gradient_rspace. */
+int bfpsh_get_rfield(bfpsh_ctx* ctx, int which, void* host);
+
+/* Replaces the projections of vorticity_equation::io_checkpoint(read=true)
+ * (bfps/cpp/vorticity_equation.hpp:107-128): symmetrize (field.cpp:340),
+ * low_pass(kM), force_divfree. */
+int bfpsh_load_projection(bfpsh_ctx* ctx);
+/* Replaces vorticity_equation::step(dt) (bfps/cpp/vorticity_equation.cpp:358-439). */
+int bfpsh_step(bfpsh_ctx* ctx, double dt);
+/* Replaces `compute_velocity(cvorticity); cvelocity->ift();`
+ * (bfps/cpp/full_code/NSVEparticles.cpp:36-37). */
+int bfpsh_velocity_rspace(bfpsh_ctx* ctx);
+/* Replaces compute_gradient(kk, cvelocity, grad) followed by grad->ift()
+ * (bfps/cpp/field.cpp:1042-1101) on the velocity of cvorticity. */
+int bfpsh_gradient_rspace(bfpsh_ctx* ctx);
+/* Replaces kspace::cospectrum (bfps/cpp/kspace.cpp:521-598) of two 3-comp
+ * spectral fields a, b (BFPSH_CVORTICITY / BFPSH_CVELOCITY); out sized
+ * nshells*9 doubles, [shell][i][j]. */
+int bfpsh_cospectrum(bfpsh_ctx* ctx, int a, int b, double* out);
+
+/* Replaces particles_system_builder(...) (bfps/cpp/particles/particles_system_builder.hpp:238-257)
+ * for one rank: np tracers, interpolation neighbours n in [1,10] and
+ * smoothness m in [0,2] (particles_generic_interp.hpp:10-313), nsteps
+ * Adams-Bashforth history length in [1,6], initial step_idx. */
+int bfpsh_particles_create(bfpsh_ctx* ctx, int64_t np, int n, int m, int nsteps, int step_idx);
+/* State as read by particles_input_hdf5 (positions [np][3], history
+ * [nsteps][np][3], global indexes [np]; idx may be NULL for 0..np-1). */
+int bfpsh_particles_put(bfpsh_ctx* ctx, const double* pos, const double* rhs, const int64_t* idx);
+/* State as written by particles_output_hdf5::save: sorted by global index;
+ * rhs may be NULL; step_idx may be NULL. */
+int bfpsh_particles_get(bfpsh_ctx* ctx, double* pos, double* rhs, int64_t* idx, int* step_idx);
+/* Replaces particles_system::completeLoop(dt) (bfps/cpp/particles/particles_system.hpp:210-217)
+ * using the velocity of the current cvorticity. */
+int bfpsh_particles_complete_loop(bfpsh_ctx* ctx, double dt);
+/* Replaces particles_system::sample_compute_field for the real-space
+ * velocity (3 comps, BFPSH_RVELOCITY) or gradient (9 comps, BFPSH_RGRADIENT);
+ * out [np][ncomp], in the current (index-sorted) order. */
+int bfpsh_particles_sample(bfpsh_ctx* ctx, int which, double* out);
+
+/* Replaces NSVEparticles::step (bfps/cpp/full_code/NSVEparticles.cpp:33-41):
+ * velocity of omega(t) -> tracers completeLoop(dt) -> vorticity step(dt),
+ * with the tracer velocity taken from the first RK stage (same data). */
+int bfpsh_nsve_particles_step(bfpsh_ctx* ctx, double dt);
+
+int bfpsh_synchronize(bfpsh_ctx* ctx);
+/* Current iteration counter (vorticity_equation::iteration). */
+int bfpsh_get_iteration(bfpsh_ctx* ctx, int* it);
+int bfpsh_set_iteration(bfpsh_ctx* ctx, int it);
+
+/* Measurement: run `nsteps` steps (with tracers when with_particles != 0)
+ * between two HIP events on the context stream; *ms = elapsed time. */
+int bfpsh_time_steps(bfpsh_ctx* ctx, int nsteps, double dt, int with_particles, double* ms);
+/* Per-kernel timing with HIP events around every launch (adds overhead;
+ * for measurement runs only).  enable != 0 resets the counters. */
+int bfpsh_profile_enable(bfpsh_ctx* ctx, int enable);
+/* i-th profiled kernel: name (buffer of namelen), total ms, launches; returns
+ * BFPSH_EINVAL when i is out of range. */
+int bfpsh_profile_get(bfpsh_ctx* ctx, int i, char* name, int namelen, double* total_ms, int* count);
+/* Bytes of device memory held by the context. */
+int bfpsh_device_bytes(bfpsh_ctx* ctx, size_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BFPSH_H */

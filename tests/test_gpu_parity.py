@@ -1,0 +1,258 @@
+"""GPU parity tests: the HIP backend (through the C ABI) against the CPU oracle,
+the reference's golden checkpoint, and closed forms.
+
+Tolerances:
+  * FP64 fields: relative L2 <= 1e-11 against the FP64 oracle (rounding only).
+  * FP32 fields: relative L2 <= 2e-6 against the FP32-field oracle after a
+    few steps; the reference's own 1e-5 max-abs criterion on its fixture
+    (bfps/test/test_bfps_NSVEparticles.py:41-50).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def B():
+    import bfps_amd
+    bfps_amd.capi.load()
+    return bfps_amd
+
+
+def rel(a, b):
+    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+
+
+def random_vorticity(n, seed, k0=3.0, dtype=np.complex128):
+    """Band-limited random spectral vorticity in the reference layout."""
+    rng = np.random.default_rng(seed)
+    shp = (n, n, n // 2 + 1, 3)
+    a = (rng.standard_normal(shp) + 1j * rng.standard_normal(shp))
+    kx = np.arange(n // 2 + 1)
+    k1 = np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n)
+    K = np.sqrt(k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    a *= (np.exp(-(K / k0) ** 2) * (K > 0))[..., None]
+    return a.astype(dtype)
+
+
+NU = (2.0 * 2 / 32) ** (4. / 3)
+DT = 0.5 / 32
+
+
+def test_fixture_nsveparticles_two_jobs(B, oracle, golden):
+    """The reference's own regression (32^3 single, 10^4 tracers, n1 m1, AB4,
+    2 x 32 iterations with a restart), asserted at the reference's 1e-5."""
+    say("case: golden fixture 32^3 single, 64 steps, restart at 32")
+    fs = B.vorticity_equation("fixture", 32, precision="single")
+    fs.nu, fs.famplitude, fs.fk0, fs.fk1, fs.forcing_type = NU, 0.5, 2.0, 4.0, "linear"
+    fs.io_checkpoint(read=True, data=golden["vort_0"])
+    ps = B.particles_system(fs, 10000, 1, 1, 4, step_idx=1)
+    ps.put(golden["state_0"], golden["rhs_0"])
+    for _ in range(32):
+        B.NSVEparticles_step(fs, DT)
+    v = fs.cvorticity
+    pos, rhs, idx, si = ps.get()
+    say("it32 errors", np.abs(v - golden["vort_32"]).max(), np.abs(pos - golden["state_32"]).max(),
+        np.abs(rhs - golden["rhs_32"]).max())
+    assert si == 33
+    assert np.abs(v - golden["vort_32"]).max() < 1e-5
+    assert np.abs(pos - golden["state_32"]).max() < 1e-5
+    assert np.abs(rhs - golden["rhs_32"]).max() < 1e-5
+    fs2 = B.vorticity_equation("fixture2", 32, precision="single")
+    fs2.nu, fs2.famplitude, fs2.fk0, fs2.fk1, fs2.forcing_type = NU, 0.5, 2.0, 4.0, "linear"
+    fs2.io_checkpoint(read=True, data=v)
+    ps2 = B.particles_system(fs2, 10000, 1, 1, 4, step_idx=33)
+    ps2.put(pos, rhs)
+    for _ in range(32):
+        B.NSVEparticles_step(fs2, DT)
+    v = fs2.cvorticity
+    pos, rhs, idx, si = ps2.get()
+    say("it64 errors", np.abs(v - golden["vort_64"]).max(), np.abs(pos - golden["state_64"]).max(),
+        np.abs(rhs - golden["rhs_64"]).max())
+    assert si == 65
+    assert np.abs(v - golden["vort_64"]).max() < 1e-5
+    assert np.abs(pos - golden["state_64"]).max() < 1e-5
+    assert np.abs(rhs - golden["rhs_64"]).max() < 1e-5
+
+
+@pytest.mark.parametrize("n", [16, 32, 48, 64])
+@pytest.mark.parametrize("dealias", ["SMOOTH", "TWO_THIRDS"])
+def test_step_fp64_vs_oracle(B, oracle, n, dealias):
+    say("case: fp64 step n=%d dealias=%s" % (n, dealias))
+    w0 = random_vorticity(n, seed=n)
+    fs = B.vorticity_equation("t", n, precision="double", dealias=dealias)
+    fs.nu, fs.famplitude, fs.forcing_type = 0.02, 0.3, "linear"
+    fs.io_checkpoint(read=True, data=w0)
+    o = oracle.Solver(n, precision="double", dealias=dealias)
+    o.set_physics(0.02, "linear", 0.3, 2.0, 4.0)
+    o.set_cfield("vorticity", w0)
+    o.io_projection()
+    e0 = rel(fs.cvorticity, o.get_cfield("vorticity"))
+    say("  after projection rel", e0)
+    assert e0 < 1e-14
+    for _ in range(4):
+        fs.step(0.01)
+        o.step(0.01)
+    e = rel(fs.cvorticity, o.get_cfield("vorticity"))
+    say("  after 4 steps rel", e)
+    assert e < 1e-11
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_step_fp32_vs_oracle(B, oracle, n):
+    say("case: fp32 step n=%d" % n)
+    w0 = random_vorticity(n, seed=7 + n).astype(np.complex64)
+    fs = B.vorticity_equation("t", n, precision="single")
+    fs.nu, fs.famplitude, fs.forcing_type = 0.02, 0.3, "linear"
+    fs.io_checkpoint(read=True, data=w0)
+    o = oracle.Solver(n, precision="single")
+    o.set_physics(0.02, "linear", 0.3, 2.0, 4.0)
+    o.set_cfield("vorticity", w0)
+    o.io_projection()
+    for _ in range(4):
+        fs.step(0.01)
+        o.step(0.01)
+    e = rel(fs.cvorticity, o.get_cfield("vorticity"))
+    say("  fp32 rel after 4 steps", e)
+    assert e < 2e-6
+
+
+@pytest.mark.parametrize("forcing", ["none", "Kolmogorov"])
+def test_forcing_variants_fp64(B, oracle, forcing):
+    n = 32
+    say("case: forcing %s" % forcing)
+    w0 = random_vorticity(n, seed=3)
+    fs = B.vorticity_equation("t", n, precision="double")
+    fs.nu, fs.famplitude, fs.forcing_type, fs.fmode = 0.05, 0.7, forcing, 2
+    fs.io_checkpoint(read=True, data=w0)
+    o = oracle.Solver(n, precision="double")
+    o.set_physics(0.05, forcing, 0.7, 2.0, 4.0, 2)
+    o.set_cfield("vorticity", w0)
+    o.io_projection()
+    for _ in range(3):
+        fs.step(0.02)
+        o.step(0.02)
+    e = rel(fs.cvorticity, o.get_cfield("vorticity"))
+    say("  rel", e)
+    assert e < 1e-11
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-12), ("single", 2e-6)])
+def test_velocity_and_gradient_rspace(B, oracle, precision, tol):
+    n = 48
+    say("case: velocity/gradient rspace n=48 %s" % precision)
+    w0 = random_vorticity(n, seed=11)
+    if precision == "single":
+        w0 = w0.astype(np.complex64)
+    fs = B.vorticity_equation("t", n, precision=precision)
+    fs.io_checkpoint(read=True, data=w0)
+    o = oracle.Solver(n, precision=precision)
+    o.set_cfield("vorticity", w0)
+    o.io_projection()
+    assert rel(fs.compute_velocity(), (o.compute_velocity(), o.get_cfield("velocity"))[1]) < tol
+    fs.velocity_rspace()
+    o.velocity_rspace()
+    e = rel(fs.get_rfield(B.capi.RVELOCITY), o.get_rfield("rvelocity"))
+    say("  velocity rel", e)
+    assert e < tol
+    fs.gradient_rspace()
+    o.gradient_rspace()
+    e = rel(fs.get_rfield(B.capi.RGRADIENT), o.get_rfield("gradient"))
+    say("  gradient rel", e)
+    assert e < tol
+
+
+def test_cospectrum(B, oracle):
+    n = 32
+    say("case: cospectrum")
+    w0 = random_vorticity(n, seed=5)
+    fs = B.vorticity_equation("t", n, precision="double")
+    fs.io_checkpoint(read=True, data=w0)
+    o = oracle.Solver(n, precision="double")
+    o.set_cfield("vorticity", w0)
+    o.io_projection()
+    o.compute_velocity()
+    for a, b in [("velocity", "velocity"), ("vorticity", "vorticity"), ("velocity", "vorticity")]:
+        g = fs.cospectrum(a, b)
+        r = o.cospectrum(a, b)
+        assert np.abs(g - r).max() <= 1e-12 * np.abs(r).max()
+    ks, ns = o.kshell()
+    assert np.allclose(fs.kk.kshell, ks) and np.array_equal(fs.kk.nshell, ns)
+
+
+@pytest.mark.parametrize("nb,m", [(1, 0), (1, 1), (2, 2), (4, 1), (6, 2), (3, 0)])
+def test_tracers_vs_oracle(B, oracle, nb, m):
+    n = 32
+    say("case: tracers n=%d m=%d" % (nb, m))
+    w0 = random_vorticity(n, seed=9)
+    fs = B.vorticity_equation("t", n, precision="double")
+    fs.nu, fs.forcing_type = 0.05, "linear"
+    fs.io_checkpoint(read=True, data=w0)
+    o = oracle.Solver(n, precision="double")
+    o.set_physics(0.05, "linear", 1.0, 2.0, 4.0)
+    o.set_cfield("vorticity", w0)
+    o.io_projection()
+    rng = np.random.default_rng(nb * 10 + m)
+    npart = 3000
+    pos = rng.random((npart, 3)) * 2 * np.pi
+    pos[:10] -= 2 * np.pi  # negative coordinates exercise the periodic wrap
+    pos[10:20] += 4 * np.pi
+    ps = B.particles_system(fs, npart, nb, m, 4, step_idx=1)
+    ps.put(pos)
+    op = oracle.Particles(o, npart, nb, m, 4, step_idx=1)
+    op.set_state(pos)
+    for _ in range(5):
+        B.NSVEparticles_step(fs, 0.01)
+        oracle.nsve_particles_step(o, op, 0.01)
+    gp, grhs, gidx, gsi = ps.get()
+    op_pos, op_rhs, osi = op.get_state()
+    say("  pos err", np.abs(gp - op_pos).max(), "rhs err", np.abs(grhs - op_rhs).max())
+    assert gsi == osi == 6
+    assert np.abs(gp - op_pos).max() < 1e-12
+    assert np.abs(grhs - op_rhs).max() < 1e-12
+    # sampling velocity and gradient at the current positions
+    fs.velocity_rspace()
+    o.velocity_rspace()
+    assert np.abs(ps.sample("velocity") - op.sample("velocity")).max() < 1e-12
+    fs.gradient_rspace()
+    o.gradient_rspace()
+    assert np.abs(ps.sample("gradient") - op.sample("gradient")).max() < 1e-11
+
+
+@pytest.mark.parametrize("n,precision", [(64, "double"), (256, "single")])
+def test_beltrami_exact_decay(B, n, precision):
+    """ABC flow u = (cos y + sin z, cos z + sin x, cos x + sin y) has
+    omega = u, so u x omega = 0 and, without forcing, omega(t) = omega(0)
+    exp(-nu t) (|k| = 1 modes; exact integrating factor)."""
+    say("case: Beltrami decay n=%d %s" % (n, precision))
+    x = np.arange(n) * 2 * np.pi / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    u = np.stack([np.cos(Y) + np.sin(Z), np.cos(Z) + np.sin(X), np.cos(X) + np.sin(Y)], axis=-1)
+    w = np.fft.rfftn(u, axes=(0, 1, 2)).transpose(1, 0, 2, 3) / n ** 3  # omega = u
+    fs = B.vorticity_equation("abc", n, precision=precision)
+    fs.nu, fs.forcing_type = 0.1, "none"
+    fs.io_checkpoint(read=True, data=w)
+    dt, nsteps = 0.01, 10
+    for _ in range(nsteps):
+        fs.step(dt)
+    got = fs.cvorticity
+    exact = w * np.exp(-0.1 * dt * nsteps)
+    e = rel(got, exact)
+    say("  rel err vs exact", e)
+    assert e < (1e-13 if precision == "double" else 1e-6)
+
+
+def test_errors_are_reported(B):
+    with pytest.raises(B.capi.BfpshError):
+        B.vorticity_equation("bad", 40)
+    fs = B.vorticity_equation("t", 16, precision="double")
+    with pytest.raises(B.capi.BfpshError):
+        B.particles_system(fs, 10, 11, 1)
