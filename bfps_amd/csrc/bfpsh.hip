@@ -143,7 +143,7 @@ int bfpsh_create(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky
     std::vector<unsigned char> tw;
     if (precision == BFPSH_FP32) build_twiddles<float>(tw, n); else build_twiddles<double>(tw, n);
     HIPCHK(hipMalloc(&ctx->tw, tw.size()));
-    HIPCHK(hipMemcpy(ctx->tw, tw.data(), tw.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(ctx->tw, tw.data(), tw.size(), hipMemcpyHostToDevice, ctx->stream));
     ctx->bytes += tw.size();
     int r = ensure_work(ctx, 6);
     if (r) return r;
@@ -293,6 +293,7 @@ int bfpsh_particles_create(bfpsh_ctx* ctx, int64_t np, int n, int m, int nsteps,
     if (np < 0 || n < 1 || n > BFPS_INTERP_NMAX || m < 0 || m > BFPS_INTERP_MMAX || nsteps < 1 || nsteps > 6)
         return fail(ctx, BFPSH_EINVAL, "bad tracer parameters (n in [1,10], m in [0,2], nsteps in [1,6])");
     if (2 * n + 2 > ctx->n) return fail(ctx, BFPSH_EINVAL, "interpolation stencil wider than the grid");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     for (void* p : {(void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab})
         if (p) HIPCHK(hipFree(p));
     ctx->d_pos = ctx->d_rhs = ctx->d_tab = nullptr;
@@ -308,7 +309,8 @@ int bfpsh_particles_create(bfpsh_ctx* ctx, int64_t np, int n, int m, int nsteps,
     ctx->deg = bfps_interp_degree[k];
     const size_t tabn = size_t(3) * (2 * n + 2) * (ctx->deg + 1);
     HIPCHK(hipMalloc((void**)&ctx->d_tab, tabn * sizeof(double)));
-    HIPCHK(hipMemcpy(ctx->d_tab, bfps_interp_coef + bfps_interp_offset[k], tabn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(ctx->d_tab, bfps_interp_coef + bfps_interp_offset[k], tabn * sizeof(double),
+                          hipMemcpyHostToDevice, ctx->stream));
     if (np > 0) {
         HIPCHK(hipMalloc((void**)&ctx->d_pos, np * 3 * sizeof(double)));
         HIPCHK(hipMalloc((void**)&ctx->d_rhs, size_t(nsteps) * np * 3 * sizeof(double)));
@@ -353,7 +355,8 @@ int bfpsh_particles_get(bfpsh_ctx* ctx, double* pos, double* rhs, int64_t* idx, 
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ctx->idx[a] < ctx->idx[b]; });
     std::vector<double> tmp(np * 3);
     if (pos) {
-        HIPCHK(hipMemcpy(tmp.data(), ctx->d_pos, np * 3 * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(tmp.data(), ctx->d_pos, np * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
         for (int64_t i = 0; i < np; i++)
             for (int d = 0; d < 3; d++) pos[i * 3 + d] = tmp[order[i] * 3 + d];
     }
@@ -364,7 +367,9 @@ int bfpsh_particles_get(bfpsh_ctx* ctx, double* pos, double* rhs, int64_t* idx, 
                 std::fill(rhs, rhs + np * 3, 0.0);
                 continue;
             }
-            HIPCHK(hipMemcpy(tmp.data(), ctx->d_rhs + size_t(slot) * np * 3, np * 3 * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpyAsync(tmp.data(), ctx->d_rhs + size_t(slot) * np * 3, np * 3 * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
             for (int64_t i = 0; i < np; i++)
                 for (int d = 0; d < 3; d++) rhs[(size_t(h) * np + i) * 3 + d] = tmp[order[i] * 3 + d];
         }
@@ -397,7 +402,8 @@ int bfpsh_particles_sample(bfpsh_ctx* ctx, int which, double* out) {
     int r = PREC_CALL(sample, ctx, which, d_out);
     if (r) { (void)hipFree(d_out); return r; }
     std::vector<double> tmp(ctx->np * nc);
-    HIPCHK(hipMemcpy(tmp.data(), d_out, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(tmp.data(), d_out, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipFree(d_out));
     std::vector<int64_t> order(ctx->np);
     std::iota(order.begin(), order.end(), int64_t(0));

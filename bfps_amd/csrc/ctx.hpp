@@ -155,6 +155,7 @@ inline PartGeom pgeom(const bfpsh_ctx* ctx) {
 inline int ensure_work(bfpsh_ctx* ctx, int planes) {
     if (ctx->work_planes >= planes) return BFPSH_OK;
     if (ctx->work) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
         HIPCHK(hipFree(ctx->work));
         ctx->bytes -= ctx->work_planes * ctx->ps * 2 * ctx->es;
     }
