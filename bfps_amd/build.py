@@ -46,14 +46,19 @@ def sources():
     return out
 
 
-def build(jobs=None, sizes=SIZES, verbose=True):
-    os.makedirs(BUILD, exist_ok=True)
-    tasks = [(os.path.join(CSRC, "bfpsh.hip"), os.path.join(BUILD, "bfpsh.o"), [])]
+def build(jobs=None, sizes=SIZES, verbose=True, out=None, defines=()):
+    """Compile and link; `out` / `defines` build an experimental variant."""
+    lib = out or LIB
+    bdir = BUILD if not defines else BUILD + "_" + "_".join(d.replace("=", "") for d in defines)
+    os.makedirs(bdir, exist_ok=True)
+    extra = ["-D" + d for d in defines]
+    tasks = [(os.path.join(CSRC, "bfpsh.hip"), os.path.join(bdir, "bfpsh.o"), extra),
+             (os.path.join(CSRC, "particles.hip"), os.path.join(bdir, "particles.o"), extra)]
     for tn, t in (("f32", "float"), ("f64", "double")):
         for n in sizes:
             tasks.append((os.path.join(CSRC, "impl_inst.hip"),
-                          os.path.join(BUILD, "impl_%s_%d.o" % (tn, n)),
-                          ["-DBFPSH_T=%s" % t, "-DBFPSH_TN=%s" % tn, "-DBFPSH_N=%d" % n]))
+                          os.path.join(bdir, "impl_%s_%d.o" % (tn, n)),
+                          ["-DBFPSH_T=%s" % t, "-DBFPSH_TN=%s" % tn, "-DBFPSH_N=%d" % n] + extra))
     jobs = jobs or min(8, os.cpu_count() or 4)
     t0 = time.time()
     failed = []
@@ -69,19 +74,21 @@ def build(jobs=None, sizes=SIZES, verbose=True):
         for obj, errs in failed:
             print("---", obj, "\n", errs, file=sys.stderr)
         raise RuntimeError("HIP compilation failed")
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     objs = [t[1] for t in tasks]
     subprocess.check_call([_hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC"] + objs + ["-o", tmp])
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built %s in %.1fs" % (LIB, time.time() - t0))
-    return LIB
+        print("built %s in %.1fs" % (lib, time.time() - t0))
+    return lib
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", "--jobs", type=int, default=None)
     ap.add_argument("--sizes", default=None)
+    ap.add_argument("--out", default=None, help="library path (experimental variants)")
+    ap.add_argument("-D", dest="defines", action="append", default=[], help="NAME=VALUE")
     a = ap.parse_args()
     sz = tuple(int(s) for s in a.sizes.split(",")) if a.sizes else SIZES
-    build(a.jobs, sz)
+    build(a.jobs, sz, out=a.out, defines=tuple(a.defines))

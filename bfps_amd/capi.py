@@ -8,7 +8,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "libbfps_hip.so")
+LIB = os.environ.get("BFPSH_LIB") or os.path.join(HERE, "libbfps_hip.so")
 
 FP32, FP64 = 0, 1
 TWO_THIRDS, SMOOTH = 0, 1

@@ -159,6 +159,7 @@ int bfpsh_destroy(bfpsh_ctx* ctx) {
         for (void* p : {ctx->v0, ctx->v1, ctx->v2, ctx->work, ctx->ru, ctx->rgrad, ctx->tw,
                         (void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab})
             if (p) (void)hipFree(p);
+        bfpsh_part_release(ctx);
         resolve_profile(ctx);
         for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(ctx->stream);
@@ -296,6 +297,10 @@ int bfpsh_particles_create(bfpsh_ctx* ctx, int64_t np, int n, int m, int nsteps,
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (void* p : {(void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab})
         if (p) HIPCHK(hipFree(p));
+    {
+        int r = bfpsh_part_release(ctx);
+        if (r) return r;
+    }
     ctx->d_pos = ctx->d_rhs = ctx->d_tab = nullptr;
     ctx->np = np;
     ctx->pn = n;
@@ -339,6 +344,7 @@ int bfpsh_particles_put(bfpsh_ctx* ctx, const double* pos, const double* rhs, co
         HIPCHK(hipMemsetAsync(ctx->d_rhs, 0, pb * ctx->nsteps, ctx->stream));
     }
     ctx->head_zero = !ctx->add_head;
+    ctx->sort_age = 1 << 30;  // positions replaced: re-sort before the next loop
     if (idx) std::copy(idx, idx + ctx->np, ctx->idx.begin());
     else std::iota(ctx->idx.begin(), ctx->idx.end(), int64_t(0));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -386,7 +392,7 @@ int bfpsh_particles_complete_loop(bfpsh_ctx* ctx, double dt) {
         int r = PREC_CALL(velocity_rspace, ctx);
         if (r) return r;
     }
-    return PREC_CALL(particles_loop, ctx, dt);
+    return bfpsh_part_loop(ctx, dt);
 }
 
 int bfpsh_particles_sample(bfpsh_ctx* ctx, int which, double* out) {
@@ -399,7 +405,7 @@ int bfpsh_particles_sample(bfpsh_ctx* ctx, int which, double* out) {
     const int nc = which == BFPSH_RVELOCITY ? 3 : 9;
     double* d_out = nullptr;
     HIPCHK(hipMalloc((void**)&d_out, ctx->np * nc * sizeof(double)));
-    int r = PREC_CALL(sample, ctx, which, d_out);
+    int r = bfpsh_part_sample(ctx, which, d_out);
     if (r) { (void)hipFree(d_out); return r; }
     std::vector<double> tmp(ctx->np * nc);
     HIPCHK(hipMemcpyAsync(tmp.data(), d_out, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

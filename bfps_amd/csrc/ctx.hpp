@@ -51,6 +51,15 @@ struct bfpsh_ctx {
     double* d_rhs = nullptr;
     double* d_tab = nullptr;
     std::vector<int64_t> idx;
+    // spatial ordering of the tracers (a permutation, data stays in place)
+    uint32_t* d_keys = nullptr;    // 2 x np
+    uint32_t* d_perm = nullptr;    // 2 x np
+    void* d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    uint32_t* d_bstart = nullptr;  // per-brick [start, end) (2 x nbricks)
+    int64_t nbricks = 0;
+    int sort_every = 1;            // re-sort period in tracer steps
+    int sort_age = 1 << 30;        // steps since the last sort
     // profiling
     bool prof = false;
     std::vector<std::string> prof_names;
@@ -140,18 +149,6 @@ inline Geom geom(const bfpsh_ctx* ctx) {
     return g;
 }
 
-inline PartGeom pgeom(const bfpsh_ctx* ctx) {
-    PartGeom p;
-    p.n = ctx->n;
-    const double dkv[3] = {ctx->dkx, ctx->dky, ctx->dkz};
-    for (int d = 0; d < 3; d++) {
-        // particles_system_builder.hpp:175-189
-        p.width[d] = 4 * std::acos(0.0) / dkv[d];
-        p.h[d] = p.width[d] / double(ctx->n);
-    }
-    return p;
-}
-
 inline int ensure_work(bfpsh_ctx* ctx, int planes) {
     if (ctx->work_planes >= planes) return BFPSH_OK;
     if (ctx->work) {
@@ -183,9 +180,13 @@ struct OpsTable {
     int (*gradient_rspace)(bfpsh_ctx*);
     int (*spectral_velocity)(bfpsh_ctx*);
     int (*cospectrum)(bfpsh_ctx*, const void*, const void*, double*);
-    int (*particles_loop)(bfpsh_ctx*, double);
-    int (*sample)(bfpsh_ctx*, int, double*);
 };
+
+// tracer loop and sampling (particles.hip), dispatched on precision and
+// neighbour count
+int bfpsh_part_loop(bfpsh_ctx* ctx, double dt);
+int bfpsh_part_sample(bfpsh_ctx* ctx, int which, double* d_out);
+int bfpsh_part_release(bfpsh_ctx* ctx);
 
 #define BFPSH_FOR_SIZES(X) X(16) X(32) X(48) X(64) X(128) X(256) X(512) X(1024)
 #define BFPSH_DECL_OPS(N) const OpsTable* bfpsh_ops_f32_##N(); const OpsTable* bfpsh_ops_f64_##N();

@@ -46,21 +46,62 @@ constexpr double cos32(int q) {
 }
 constexpr double sin32(int q) { return cos32(8 - q); }
 
-// multiply by exp(SIGN 2 pi i M / R), R | 32, M a compile-time constant
+// cos / sin of 2 pi q / R at compile time (Taylor series in long double on
+// the reduced angle; exact to double precision)
+constexpr long double kPiL = 3.14159265358979323846264338327950288L;
+constexpr long double taylor_sin(long double x) {
+    long double term = x, sum = x;
+    for (int k = 1; k < 30; k++) {
+        term *= -x * x / ((2 * k) * (2 * k + 1));
+        sum += term;
+    }
+    return sum;
+}
+constexpr long double taylor_cos(long double x) {
+    long double term = 1, sum = 1;
+    for (int k = 1; k < 30; k++) {
+        term *= -x * x / ((2 * k - 1) * (2 * k));
+        sum += term;
+    }
+    return sum;
+}
+// angle 2 pi q / R with q reduced to [0, R): use the octant symmetry of the
+// unit circle so the series argument stays within [0, pi/4]
+constexpr double cos_frac(int q, int R) {
+    q = ((q % R) + R) % R;
+    // 8 q / R in [0, 8): octant index and remainder
+    const long double t = (long double)(8 * q) / R;
+    const int o = int(t);
+    const long double a = (t - o) * kPiL / 4;  // angle within the octant
+    const long double b = kPiL / 4 - a;
+    switch (o) {
+    case 0: return double(taylor_cos(a));
+    case 1: return double(taylor_sin(b));
+    case 2: return double(-taylor_sin(a));
+    case 3: return double(-taylor_cos(b));
+    case 4: return double(-taylor_cos(a));
+    case 5: return double(-taylor_sin(b));
+    case 6: return double(taylor_sin(a));
+    default: return double(taylor_cos(b));
+    }
+}
+constexpr double sin_frac(int q, int R) { return cos_frac(4 * q - R, 4 * R); }
+
+// multiply by exp(SIGN 2 pi i M / R), M a compile-time constant
 template <typename T, int R, int M, int SIGN>
 BFPSH_DEV C2<T> twc(C2<T> a) {
-    constexpr int q = ((M % R) * (32 / R)) % 32;
+    constexpr int q = ((M % R) + R) % R;
     if constexpr (q == 0) {
         return a;
-    } else if constexpr (q == 8) {
+    } else if constexpr (4 * q == R) {
         return cmul_i<T, SIGN>(a);
-    } else if constexpr (q == 16) {
+    } else if constexpr (2 * q == R) {
         return mk<T>(-a.x, -a.y);
-    } else if constexpr (q == 24) {
+    } else if constexpr (4 * q == 3 * R) {
         return cmul_i<T, -SIGN>(a);
     } else {
-        constexpr T c = T(cos32(q));
-        constexpr T s = T(SIGN * sin32(q));
+        constexpr T c = T(cos_frac(q, R));
+        constexpr T s = T(SIGN * sin_frac(q, R));
         return mk<T>(a.x * c - a.y * s, a.x * s + a.y * c);
     }
 }
@@ -124,6 +165,15 @@ template <typename T, int SIGN> struct Dft<T, 16, SIGN> {
 template <typename T, int SIGN> struct Dft<T, 32, SIGN> {
     static BFPSH_DEV void run(C2<T>* a) { dft_compose<T, 4, 8, SIGN>(a); }
 };
+template <typename T, int SIGN> struct Dft<T, 6, SIGN> {
+    static BFPSH_DEV void run(C2<T>* a) { dft_compose<T, 2, 3, SIGN>(a); }
+};
+template <typename T, int SIGN> struct Dft<T, 12, SIGN> {
+    static BFPSH_DEV void run(C2<T>* a) { dft_compose<T, 4, 3, SIGN>(a); }
+};
+template <typename T, int SIGN> struct Dft<T, 24, SIGN> {
+    static BFPSH_DEV void run(C2<T>* a) { dft_compose<T, 8, 3, SIGN>(a); }
+};
 
 // ---------------------------------------------------------------- plans
 constexpr int ilog2c(int n) {
@@ -148,6 +198,36 @@ constexpr PlanInfo make_plan(int N, int RMAX) {
     for (int s = 0; s < n3; s++) p.r[p.ns++] = 3;
     return p;
 }
+// radices for the register-resident FFT: every radix divides E (each thread
+// then owns whole butterflies at every stage), largest first
+constexpr PlanInfo make_plan_e(int N, int E) {
+    PlanInfo p{};
+    p.ns = 0;
+    const int allowed[9] = {32, 24, 16, 12, 8, 6, 4, 3, 2};
+    int m = N;
+    while (m > 1 && p.ns < 12) {
+        int pick = 0;
+        for (int i = 0; i < 9; i++)
+            if (E % allowed[i] == 0 && m % allowed[i] == 0) { pick = allowed[i]; break; }
+        if (pick == 0) { p.ns = -1; return p; }
+        p.r[p.ns++] = pick;
+        m /= pick;
+    }
+    return p;
+}
+template <int N, int E>
+struct PlanE {
+    static constexpr PlanInfo info = make_plan_e(N, E);
+    static_assert(info.ns > 0, "no radix plan for this length / register count");
+    static constexpr int nstages = info.ns;
+    static constexpr int radix(int s) { return info.r[s]; }
+    static constexpr int sublen(int s) {
+        int l = 1;
+        for (int i = 0; i < s; i++) l *= info.r[i];
+        return l;
+    }
+};
+
 template <int N, int RMAX>
 struct Plan {
     static constexpr PlanInfo info = make_plan(N, RMAX);
@@ -216,5 +296,82 @@ BFPSH_DEV void fft_tile(C2<T>* lds, const C2<T>* __restrict__ tw, int tid) {
         stockham_stage<T, N, B, LS, NT, R, L, SIGN>(lds, tw, tid);
     });
 }
+
+}  // namespace bfpsh
+
+namespace bfpsh {
+
+// ---------------------------------------------------------------- register-resident Stockham
+// Tile of B columns x N rows, NT threads: column b = t % B, group g = t / B
+// (G = NT / B groups per column).  Every thread holds E = N / G complex
+// values of its column in registers.  At stage s (radix R, sub-length L) a
+// thread owns butterflies j = g + i G, i < N / (R G), and its register
+// element e = i R + r is butterfly input r, i.e. row j + r N / R.  After the
+// stage the element holds output r at row (j / L) L R + j mod L + r L; the
+// rows are exchanged through the LDS tile (element (n, b) at lds[n LS + b])
+// between stages only.  Callers load stage-0 inputs (rows in_row(g, e)) and
+// read last-stage outputs (rows out_row(g, e)) directly from registers.
+template <typename T, int N, int B, int LS, int NT, int SIGN>
+struct RegFft {
+    static constexpr int G = NT / B;
+    static constexpr int E = N / G;
+    using P = PlanE<N, E>;
+    static constexpr int S = P::nstages;
+    static_assert(NT % B == 0 && N % G == 0, "tile shape");
+
+    static constexpr int per(int s) { return N / (P::radix(s) * G); }
+
+    // row of register element e at the input of stage s
+    static BFPSH_DEV int in_row_s(int s, int g, int e) {
+        const int R = P::radix(s);
+        const int i = e / R, r = e % R;
+        return g + i * G + r * (N / R);
+    }
+    static BFPSH_DEV int out_row_s(int s, int g, int e) {
+        const int R = P::radix(s), L = P::sublen(s);
+        const int i = e / R, r = e % R;
+        const int j = g + i * G;
+        return (j / L) * L * R + j % L + r * L;
+    }
+    static BFPSH_DEV int in_row(int g, int e) { return in_row_s(0, g, e); }
+    static BFPSH_DEV int out_row(int g, int e) { return out_row_s(S - 1, g, e); }
+
+    template <int s>
+    static BFPSH_DEV void compute_stage(C2<T> (&v)[E], const C2<T>* __restrict__ tw, int g) {
+        constexpr int R = P::radix(s), L = P::sublen(s), PER = N / (R * G);
+        static_assert(PER >= 1, "radix larger than the register set");
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            C2<T>* a = v + i * R;
+            if constexpr (L > 1) {
+                const int k = (g + i * G) % L;
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    C2<T> w = tw[r * k * (N / (L * R))];
+                    if constexpr (SIGN > 0) w.y = -w.y;
+                    a[r] = cmul<T>(a[r], w);
+                }
+            }
+            Dft<T, R, SIGN>::run(a);
+        }
+    }
+
+    // v: stage-0 inputs on entry, last-stage outputs on exit.  The LDS tile
+    // is free again when this returns (trailing barrier included only when
+    // an exchange happened).
+    static BFPSH_DEV void run(C2<T> (&v)[E], C2<T>* lds, const C2<T>* __restrict__ tw, int g, int b) {
+        sfor<0, S>([&](auto s) {
+            compute_stage<s>(v, tw, g);
+            if constexpr (s + 1 < S) {
+#pragma unroll
+                for (int e = 0; e < E; e++) lds[out_row_s(s, g, e) * LS + b] = v[e];
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < E; e++) v[e] = lds[in_row_s(s + 1, g, e) * LS + b];
+                __syncthreads();
+            }
+        });
+    }
+};
 
 }  // namespace bfpsh

@@ -10,6 +10,17 @@ struct Impl {
     static constexpr int B = CF::B, NT = CF::NT, NXH = CF::NXH;
     typedef C2<T> CT;
 
+    // Biot-Savart + first inverse pass: kx tile 0 with the symmetrize
+    // substitution, the remaining tiles without
+    template <int MODE>
+    static int bs_pass(bfpsh_ctx* ctx, const CT* src, CT* w, const CT* tw, const Geom& g, const char* name) {
+        LAUNCH(name, (k_inv_z_bs<T, N, MODE, true><<<dim3(1, N), CF::NT_BS, 0, ctx->stream>>>(src, w, tw, g, 0)));
+        if (NXH / B > 1)
+            LAUNCH(name, (k_inv_z_bs<T, N, MODE, false><<<dim3(NXH / B - 1, N), CF::NT_BS, 0, ctx->stream>>>(
+                              src, w, tw, g, 1)));
+        return BFPSH_OK;
+    }
+
     static int stage(bfpsh_ctx* ctx, int s, double dt, T* ru) {
         const Geom g = geom(ctx);
         StepParams sp = ctx->sp;
@@ -21,7 +32,10 @@ struct Impl {
         const CT* tw = (const CT*)ctx->tw;
         const CT* src = s == 1 ? v0 : (s == 2 ? v1 : v2);
         const dim3 tiles(NXH / B, N);
-        LAUNCH("inv_z_bs", (k_inv_z_bs<T, N, 0><<<tiles, NT, 0, ctx->stream>>>(src, w, tw, g)));
+        {
+            int r = bs_pass<0>(ctx, src, w, tw, g, "inv_z_bs");
+            if (r) return r;
+        }
         LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 6), NT, 0, ctx->stream>>>(
                              w, size_t(N) * NXH, size_t(NXH), tw)));
         LAUNCH("rows_cross", (k_rows_cross<T, N><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(
@@ -49,7 +63,7 @@ struct Impl {
         r = stage(ctx, 1, dt, ru);
         if (r) return r;
         if (particles) {
-            r = particles_loop(ctx, dt);
+            r = bfpsh_part_loop(ctx, dt);
             if (r) return r;
         }
         r = stage(ctx, 2, dt, nullptr);
@@ -86,7 +100,8 @@ struct Impl {
         if (r) return r;
         CT* w = (CT*)ctx->work;
         const CT* tw = (const CT*)ctx->tw;
-        LAUNCH("inv_z_bs_u", (k_inv_z_bs<T, N, 1><<<dim3(NXH / B, N), NT, 0, ctx->stream>>>((const CT*)ctx->v0, w, tw, geom(ctx))));
+        r = bs_pass<1>(ctx, (const CT*)ctx->v0, w, tw, geom(ctx), "inv_z_bs_u");
+        if (r) return r;
         LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 3), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw)));
         LAUNCH("rows_c2r3", (k_rows_c2r<T, N, 3><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->ru, tw)));
         ctx->ru_valid = true;
@@ -100,7 +115,8 @@ struct Impl {
         if (r) return r;
         CT* w = (CT*)ctx->work;
         const CT* tw = (const CT*)ctx->tw;
-        LAUNCH("inv_z_bs_grad", (k_inv_z_bs<T, N, 2><<<dim3(NXH / B, N), NT, 0, ctx->stream>>>((const CT*)ctx->v0, w, tw, geom(ctx))));
+        r = bs_pass<2>(ctx, (const CT*)ctx->v0, w, tw, geom(ctx), "inv_z_bs_grad");
+        if (r) return r;
         LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 9), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw)));
         LAUNCH("rows_c2r9", (k_rows_c2r<T, N, 9><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->rgrad, tw)));
         return BFPSH_OK;
@@ -128,63 +144,5 @@ struct Impl {
         return BFPSH_OK;
     }
 
-    template <int NB>
-    static int interp_ab_nb(bfpsh_ctx* ctx, double dt, int order) {
-        const int nthreads = 256;
-        const int64_t nblocks = (ctx->np + nthreads - 1) / nthreads;
-        const size_t lds = size_t(3) * (2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
-        LAUNCH("interp_ab", (k_interp_ab<T, NB><<<dim3(unsigned(nblocks)), nthreads, lds, ctx->stream>>>(
-                                (const T*)ctx->ru, ctx->d_pos, ctx->d_rhs, ctx->np, ctx->nsteps, ctx->head,
-                                order, ctx->add_head ? 1 : 0, dt, ctx->d_tab, ctx->deg, pgeom(ctx))));
-        return BFPSH_OK;
-    }
-    template <int NB, int NC>
-    static int sample_nb(bfpsh_ctx* ctx, const void* f, double* d_out) {
-        const int nthreads = 256;
-        const int64_t nblocks = (ctx->np + nthreads - 1) / nthreads;
-        const size_t lds = size_t(3) * (2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
-        LAUNCH("sample", (k_sample<T, NB, NC><<<dim3(unsigned(nblocks)), nthreads, lds, ctx->stream>>>(
-                             (const T*)f, ctx->d_pos, d_out, ctx->np, ctx->d_tab, ctx->deg, pgeom(ctx))));
-        return BFPSH_OK;
-    }
-
-    static int particles_loop(bfpsh_ctx* ctx, double dt) {
-        if (ctx->np == 0) return BFPSH_OK;
-        const int order = std::min(ctx->step_idx, ctx->nsteps);
-        int r;
-        switch (ctx->pn) {
-        case 1: r = interp_ab_nb<1>(ctx, dt, order); break;
-        case 2: r = interp_ab_nb<2>(ctx, dt, order); break;
-        case 3: r = interp_ab_nb<3>(ctx, dt, order); break;
-        case 4: r = interp_ab_nb<4>(ctx, dt, order); break;
-        case 5: r = interp_ab_nb<5>(ctx, dt, order); break;
-        case 6: r = interp_ab_nb<6>(ctx, dt, order); break;
-        case 7: r = interp_ab_nb<7>(ctx, dt, order); break;
-        case 8: r = interp_ab_nb<8>(ctx, dt, order); break;
-        case 9: r = interp_ab_nb<9>(ctx, dt, order); break;
-        default: r = interp_ab_nb<10>(ctx, dt, order); break;
-        }
-        if (r) return r;
-        // inc_step_idx + shift_rhs_vectors: the oldest slot becomes the (zero) head
-        ctx->step_idx += 1;
-        ctx->head = (ctx->head + ctx->nsteps - 1) % ctx->nsteps;
-        ctx->head_zero = true;
-        ctx->add_head = false;
-        return BFPSH_OK;
-    }
-
-    static int sample(bfpsh_ctx* ctx, int which, double* d_out) {
-        const void* f = which == BFPSH_RVELOCITY ? ctx->ru : ctx->rgrad;
-        const bool three = which == BFPSH_RVELOCITY;
-        switch (ctx->pn) {
-#define BFPSH_SAMPLE_CASE(k) \
-        case k: return three ? sample_nb<k, 3>(ctx, f, d_out) : sample_nb<k, 9>(ctx, f, d_out);
-        BFPSH_SAMPLE_CASE(1) BFPSH_SAMPLE_CASE(2) BFPSH_SAMPLE_CASE(3) BFPSH_SAMPLE_CASE(4)
-        BFPSH_SAMPLE_CASE(5) BFPSH_SAMPLE_CASE(6) BFPSH_SAMPLE_CASE(7) BFPSH_SAMPLE_CASE(8)
-        BFPSH_SAMPLE_CASE(9) BFPSH_SAMPLE_CASE(10)
-#undef BFPSH_SAMPLE_CASE
-        }
-        return fail(ctx, BFPSH_EINVAL, "bad neighbour count");
-    }
 };
 

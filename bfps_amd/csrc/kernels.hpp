@@ -17,25 +17,43 @@
 #pragma once
 #include "fft.hpp"
 
+#ifndef BFPSH_TILE_BYTES
+#define BFPSH_TILE_BYTES 128
+#endif
+
 namespace bfpsh {
 
 // ------------------------------------------------------------ configuration
 template <typename T, int N>
 struct Cfg {
     static constexpr int NXH = N / 2;
-    static constexpr int RMAX = 16;
+    static constexpr int RMAX = 16;  // radix cap of the LDS-tile FFT (row passes)
+    // strided passes: register-resident FFT, E complex values per thread and
+    // field (all radices divide E)
+    static constexpr int ereg() {
+        if (N % 3 == 0) return 12;
+        if (N <= 16) return N;
+        if (sizeof(T) == 4 && (N == 128 || N == 256 || N == 1024)) return 16;
+        return 8;
+    }
+    static constexpr int E = ereg();
+    static constexpr int G = N / E;  // thread groups per column
+    // columns per strided tile: BFPSH_TILE_BYTES-byte row segments (default
+    // 128), tile <= 64 KB of LDS
     static constexpr int bcap() {
-        int b = int(128 / sizeof(C2<T>));
+        int b = int(BFPSH_TILE_BYTES / sizeof(C2<T>));
+        const int lim = int(65536 / (N * sizeof(C2<T>)));
+        if (b > lim) b = lim;
         if (b > NXH) b = NXH;
         while (NXH % b) b /= 2;
         return b;
     }
-    static constexpr int B = bcap();  // kx columns per strided tile
-    static constexpr int NT = (N * B / 8) < 64 ? 64 : ((N * B / 8) > 1024 ? 1024 : (N * B / 8));
-    static constexpr int E = (N * B + NT - 1) / NT;  // tile elements per thread
+    static constexpr int B = bcap();
+    static constexpr int NT = G * B;
+    static constexpr int NT_BS = NT;
     static constexpr int RP = (N * int(sizeof(C2<T>)) >= 8192) ? 1 : 2;  // row pairs per row block
     static constexpr int NT_ROW = N >= 256 ? 256 : (N >= 64 ? 128 : 64);
-    static_assert(NT % B == 0, "tile threads must cover whole columns");
+    static_assert(NT <= 1024, "tile threads");
 };
 
 // ------------------------------------------------------------ parameters
@@ -63,77 +81,70 @@ BFPSH_DEV double kwave(int i, int n, double dk) { return (i <= n / 2 ? i : i - n
 // conjugated mirror mode; compute_gradient, bfps/cpp/field.cpp:1077-1097).
 template <int MODE> struct BsOut { static constexpr int NF = MODE == 0 ? 6 : (MODE == 1 ? 3 : 9); };
 
-template <typename T, int N, int MODE>
+// MIRROR: the launch covers the kx tile holding kx = 0 and applies the
+// symmetrize substitution there; the other tiles use MIRROR = false
+// (blockIdx.x offset by tile0) and carry no mirror code.
+template <typename T, int N, int MODE, bool MIRROR>
 __global__ void __launch_bounds__((Cfg<T, N>::NT))
-k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __restrict__ tw, Geom g) {
+k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __restrict__ tw, Geom g, int tile0) {
     using CF = Cfg<T, N>;
     constexpr int B = CF::B, NT = CF::NT, E = CF::E, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
     constexpr int NF = BsOut<MODE>::NF;
+    using F = RegFft<T, N, B, B, NT, +1>;
     __shared__ C2<T> lds[N * B];
     const int tid = threadIdx.x;
-    const int kx0 = blockIdx.x * B;
+    const int b = tid % B, grp = tid / B;
+    const int kxi = (blockIdx.x + tile0) * B + b;
     const int ky = blockIdx.y;
-    const int b = tid % B;
-    const int kxi = kx0 + b;
     const double kx = kxi * g.dkx, kyv = kwave(ky, N, g.dky);
     C2<T> wr[3][E];
 #pragma unroll
-    for (int i = 0; i < E; i++) {
-        const int n = tid / B + i * (NT / B);
-        if (n < N) {
-            const size_t off = (size_t(ky) * N + n) * NXH + kxi;
+    for (int e = 0; e < E; e++) {
+        const int n = F::in_row(grp, e);
+        const size_t off = (size_t(ky) * N + n) * NXH + kxi;
 #pragma unroll
-            for (int c = 0; c < 3; c++) wr[c][i] = w[c * PS + off];
-        }
+        for (int c = 0; c < 3; c++) wr[c][e] = w[c * PS + off];
     }
+#pragma unroll 1
     for (int f = 0; f < NF; f++) {
+        C2<T> v[E];
 #pragma unroll
-        for (int i = 0; i < E; i++) {
-            const int n = tid / B + i * (NT / B);
-            if (n < N) {
-                const double kz = kwave(n, N, g.dkz);
-                const double k2 = kx * kx + kyv * kyv + kz * kz;
-                C2<T> val;
-                if (MODE == 0 && f >= 3) {
-                    val = wr[f - 3][i];
-                } else {
-                    const int cu = MODE == 2 ? f % 3 : f;
-                    const bool keep = MODE == 2 ? (k2 < g.kM2) : true;
-                    C2<T> a0 = wr[0][i], a1 = wr[1][i], a2 = wr[2][i];
-                    // symmetrized kx = 0 plane: lower half takes conj(mirror)
-                    if (kxi == 0 && ((ky > N / 2) || (ky == 0 && n > N / 2))) {
-                        const size_t moff = (size_t((N - ky) % N) * N + (N - n) % N) * NXH;
-                        a0 = cconj<T>(w[moff]);
-                        a1 = cconj<T>(w[PS + moff]);
-                        a2 = cconj<T>(w[2 * PS + moff]);
-                    }
-                    C2<T> u = mk<T>(T(0), T(0));
-                    if (k2 <= g.kM2 && k2 > 0) {
-                        if (cu == 0) u = mk<T>(T(-(kyv * a2.y - kz * a1.y) / k2), T((kyv * a2.x - kz * a1.x) / k2));
-                        if (cu == 1) u = mk<T>(T(-(kz * a0.y - kx * a2.y) / k2), T((kz * a0.x - kx * a2.x) / k2));
-                        if (cu == 2) u = mk<T>(T(-(kx * a1.y - kyv * a0.y) / k2), T((kx * a1.x - kyv * a0.x) / k2));
-                    }
-                    if (MODE == 2) {
-                        const int j = f / 3;
-                        const double kj = j == 0 ? kx : (j == 1 ? kyv : kz);
-                        val = keep ? mk<T>(T(-kj * u.y), T(kj * u.x)) : mk<T>(T(0), T(0));
-                    } else {
-                        val = u;
-                    }
-                }
-                lds[n * B + b] = val;
+        for (int e = 0; e < E; e++) {
+            const int n = F::in_row(grp, e);
+            if (MODE == 0 && f >= 3) {
+                v[e] = wr[f - 3][e];
+                continue;
+            }
+            const double kz = kwave(n, N, g.dkz);
+            const double k2 = kx * kx + kyv * kyv + kz * kz;
+            const int cu = MODE == 2 ? f % 3 : f;
+            C2<T> a0 = wr[0][e], a1 = wr[1][e], a2 = wr[2][e];
+            // symmetrized kx = 0 plane: the lower half takes conj(mirror)
+            if (MIRROR && kxi == 0 && ((ky > N / 2) || (ky == 0 && n > N / 2))) {
+                const size_t moff = (size_t((N - ky) % N) * N + (N - n) % N) * NXH;
+                a0 = cconj<T>(w[moff]);
+                a1 = cconj<T>(w[PS + moff]);
+                a2 = cconj<T>(w[2 * PS + moff]);
+            }
+            C2<T> u = mk<T>(T(0), T(0));
+            if (k2 <= g.kM2 && k2 > 0) {
+                if (cu == 0) u = mk<T>(T(-(kyv * a2.y - kz * a1.y) / k2), T((kyv * a2.x - kz * a1.x) / k2));
+                if (cu == 1) u = mk<T>(T(-(kz * a0.y - kx * a2.y) / k2), T((kz * a0.x - kx * a2.x) / k2));
+                if (cu == 2) u = mk<T>(T(-(kx * a1.y - kyv * a0.y) / k2), T((kx * a1.x - kyv * a0.x) / k2));
+            }
+            if (MODE == 2) {
+                const int j = f / 3;
+                const double kj = j == 0 ? kx : (j == 1 ? kyv : kz);
+                v[e] = k2 < g.kM2 ? mk<T>(T(-kj * u.y), T(kj * u.x)) : mk<T>(T(0), T(0));
+            } else {
+                v[e] = u;
             }
         }
-        __syncthreads();
-        fft_tile<T, N, B, B, NT, CF::RMAX, +1>(lds, tw, tid);
+        F::run(v, lds, tw, grp, b);
         C2<T>* o = out + size_t(f) * PS;
 #pragma unroll
-        for (int i = 0; i < E; i++) {
-            const int n = tid / B + i * (NT / B);
-            if (n < N) o[(size_t(ky) * N + n) * NXH + kxi] = lds[n * B + b];
-        }
-        __syncthreads();
+        for (int e = 0; e < E; e++) o[(size_t(ky) * N + F::out_row(grp, e)) * NXH + kxi] = v[e];
     }
 }
 
@@ -146,22 +157,17 @@ k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __
     using CF = Cfg<T, N>;
     constexpr int B = CF::B, NT = CF::NT, E = CF::E, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
+    using F = RegFft<T, N, B, B, NT, SIGN>;
     __shared__ C2<T> lds[N * B];
     const int tid = threadIdx.x;
-    const int b = tid % B;
+    const int b = tid % B, grp = tid / B;
     C2<T>* base = data + blockIdx.z * PS + blockIdx.y * ostride + blockIdx.x * B + b;
+    C2<T> v[E];
 #pragma unroll
-    for (int i = 0; i < E; i++) {
-        const int n = tid / B + i * (NT / B);
-        if (n < N) lds[n * B + b] = base[n * estride];
-    }
-    __syncthreads();
-    fft_tile<T, N, B, B, NT, CF::RMAX, SIGN>(lds, tw, tid);
+    for (int e = 0; e < E; e++) v[e] = base[F::in_row(grp, e) * estride];
+    F::run(v, lds, tw, grp, b);
 #pragma unroll
-    for (int i = 0; i < E; i++) {
-        const int n = tid / B + i * (NT / B);
-        if (n < N) base[n * estride] = lds[n * B + b];
-    }
+    for (int e = 0; e < E; e++) base[F::out_row(grp, e) * estride] = v[e];
 }
 
 // ------------------------------------------------------------ x passes on row pairs
@@ -171,18 +177,32 @@ k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __
 // Z[N-k] = conj(A[k]) + i conj(B[k]); z = IDFT(Z); row z = Re z, row z+1 = Im z.
 // Forward: Z = DFT(a + i b); A[k] = (Z[k] + conj Z[N-k]) / 2,
 // B[k] = (Z[k] - conj Z[N-k]) / (2 i).
-template <typename T, int N, int LS>
-BFPSH_DEV void rows_unpack_inverse(C2<T>* lds, int col, const C2<T>* __restrict__ ra,
-                                   const C2<T>* __restrict__ rb, int tid, int nt) {
+// NCOL columns at once: all global loads of a thread are issued before its
+// LDS stores.  Column col reads rows ra(col), rb(col) = base + off[col].
+template <typename T, int N, int LS, int NCOL, int NT>
+BFPSH_DEV void rows_unpack_inverse(C2<T>* lds, const C2<T>* __restrict__ const (&ra)[NCOL],
+                                   const C2<T>* __restrict__ const (&rb)[NCOL], int tid) {
     constexpr int H = N / 2;
-    for (int k = tid; k < H; k += nt) {
-        C2<T> A = ra[k], Bv = rb[k];
-        if (k == 0) {
-            lds[col] = mk<T>(A.x, Bv.x);
-            lds[H * LS + col] = mk<T>(T(0), T(0));
-        } else {
-            lds[k * LS + col] = mk<T>(A.x - Bv.y, A.y + Bv.x);
-            lds[(N - k) * LS + col] = mk<T>(A.x + Bv.y, Bv.x - A.y);
+    constexpr int PER = (H + NT - 1) / NT;
+#pragma unroll
+    for (int it = 0; it < PER; it++) {
+        const int k = tid + it * NT;
+        if (H % NT != 0 && k >= H) break;
+        C2<T> A[NCOL], Bv[NCOL];
+#pragma unroll
+        for (int col = 0; col < NCOL; col++) {
+            A[col] = ra[col][k];
+            Bv[col] = rb[col][k];
+        }
+#pragma unroll
+        for (int col = 0; col < NCOL; col++) {
+            if (k == 0) {
+                lds[col] = mk<T>(A[col].x, Bv[col].x);
+                lds[H * LS + col] = mk<T>(T(0), T(0));
+            } else {
+                lds[k * LS + col] = mk<T>(A[col].x - Bv[col].y, A[col].y + Bv[col].x);
+                lds[(N - k) * LS + col] = mk<T>(A[col].x + Bv[col].y, Bv[col].x - A[col].y);
+            }
         }
     }
 }
@@ -212,12 +232,19 @@ k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restri
     __shared__ C2<T> lds[N * LS];
     const int tid = threadIdx.x;
     const int q0 = blockIdx.x * RP;  // first row pair; rows (y, z) flattened: row = y*N + z
-    for (int c = 0; c < 6; c++)
-        for (int p = 0; p < RP; p++) {
-            const size_t row = size_t(q0 + p) * 2;
-            rows_unpack_inverse<T, N, LS>(lds, c * RP + p, data + c * PS + row * NXH,
-                                          data + c * PS + (row + 1) * NXH, tid, NT);
-        }
+    {
+        const C2<T>* ra[6 * RP];
+        const C2<T>* rb[6 * RP];
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+#pragma unroll
+            for (int p = 0; p < RP; p++) {
+                const size_t row = size_t(q0 + p) * 2;
+                ra[c * RP + p] = data + c * PS + row * NXH;
+                rb[c * RP + p] = data + c * PS + (row + 1) * NXH;
+            }
+        rows_unpack_inverse<T, N, LS, 6 * RP, NT>(lds, ra, rb, tid);
+    }
     __syncthreads();
     fft_tile<T, N, LS, LS, NT, CF::RMAX, +1>(lds, tw, tid);
     for (int e = tid; e < N * RP; e += NT) {
@@ -265,12 +292,19 @@ k_rows_c2r(const C2<T>* __restrict__ data, T* __restrict__ out, const C2<T>* __r
     const int tid = threadIdx.x;
     const int q0 = blockIdx.x * RP;
     for (int c0 = 0; c0 < NC; c0 += 3) {
-        for (int c = 0; c < 3; c++)
-            for (int p = 0; p < RP; p++) {
-                const size_t row = size_t(q0 + p) * 2;
-                rows_unpack_inverse<T, N, LS>(lds, c * RP + p, data + (c0 + c) * PS + row * NXH,
-                                              data + (c0 + c) * PS + (row + 1) * NXH, tid, NT);
-            }
+        {
+            const C2<T>* ra[3 * RP];
+            const C2<T>* rb[3 * RP];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int p = 0; p < RP; p++) {
+                    const size_t row = size_t(q0 + p) * 2;
+                    ra[c * RP + p] = data + (c0 + c) * PS + row * NXH;
+                    rb[c * RP + p] = data + (c0 + c) * PS + (row + 1) * NXH;
+                }
+            rows_unpack_inverse<T, N, LS, 3 * RP, NT>(lds, ra, rb, tid);
+        }
         __syncthreads();
         fft_tile<T, N, LS, LS, NT, CF::RMAX, +1>(lds, tw, tid);
         for (int e = tid; e < N * LS; e += NT) {
@@ -300,33 +334,25 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
     using CF = Cfg<T, N>;
     constexpr int B = CF::B, NT = CF::NT, E = CF::E, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
+    using F = RegFft<T, N, B, B, NT, -1>;
     __shared__ C2<T> lds[N * B];
     const int tid = threadIdx.x;
-    const int b = tid % B;
+    const int b = tid % B, grp = tid / B;
     const int kxi = blockIdx.x * B + b;
     const int kz = blockIdx.y;
     C2<T> res[3][E];
+#pragma unroll
     for (int c = 0; c < 3; c++) {
         const C2<T>* src = data + c * PS + size_t(kz) * NXH + kxi;
 #pragma unroll
-        for (int i = 0; i < E; i++) {
-            const int n = tid / B + i * (NT / B);
-            if (n < N) lds[n * B + b] = src[size_t(n) * N * NXH];
-        }
-        __syncthreads();
-        fft_tile<T, N, B, B, NT, CF::RMAX, -1>(lds, tw, tid);
-#pragma unroll
-        for (int i = 0; i < E; i++) {
-            const int n = tid / B + i * (NT / B);
-            if (n < N) res[c][i] = lds[n * B + b];
-        }
-        __syncthreads();
+        for (int e = 0; e < E; e++) res[c][e] = src[size_t(F::in_row(grp, e)) * N * NXH];
     }
+#pragma unroll
+    for (int c = 0; c < 3; c++) F::run(res[c], lds, tw, grp, b);
     const double kx = kxi * g.dkx, kzv = kwave(kz, N, g.dkz);
 #pragma unroll
     for (int i = 0; i < E; i++) {
-        const int ky = tid / B + i * (NT / B);
-        if (ky >= N) continue;
+        const int ky = F::out_row(grp, i);
         const double kyv = kwave(ky, N, g.dky);
         const double k2 = kx * kx + kyv * kyv + kzv * kzv;
         const size_t off = (size_t(ky) * N + kz) * NXH + kxi;
@@ -525,140 +551,6 @@ __global__ void k_cospectrum(const C2<T>* __restrict__ a, const C2<T>* __restric
     __syncthreads();
     for (int i = threadIdx.x; i < nshells * 9; i += blockDim.x)
         if (acc[i] != 0.0) atomicAdd(&out[i], acc[i]);
-}
-
-// ------------------------------------------------------------ tracers
-// Interpolation (particles_field_computer::apply_computation,
-// bfps/cpp/particles/particles_field_computer.hpp:65-157, one rank):
-// per axis, cell = floor(pos / h) mod N, x = position inside the cell in
-// [0,1) (get_norm_pos_in_cell, :55-63), weights beta_j(x), j = 0..2n+1 for
-// nodes cell-n .. cell+n+1 (wrapped); sum over z, x, y of
-// f(x,y,z) * ((bz * by) * bx) in double.  Field planes are [y][z][x].
-struct PartGeom {
-    int n;            // grid size
-    double width[3];  // box widths 2 pi / dk
-    double h[3];      // grid spacing width / N
-};
-
-template <int NB>
-BFPSH_DEV void interp_setup(const double* pos, const double* __restrict__ tab, int deg, const PartGeom& pg,
-                            double (&bw)[3][2 * NB + 2], int (&cell)[3]) {
-    constexpr int W = 2 * NB + 2;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double s = pos[d];
-        const double rep = floor(s / pg.width[d]);
-        const double inbox = s - rep * pg.width[d];
-        const double ci = floor(inbox / pg.h[d]);
-        const double xr = (inbox - ci * pg.h[d]) / pg.h[d];
-#pragma unroll
-        for (int j = 0; j < W; j++) {
-            double acc = 0.0;
-            for (int q = deg; q >= 0; q--) acc = acc * xr + tab[j * (deg + 1) + q];
-            bw[d][j] = acc;
-        }
-        const int lvl = int(floor(s / pg.h[d]));
-        cell[d] = ((lvl % pg.n) + pg.n) % pg.n;
-    }
-}
-
-template <typename T, int NB, int NC>
-BFPSH_DEV void interp_gather(const T* __restrict__ f, const PartGeom& pg, const double (&bw)[3][2 * NB + 2],
-                             const int (&cell)[3], double (&acc)[NC]) {
-    constexpr int W = 2 * NB + 2;
-    const int n = pg.n;
-    const size_t ps = size_t(n) * n * n;
-#pragma unroll
-    for (int c = 0; c < NC; c++) acc[c] = 0.0;
-    for (int jz = 0; jz < W; jz++) {
-        int zz = cell[2] - NB + jz;
-        zz = zz < 0 ? zz + n : (zz >= n ? zz - n : zz);
-        for (int jx = 0; jx < W; jx++) {
-            int xx = cell[0] - NB + jx;
-            xx = xx < 0 ? xx + n : (xx >= n ? xx - n : xx);
-            for (int jy = 0; jy < W; jy++) {
-                int yy = cell[1] - NB + jy;
-                yy = yy < 0 ? yy + n : (yy >= n ? yy - n : yy);
-                const double coef = bw[2][jz] * bw[1][jy] * bw[0][jx];
-                const size_t ri = (size_t(yy) * n + zz) * n + xx;
-#pragma unroll
-                for (int c = 0; c < NC; c++) acc[c] += double(f[c * ps + ri]) * coef;
-            }
-        }
-    }
-}
-
-// interpolate velocity into the head of the rhs ring, then Adams-Bashforth
-// (particles_adams_bashforth.hpp:41-131) with order = min(step_idx, nsteps)
-// (particles_system.hpp:173-178).  rhs slot of history index i is
-// (head + i) mod nsteps.  add_head: the head slot held a caller-supplied
-// value to accumulate onto (first loop after a put only).
-template <typename T, int NB>
-__global__ void k_interp_ab(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ rhs,
-                            int64_t np, int nsteps, int head, int order, int add_head, double dt,
-                            const double* __restrict__ tab, int deg, PartGeom pg) {
-    constexpr int W = 2 * NB + 2;
-    extern __shared__ double stab[];
-    const int tabsize = 3 * W * (deg + 1);
-    for (int i = threadIdx.x; i < tabsize; i += blockDim.x) stab[i] = tab[i];
-    __syncthreads();
-    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    double x[3] = {pos[p * 3 + 0], pos[p * 3 + 1], pos[p * 3 + 2]};
-    double bw[3][W];
-    int cell[3];
-    interp_setup<NB>(x, stab, deg, pg, bw, cell);
-    double r0[3];
-    interp_gather<T, NB, 3>(f, pg, bw, cell, r0);
-    double* h0 = rhs + size_t(head) * np * 3 + p * 3;
-    if (add_head) {
-        r0[0] += h0[0];
-        r0[1] += h0[1];
-        r0[2] += h0[2];
-    }
-    h0[0] = r0[0];
-    h0[1] = r0[1];
-    h0[2] = r0[2];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        double r[6];
-        r[0] = r0[d];
-        for (int i = 1; i < order; i++) r[i] = rhs[size_t((head + i) % nsteps) * np * 3 + p * 3 + d];
-        double incr;
-        switch (order) {
-        case 1: incr = dt * r[0]; break;
-        case 2: incr = dt * (3. * r[0] - r[1]) / 2.; break;
-        case 3: incr = dt * (23. * r[0] - 16. * r[1] + 5. * r[2]) / 12.; break;
-        case 4: incr = dt * (55. * r[0] - 59. * r[1] + 37. * r[2] - 9. * r[3]) / 24.; break;
-        case 5: incr = dt * (1901. * r[0] - 2774. * r[1] + 2616. * r[2] - 1274. * r[3] + 251. * r[4]) / 720.; break;
-        default: incr = dt * (4277. * r[0] - 7923. * r[1] + 9982. * r[2] - 7298. * r[3] + 2877. * r[4] - 475. * r[5]) / 1440.; break;
-        }
-        x[d] += incr;
-    }
-    pos[p * 3 + 0] = x[0];
-    pos[p * 3 + 1] = x[1];
-    pos[p * 3 + 2] = x[2];
-}
-
-// sample an NC-component real field at the tracer positions
-template <typename T, int NB, int NC>
-__global__ void k_sample(const T* __restrict__ f, const double* __restrict__ pos, double* __restrict__ out,
-                         int64_t np, const double* __restrict__ tab, int deg, PartGeom pg) {
-    constexpr int W = 2 * NB + 2;
-    extern __shared__ double stab[];
-    const int tabsize = 3 * W * (deg + 1);
-    for (int i = threadIdx.x; i < tabsize; i += blockDim.x) stab[i] = tab[i];
-    __syncthreads();
-    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    double x[3] = {pos[p * 3 + 0], pos[p * 3 + 1], pos[p * 3 + 2]};
-    double bw[3][W];
-    int cell[3];
-    interp_setup<NB>(x, stab, deg, pg, bw, cell);
-    double acc[NC];
-    interp_gather<T, NB, NC>(f, pg, bw, cell, acc);
-#pragma unroll
-    for (int c = 0; c < NC; c++) out[p * NC + c] = acc[c];
 }
 
 }  // namespace bfpsh

@@ -1,0 +1,633 @@
+// Tracer interpolation and Adams-Bashforth update (gfx950).
+//
+// Reference: particles_field_computer::apply_computation
+// (bfps/cpp/particles/particles_field_computer.hpp:55-183) on one rank,
+// particles_adams_bashforth::move_particles (particles_adams_bashforth.hpp:18-133),
+// particles_system::completeLoop (particles_system.hpp:173-217).
+//
+// Tracer data stay in index order in device memory.  Each loop first sorts a
+// permutation of the tracers by the Morton code of their 4^3-cell
+// micro-brick (the reference buckets by z-slab for the same reason,
+// particles_utils.hpp:106-165), so consecutive threads interpolate nearby
+// tracers and share stencil rows in L1/L2; blocks are mapped so that each
+// XCD receives a contiguous run of the sorted order.
+//
+// Per tracer and axis: cell = floor(pos / h) mod N, x = position in the cell
+// in [0, 1); weights beta_j(x), j = 0..2n+1, for nodes cell-n .. cell+n+1
+// (wrapped).  The field sum is taken row by row along x (contiguous in the
+// [y][z][x] real layout): acc += (bz * by) * sum_x f * bx, in double; the
+// reference sums f * ((bz * by) * bx) point by point, which differs only by
+// rounding.
+#include <hipcub/hipcub.hpp>
+
+#include "ctx.hpp"
+
+namespace {
+
+struct PartGeomD {
+    int n;
+    double width[3];  // box widths 2 pi / dk (particles_system_builder.hpp:175-177)
+    double h[3];      // grid spacings width / N (:187-189)
+};
+
+PartGeomD pgeom_d(const bfpsh_ctx* ctx) {
+    PartGeomD p;
+    p.n = ctx->n;
+    const double dkv[3] = {ctx->dkx, ctx->dky, ctx->dkz};
+    for (int d = 0; d < 3; d++) {
+        p.width[d] = 4 * std::acos(0.0) / dkv[d];
+        p.h[d] = p.width[d] / double(ctx->n);
+    }
+    return p;
+}
+
+// pbc_field_layer (particles_field_computer.hpp:176-183)
+__device__ __forceinline__ int cell_of(double s, double h, int n) {
+    const int lvl = int(floor(s / h));
+    return ((lvl % n) + n) % n;
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void k_part_keys(const double* __restrict__ pos, int64_t np, PartGeomD pg, int shift,
+                            uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int cx = cell_of(pos[p * 3 + 0], pg.h[0], pg.n) >> shift;
+    const int cy = cell_of(pos[p * 3 + 1], pg.h[1], pg.n) >> shift;
+    const int cz = cell_of(pos[p * 3 + 2], pg.h[2], pg.n) >> shift;
+    keys[p] = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+    vals[p] = uint32_t(p);
+}
+
+// logical block index so that XCD k (blocks dispatched round-robin over 8
+// XCDs) processes the k-th contiguous eighth of the sorted order
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t nb8 = gridDim.x / 8;
+    const int64_t b = blockIdx.x;
+    return (b % 8) * nb8 + b / 8;
+}
+
+template <int NB>
+__device__ __forceinline__ void setup_axis(double s, const double* __restrict__ tab, int deg, double width, double h,
+                                           int n, double (&bw)[2 * NB + 2], int (&idx)[2 * NB + 2]) {
+    constexpr int W = 2 * NB + 2;
+    // get_norm_pos_in_cell (particles_field_computer.hpp:55-63)
+    const double rep = floor(s / width);
+    const double inbox = s - rep * width;
+    const double ci = floor(inbox / h);
+    const double xr = (inbox - ci * h) / h;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        double acc = 0.0;
+        for (int q = deg; q >= 0; q--) acc = acc * xr + tab[j * (deg + 1) + q];
+        bw[j] = acc;
+    }
+    const int c = cell_of(s, h, n);
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        int v = c - NB + j;
+        v = v < 0 ? v + n : (v >= n ? v - n : v);
+        idx[j] = v;
+    }
+}
+
+// Lanes per tracer: the W = 2n+2 stencil columns along x are spread over LP
+// lanes (LP = 16 for n <= 7, else 32), so one wavefront load instruction
+// reads 64/LP contiguous row segments instead of 64 scattered words.
+template <int NB> struct Lanes { static constexpr int LP = (2 * NB + 2) <= 16 ? 16 : 32; };
+
+// sum over the stencil of one tracer, computed by the LP lanes of its group;
+// lane jx < W holds column jx.  Result valid in every lane of the group.
+template <typename T, int NB, int NC>
+__device__ __forceinline__ void gather_coop(const T* __restrict__ f, int n, const double* x,
+                                            const double* __restrict__ tab, int deg, const PartGeomD& pg,
+                                            int jx, double (&acc)[NC]) {
+    constexpr int W = 2 * NB + 2;
+    constexpr int LP = Lanes<NB>::LP;
+    double by[W], bz[W];
+    int iy[W], iz[W];
+    setup_axis<NB>(x[1], tab, deg, pg.width[1], pg.h[1], n, by, iy);
+    setup_axis<NB>(x[2], tab, deg, pg.width[2], pg.h[2], n, bz, iz);
+    // this lane's x column
+    double bxl = 0.0;
+    int ixl = 0;
+    {
+        const double s = x[0];
+        const double rep = floor(s / pg.width[0]);
+        const double inbox = s - rep * pg.width[0];
+        const double ci = floor(inbox / pg.h[0]);
+        const double xr = (inbox - ci * pg.h[0]) / pg.h[0];
+        const int jj = jx < W ? jx : 0;
+        double a = 0.0;
+        for (int q = deg; q >= 0; q--) a = a * xr + tab[jj * (deg + 1) + q];
+        bxl = jx < W ? a : 0.0;
+        int v = cell_of(s, pg.h[0], n) - NB + jj;
+        v = v < 0 ? v + n : (v >= n ? v - n : v);
+        ixl = v;
+    }
+    const size_t ps = size_t(n) * n * n;
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    if (jx < W) {
+        for (int jz = 0; jz < W; jz++) {
+            double col[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++) col[c] = 0.0;
+#pragma unroll 2
+            for (int jy = 0; jy < W; jy++) {
+                const size_t pt = (size_t(iy[jy]) * n + iz[jz]) * n + ixl;
+#pragma unroll
+                for (int c = 0; c < NC; c++) col[c] = fma(by[jy], double(f[c * ps + pt]), col[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[c] = fma(bz[jz], col[c], acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[c] *= bxl;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+#pragma unroll
+        for (int o = LP / 2; o > 0; o /= 2) acc[c] += __shfl_xor(acc[c], o, LP);
+    }
+}
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(256)
+k_interp_ab(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ rhs,
+            const uint32_t* __restrict__ perm, int64_t np, int nsteps, int head, int order, int add_head,
+            double dt, const double* __restrict__ tab, int deg, PartGeomD pg) {
+    constexpr int LP = Lanes<NB>::LP;
+    extern __shared__ double stab[];
+    const int tabsize = (2 * NB + 2) * (deg + 1);  // derivative-0 block only
+    for (int i = threadIdx.x; i < tabsize; i += blockDim.x) stab[i] = tab[i];
+    __syncthreads();
+    const int64_t t = (xcd_block() * blockDim.x + threadIdx.x) / LP;
+    const int jx = threadIdx.x % LP;
+    if (t >= np) return;  // whole lane groups exit together
+    const int64_t p = perm[t];
+    double x[3] = {pos[p * 3 + 0], pos[p * 3 + 1], pos[p * 3 + 2]};
+    double r0[3];
+    gather_coop<T, NB, 3>(f, pg.n, x, stab, deg, pg, jx, r0);
+    if (jx != 0) return;
+    double* h0 = rhs + size_t(head) * np * 3 + p * 3;
+    if (add_head) {
+        r0[0] += h0[0];
+        r0[1] += h0[1];
+        r0[2] += h0[2];
+    }
+    h0[0] = r0[0];
+    h0[1] = r0[1];
+    h0[2] = r0[2];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double r[6];
+        r[0] = r0[d];
+        for (int i = 1; i < order; i++) r[i] = rhs[size_t((head + i) % nsteps) * np * 3 + p * 3 + d];
+        double incr;
+        switch (order) {
+        case 1: incr = dt * r[0]; break;
+        case 2: incr = dt * (3. * r[0] - r[1]) / 2.; break;
+        case 3: incr = dt * (23. * r[0] - 16. * r[1] + 5. * r[2]) / 12.; break;
+        case 4: incr = dt * (55. * r[0] - 59. * r[1] + 37. * r[2] - 9. * r[3]) / 24.; break;
+        case 5: incr = dt * (1901. * r[0] - 2774. * r[1] + 2616. * r[2] - 1274. * r[3] + 251. * r[4]) / 720.; break;
+        default: incr = dt * (4277. * r[0] - 7923. * r[1] + 9982. * r[2] - 7298. * r[3] + 2877. * r[4] - 475. * r[5]) / 1440.; break;
+        }
+        x[d] += incr;
+    }
+    pos[p * 3 + 0] = x[0];
+    pos[p * 3 + 1] = x[1];
+    pos[p * 3 + 2] = x[2];
+}
+
+template <typename T, int NB, int NC>
+__global__ void __launch_bounds__(256)
+k_sample(const T* __restrict__ f, const double* __restrict__ pos, double* __restrict__ out,
+         const uint32_t* __restrict__ perm, int64_t np, const double* __restrict__ tab, int deg, PartGeomD pg) {
+    constexpr int LP = Lanes<NB>::LP;
+    extern __shared__ double stab[];
+    const int tabsize = (2 * NB + 2) * (deg + 1);
+    for (int i = threadIdx.x; i < tabsize; i += blockDim.x) stab[i] = tab[i];
+    __syncthreads();
+    const int64_t t = (xcd_block() * blockDim.x + threadIdx.x) / LP;
+    const int jx = threadIdx.x % LP;
+    if (t >= np) return;
+    const int64_t p = perm[t];
+    const double x[3] = {pos[p * 3 + 0], pos[p * 3 + 1], pos[p * 3 + 2]};
+    double acc[NC];
+    gather_coop<T, NB, NC>(f, pg.n, x, stab, deg, pg, jx, acc);
+    if (jx != 0) return;
+#pragma unroll
+    for (int c = 0; c < NC; c++) out[p * NC + c] = acc[c];
+}
+
+// ---------------------------------------------------------------- LDS bricks
+// Tracers are bucketed by brick (KX x KY x KZ cells).  One workgroup per
+// brick stages the brick plus the stencil halo (n cells below, n+1 above, per
+// axis, periodic) of one field component at a time in LDS with contiguous
+// row reads, then 16-lane groups (lane = stencil column along x) accumulate
+// their tracer's stencil from LDS.  Every lane evaluates one weight per axis;
+// the group exchanges them with shuffles.
+template <typename T, int NB>
+struct BrickCfg {
+    static constexpr int W = 2 * NB + 2;
+    static constexpr int KX = sizeof(T) == 4 ? 16 : 8;
+    static constexpr int KY = sizeof(T) == 4 ? 16 : 8;
+    static constexpr int KZ = 8;
+    static constexpr int WX = KX + W - 1, WY = KY + W - 1, WZ = KZ + W - 1;
+    static constexpr int TILE = WX * WY * WZ;
+    static constexpr size_t TILE_BYTES = size_t(TILE) * sizeof(T);
+#ifdef BFPSH_NO_BRICKS
+    static constexpr bool ok = false;
+#else
+    static constexpr bool ok = W <= 16 && TILE_BYTES <= 80 * 1024;
+#endif
+    static constexpr int LP = 16;
+    static constexpr int NT = 512;
+    static constexpr int GROUPS = NT / LP;
+    // tracers per batch (their results are kept in LDS)
+    template <int NC> static constexpr int pb() { return NC <= 3 ? 4 * GROUPS : 2 * GROUPS; }
+    template <int NC> static constexpr size_t lds_bytes(int deg) {
+        return (size_t(pb<NC>()) * NC + size_t(W) * (deg + 1)) * sizeof(double) + TILE_BYTES;
+    }
+};
+
+__global__ void k_brick_keys(const double* __restrict__ pos, int64_t np, PartGeomD pg, int kx, int ky, int kz,
+                             int nbx, int nby, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int cx = cell_of(pos[p * 3 + 0], pg.h[0], pg.n) / kx;
+    const int cy = cell_of(pos[p * 3 + 1], pg.h[1], pg.n) / ky;
+    const int cz = cell_of(pos[p * 3 + 2], pg.h[2], pg.n) / kz;
+    keys[p] = uint32_t((cz * nby + cy) * nbx + cx);
+    vals[p] = uint32_t(p);
+}
+
+// [bstart, bend) of each brick in the sorted order (bricks without tracers
+// keep the zero fill)
+__global__ void k_brick_bounds(const uint32_t* __restrict__ skeys, int64_t np, uint32_t* __restrict__ bstart,
+                               uint32_t* __restrict__ bend) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const uint32_t k = skeys[i];
+    if (i == 0 || skeys[i - 1] != k) bstart[k] = uint32_t(i);
+    if (i == np - 1 || skeys[i + 1] != k) bend[k] = uint32_t(i + 1);
+}
+
+// MODE 0: velocity + Adams-Bashforth into the history ring; MODE 1: write
+// the NC sampled components to out[p][NC]
+template <typename T, int NB, int NC, int MODE>
+__global__ void __launch_bounds__((BrickCfg<T, NB>::NT))
+k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ rhs, double* __restrict__ out,
+        const uint32_t* __restrict__ perm, const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bend,
+        int nbx, int nby, int64_t np, int nsteps, int head, int order, int add_head, double dt,
+        const double* __restrict__ tab, int deg, PartGeomD pg) {
+    using BC = BrickCfg<T, NB>;
+    constexpr int W = BC::W, LP = BC::LP, NT = BC::NT, PB = BC::template pb<NC>();
+    constexpr int WX = BC::WX, WY = BC::WY, WZ = BC::WZ;
+    extern __shared__ double smem[];
+    double* res = smem;                          // [PB][NC]
+    double* stab = res + PB * NC;                // weight table, derivative 0
+    T* tile = reinterpret_cast<T*>(stab + W * (deg + 1));
+    const int64_t brick = xcd_block();
+    if (brick >= int64_t(nbx) * nby * (pg.n / BC::KZ)) return;
+    const uint32_t s0 = bstart[brick], e0 = bend[brick];
+    if (s0 >= e0) return;  // uniform over the workgroup
+    const int tid = threadIdx.x;
+    const int grp = tid / LP, jx = tid % LP;
+    for (int i = tid; i < W * (deg + 1); i += NT) stab[i] = tab[i];
+    const int n = pg.n;
+    const int bz = int(brick / (int64_t(nbx) * nby)), by = int((brick / nbx) % nby), bx = int(brick % nbx);
+    const int x0 = bx * BC::KX - NB, y0 = by * BC::KY - NB, z0 = bz * BC::KZ - NB;
+    const size_t ps = size_t(n) * n * n;
+    for (uint32_t bs = s0; bs < e0; bs += PB) {
+        const int cnt = int(min(uint32_t(PB), e0 - bs));
+        for (int c = 0; c < NC; c++) {
+            __syncthreads();  // previous users of the tile / res are done
+            // tile fill in chunks of CH loads per thread, all issued before
+            // the chunk's LDS stores
+            constexpr int PERT = (BC::TILE + NT - 1) / NT;
+            constexpr int CH = 12;
+            for (int k0 = 0; k0 < PERT; k0 += CH) {
+                T vals[CH];
+#pragma unroll
+                for (int k = 0; k < CH; k++) {
+                    const int i = tid + (k0 + k) * NT;
+                    if (i < BC::TILE) {
+                        const int lx = i % WX, lz = (i / WX) % WZ, ly = i / (WX * WZ);
+                        int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+                        gx = gx < 0 ? gx + n : (gx >= n ? gx - n : gx);
+                        gy = gy < 0 ? gy + n : (gy >= n ? gy - n : gy);
+                        gz = gz < 0 ? gz + n : (gz >= n ? gz - n : gz);
+                        vals[k] = f[c * ps + (size_t(gy) * n + gz) * n + gx];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < CH; k++) {
+                    const int i = tid + (k0 + k) * NT;
+                    if (i < BC::TILE) tile[i] = vals[k];
+                }
+            }
+            __syncthreads();
+            for (int q = grp; q < cnt; q += NT / LP) {  // whole lane groups iterate together
+                const int64_t p = perm[bs + q];
+                // lane jx evaluates weight jx of each axis (get_norm_pos_in_cell)
+                double wv[3] = {0.0, 0.0, 0.0};
+                int lc[3];
+                const int base[3] = {x0 + NB, y0 + NB, z0 + NB};
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const double sp = pos[p * 3 + d];
+                    const double rep = floor(sp / pg.width[d]);
+                    const double inbox = sp - rep * pg.width[d];
+                    const double ci = floor(inbox / pg.h[d]);
+                    const double xr = (inbox - ci * pg.h[d]) / pg.h[d];
+                    if (jx < W) {
+                        double a = 0.0;
+                        for (int k = deg; k >= 0; k--) a = a * xr + stab[jx * (deg + 1) + k];
+                        wv[d] = a;
+                    }
+                    lc[d] = cell_of(sp, pg.h[d], n) - base[d];  // cell within the brick
+                }
+                // every lane needs all y and z weights of its tracer
+                double wy[W], wz[W];
+#pragma unroll
+                for (int j = 0; j < W; j++) {
+                    wy[j] = __shfl(wv[1], j, LP);
+                    wz[j] = __shfl(wv[2], j, LP);
+                }
+                double acc = 0.0;
+                if (jx < W) {
+                    const T* tcol = tile + lc[0] + jx;
+                    for (int jz = 0; jz < W; jz++) {
+                        double col = 0.0;
+#pragma unroll
+                        for (int jy = 0; jy < W; jy++)
+                            col = fma(wy[jy], double(tcol[((lc[1] + jy) * WZ + lc[2] + jz) * WX]), col);
+                        acc = fma(wz[jz], col, acc);
+                    }
+                    acc *= wv[0];
+                }
+#pragma unroll
+                for (int o = LP / 2; o > 0; o /= 2) acc += __shfl_xor(acc, o, LP);
+                if (jx == 0) res[q * NC + c] = acc;
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < cnt; q += NT) {
+            const int64_t p = perm[bs + q];
+            if (MODE == 1) {
+#pragma unroll
+                for (int c = 0; c < NC; c++) out[p * NC + c] = res[q * NC + c];
+                continue;
+            }
+            double r0[3] = {res[q * NC + 0], res[q * NC + 1], res[q * NC + 2]};
+            double* h0 = rhs + size_t(head) * np * 3 + p * 3;
+            if (add_head) {
+                r0[0] += h0[0];
+                r0[1] += h0[1];
+                r0[2] += h0[2];
+            }
+            h0[0] = r0[0];
+            h0[1] = r0[1];
+            h0[2] = r0[2];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double r[6];
+                r[0] = r0[d];
+                for (int i = 1; i < order; i++) r[i] = rhs[size_t((head + i) % nsteps) * np * 3 + p * 3 + d];
+                double incr;
+                switch (order) {
+                case 1: incr = dt * r[0]; break;
+                case 2: incr = dt * (3. * r[0] - r[1]) / 2.; break;
+                case 3: incr = dt * (23. * r[0] - 16. * r[1] + 5. * r[2]) / 12.; break;
+                case 4: incr = dt * (55. * r[0] - 59. * r[1] + 37. * r[2] - 9. * r[3]) / 24.; break;
+                case 5: incr = dt * (1901. * r[0] - 2774. * r[1] + 2616. * r[2] - 1274. * r[3] + 251. * r[4]) / 720.; break;
+                default: incr = dt * (4277. * r[0] - 7923. * r[1] + 9982. * r[2] - 7298. * r[3] + 2877. * r[4] - 475. * r[5]) / 1440.; break;
+                }
+                pos[p * 3 + d] += incr;
+            }
+        }
+    }
+}
+
+constexpr int kThreads = 256;
+
+unsigned blocks_for(int64_t np, int lp) {
+    int64_t nb = (np * lp + kThreads - 1) / kThreads;
+    nb = (nb + 7) / 8 * 8;  // whole rounds over the 8 XCDs (xcd_block)
+    return unsigned(nb);
+}
+
+int ensure_sort(bfpsh_ctx* ctx) {
+    if (ctx->d_keys) return BFPSH_OK;
+    const int64_t np = ctx->np;
+    HIPCHK(hipMalloc((void**)&ctx->d_keys, 2 * np * sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void**)&ctx->d_perm, 2 * np * sizeof(uint32_t)));
+    size_t tmp = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ctx->d_keys, ctx->d_keys + np, ctx->d_perm,
+                                              ctx->d_perm + np, int(np), 0, 30, ctx->stream));
+    HIPCHK(hipMalloc(&ctx->d_sort_tmp, tmp));
+    ctx->sort_tmp_bytes = tmp;
+    ctx->bytes += 4 * np * sizeof(uint32_t) + tmp;
+    ctx->sort_age = 1 << 30;
+    return BFPSH_OK;
+}
+
+// sorted permutation in d_perm + np
+int sort_tracers(bfpsh_ctx* ctx) {
+    int r = ensure_sort(ctx);
+    if (r) return r;
+    const int64_t np = ctx->np;
+    const int shift = ctx->n >= 64 ? 2 : 0;
+    int nbits = 0;
+    while ((1 << nbits) < (ctx->n >> shift)) nbits++;
+    nbits *= 3;
+    LAUNCH("part_keys", (k_part_keys<<<unsigned((np + 255) / 256), 256, 0, ctx->stream>>>(
+                            ctx->d_pos, np, pgeom_d(ctx), shift, ctx->d_keys, ctx->d_perm)));
+    size_t tmp = ctx->sort_tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_tmp, tmp, ctx->d_keys, ctx->d_keys + np, ctx->d_perm,
+                                              ctx->d_perm + np, int(np), 0, std::max(nbits, 1), ctx->stream));
+    ctx->sort_age = 0;
+    return BFPSH_OK;
+}
+
+template <typename T, int NB>
+int ensure_bricks(bfpsh_ctx* ctx, int64_t nb) {
+    if (ctx->d_bstart && ctx->nbricks == nb) return BFPSH_OK;
+    if (ctx->d_bstart) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipFree(ctx->d_bstart));
+        ctx->d_bstart = nullptr;
+    }
+    HIPCHK(hipMalloc((void**)&ctx->d_bstart, 2 * nb * sizeof(uint32_t)));
+    ctx->nbricks = nb;
+    ctx->bytes += 2 * nb * sizeof(uint32_t);
+    return BFPSH_OK;
+}
+
+// bucket the tracers by brick: sorted permutation in d_perm + np, per-brick
+// ranges in d_bstart[0..nb), d_bstart[nb..2nb)
+template <typename T, int NB>
+int sort_bricks(bfpsh_ctx* ctx) {
+    using BC = BrickCfg<T, NB>;
+    int r = ensure_sort(ctx);
+    if (r) return r;
+    const int n = ctx->n;
+    const int nbx = n / BC::KX, nby = n / BC::KY, nbz = n / BC::KZ;
+    const int64_t nb = int64_t(nbx) * nby * nbz;
+    r = ensure_bricks<T, NB>(ctx, nb);
+    if (r) return r;
+    const int64_t np = ctx->np;
+    int nbits = 1;
+    while ((int64_t(1) << nbits) < nb) nbits++;
+    HIPCHK(hipMemsetAsync(ctx->d_bstart, 0, 2 * nb * sizeof(uint32_t), ctx->stream));
+    LAUNCH("part_keys", (k_brick_keys<<<unsigned((np + 255) / 256), 256, 0, ctx->stream>>>(
+                            ctx->d_pos, np, pgeom_d(ctx), BC::KX, BC::KY, BC::KZ, nbx, nby, ctx->d_keys, ctx->d_perm)));
+    size_t tmp = ctx->sort_tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_tmp, tmp, ctx->d_keys, ctx->d_keys + np, ctx->d_perm,
+                                              ctx->d_perm + np, int(np), 0, nbits, ctx->stream));
+    LAUNCH("part_bounds", (k_brick_bounds<<<unsigned((np + 255) / 256), 256, 0, ctx->stream>>>(
+                              ctx->d_keys + np, np, ctx->d_bstart, ctx->d_bstart + nb)));
+    ctx->sort_age = 0;
+    return BFPSH_OK;
+}
+
+template <typename T, int NB, int NC, int MODE>
+int launch_brick(bfpsh_ctx* ctx, const void* f, double* d_out, double dt, int order, const char* name) {
+    using BC = BrickCfg<T, NB>;
+    const int n = ctx->n;
+    const int nbx = n / BC::KX, nby = n / BC::KY;
+    const int64_t nb = int64_t(nbx) * nby * (n / BC::KZ);
+    const size_t lds = BC::template lds_bytes<NC>(ctx->deg);
+    auto kern = k_brick<T, NB, NC, MODE>;
+    HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    const unsigned grid = unsigned((nb + 7) / 8 * 8);
+    LAUNCH(name, (kern<<<grid, BC::NT, lds, ctx->stream>>>(
+                     (const T*)f, ctx->d_pos, ctx->d_rhs, d_out, ctx->d_perm + ctx->np, ctx->d_bstart,
+                     ctx->d_bstart + nb, nbx, nby, ctx->np, ctx->nsteps, ctx->head, order, ctx->add_head ? 1 : 0, dt,
+                     ctx->d_tab, ctx->deg, pgeom_d(ctx))));
+    return BFPSH_OK;
+}
+
+template <typename T, int NB>
+int loop_nb(bfpsh_ctx* ctx, double dt, int order) {
+    if constexpr (BrickCfg<T, NB>::ok) {
+        int r = sort_bricks<T, NB>(ctx);
+        if (r) return r;
+        return launch_brick<T, NB, 3, 0>(ctx, ctx->ru, nullptr, dt, order, "interp_ab");
+    }
+    if (ctx->sort_age >= ctx->sort_every) {
+        int r = sort_tracers(ctx);
+        if (r) return r;
+    }
+    const size_t lds = size_t(2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
+    LAUNCH("interp_ab", (k_interp_ab<T, NB><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
+                            (const T*)ctx->ru, ctx->d_pos, ctx->d_rhs, ctx->d_perm + ctx->np, ctx->np, ctx->nsteps,
+                            ctx->head, order, ctx->add_head ? 1 : 0, dt, ctx->d_tab, ctx->deg, pgeom_d(ctx))));
+    return BFPSH_OK;
+}
+
+template <typename T, int NB>
+int sample_nb(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
+    if constexpr (BrickCfg<T, NB>::ok) {
+        int r = sort_bricks<T, NB>(ctx);
+        if (r) return r;
+        if (nc == 3) return launch_brick<T, NB, 3, 1>(ctx, f, d_out, 0.0, 0, "sample");
+        return launch_brick<T, NB, 9, 1>(ctx, f, d_out, 0.0, 0, "sample9");
+    }
+    {
+        int r = sort_tracers(ctx);
+        if (r) return r;
+    }
+    const size_t lds = size_t(2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
+    if (nc == 3)
+        LAUNCH("sample", (k_sample<T, NB, 3><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
+                             (const T*)f, ctx->d_pos, d_out, ctx->d_perm + ctx->np, ctx->np, ctx->d_tab, ctx->deg,
+                             pgeom_d(ctx))));
+    else
+        LAUNCH("sample9", (k_sample<T, NB, 9><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
+                              (const T*)f, ctx->d_pos, d_out, ctx->d_perm + ctx->np, ctx->np, ctx->d_tab, ctx->deg,
+                              pgeom_d(ctx))));
+    return BFPSH_OK;
+}
+
+template <typename T>
+int loop_t(bfpsh_ctx* ctx, double dt, int order) {
+    switch (ctx->pn) {
+    case 1: return loop_nb<T, 1>(ctx, dt, order);
+    case 2: return loop_nb<T, 2>(ctx, dt, order);
+    case 3: return loop_nb<T, 3>(ctx, dt, order);
+    case 4: return loop_nb<T, 4>(ctx, dt, order);
+    case 5: return loop_nb<T, 5>(ctx, dt, order);
+    case 6: return loop_nb<T, 6>(ctx, dt, order);
+    case 7: return loop_nb<T, 7>(ctx, dt, order);
+    case 8: return loop_nb<T, 8>(ctx, dt, order);
+    case 9: return loop_nb<T, 9>(ctx, dt, order);
+    case 10: return loop_nb<T, 10>(ctx, dt, order);
+    }
+    return fail(ctx, BFPSH_EINVAL, "bad neighbour count");
+}
+
+template <typename T>
+int sample_t(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
+    switch (ctx->pn) {
+    case 1: return sample_nb<T, 1>(ctx, f, nc, d_out);
+    case 2: return sample_nb<T, 2>(ctx, f, nc, d_out);
+    case 3: return sample_nb<T, 3>(ctx, f, nc, d_out);
+    case 4: return sample_nb<T, 4>(ctx, f, nc, d_out);
+    case 5: return sample_nb<T, 5>(ctx, f, nc, d_out);
+    case 6: return sample_nb<T, 6>(ctx, f, nc, d_out);
+    case 7: return sample_nb<T, 7>(ctx, f, nc, d_out);
+    case 8: return sample_nb<T, 8>(ctx, f, nc, d_out);
+    case 9: return sample_nb<T, 9>(ctx, f, nc, d_out);
+    case 10: return sample_nb<T, 10>(ctx, f, nc, d_out);
+    }
+    return fail(ctx, BFPSH_EINVAL, "bad neighbour count");
+}
+
+}  // namespace
+
+int bfpsh_part_release(bfpsh_ctx* ctx) {
+    for (void* p : {(void*)ctx->d_keys, (void*)ctx->d_perm, ctx->d_sort_tmp, (void*)ctx->d_bstart})
+        if (p) (void)hipFree(p);
+    ctx->d_bstart = nullptr;
+    ctx->nbricks = 0;
+    ctx->d_keys = nullptr;
+    ctx->d_perm = nullptr;
+    ctx->d_sort_tmp = nullptr;
+    ctx->sort_tmp_bytes = 0;
+    ctx->sort_age = 1 << 30;
+    return BFPSH_OK;
+}
+
+// completeLoop: compute (interpolate into the head of the history ring) +
+// move (Adams-Bashforth of order min(step_idx, nsteps)) + inc_step_idx +
+// shift_rhs_vectors (the oldest slot becomes the new, logically zero, head)
+int bfpsh_part_loop(bfpsh_ctx* ctx, double dt) {
+    if (ctx->np == 0) return BFPSH_OK;
+    const int order = std::min(ctx->step_idx, ctx->nsteps);
+    int r = ctx->prec == BFPSH_FP32 ? loop_t<float>(ctx, dt, order) : loop_t<double>(ctx, dt, order);
+    if (r) return r;
+    ctx->sort_age += 1;
+    ctx->step_idx += 1;
+    ctx->head = (ctx->head + ctx->nsteps - 1) % ctx->nsteps;
+    ctx->head_zero = true;
+    ctx->add_head = false;
+    return BFPSH_OK;
+}
+
+int bfpsh_part_sample(bfpsh_ctx* ctx, int which, double* d_out) {
+    if (ctx->np == 0) return BFPSH_OK;
+    const void* f = which == BFPSH_RVELOCITY ? ctx->ru : ctx->rgrad;
+    const int nc = which == BFPSH_RVELOCITY ? 3 : 9;
+    return ctx->prec == BFPSH_FP32 ? sample_t<float>(ctx, f, nc, d_out) : sample_t<double>(ctx, f, nc, d_out);
+}

@@ -268,3 +268,45 @@ def test_errors_are_reported(B):
     fs = B.vorticity_equation("t", 16, precision="double")
     with pytest.raises(B.capi.BfpshError):
         B.particles_system(fs, 10, 11, 1)
+
+
+@pytest.mark.parametrize("n,nb,m", [(64, 6, 2), (64, 4, 1), (48, 1, 1)])
+def test_tracers_fp32_bricks(B, oracle, n, nb, m):
+    """FP32 fields, many tracers per brick and several bricks per axis; the
+    interpolated velocities / gradients are FP64 sums of FP32 field values."""
+    from bfps_amd import ic
+    say("case: fp32 tracers n=%d nb=%d m=%d" % (n, nb, m))
+    w0 = ic.solenoidal_vorticity(n, seed=21, k0=4.0)
+    fs = B.vorticity_equation("t", n, precision="single")
+    fs.nu, fs.forcing_type = 0.01, "linear"
+    fs.io_checkpoint(read=True, data=w0)
+    o = oracle.Solver(n, precision="single")
+    o.set_physics(0.01, "linear", 1.0, 2.0, 4.0)
+    o.set_cfield("vorticity", w0)
+    o.io_projection()
+    npart = 20000
+    pos = ic.uniform_positions(npart, seed=5)
+    ps = B.particles_system(fs, npart, nb, m, 4, step_idx=1)
+    ps.put(pos)
+    op = oracle.Particles(o, npart, nb, m, 4, step_idx=1)
+    op.set_state(pos)
+    dt = 0.5 / n
+    for _ in range(3):
+        B.NSVEparticles_step(fs, dt)
+        oracle.nsve_particles_step(o, op, dt)
+    gp, grhs, _, gsi = ps.get()
+    opos, orhs, osi = op.get_state()
+    say("  pos err %.3e rhs err %.3e" % (np.abs(gp - opos).max(), np.abs(grhs - orhs).max()))
+    assert gsi == osi
+    # FP32 fluid fields differ by rounding between the two FFTs
+    assert np.abs(grhs - orhs).max() < 1e-5 * max(1.0, np.abs(orhs).max())
+    assert np.abs(gp - opos).max() < 1e-7
+    fs.velocity_rspace()
+    o.velocity_rspace()
+    sv, ov = ps.sample("velocity"), op.sample("velocity")
+    assert np.abs(sv - ov).max() < 1e-5 * np.abs(ov).max()
+    fs.gradient_rspace()
+    o.gradient_rspace()
+    sg, og = ps.sample("gradient"), op.sample("gradient")
+    say("  gradient sample rel err %.3e" % (np.abs(sg - og).max() / np.abs(og).max()))
+    assert np.abs(sg - og).max() < 1e-5 * np.abs(og).max()
