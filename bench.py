@@ -35,7 +35,8 @@ def step_bytes(n, es, with_tracer_u):
 
 # algorithmic bytes per launch of each kernel, in field volumes N^3 * es
 KERNEL_VOLUMES = {
-    "inv_z_bs": 9,         # read omega (3 complex planes), write u, omega (6)
+    "inv_z_bs": 9 * 15 / 16,   # kx tiles 1..15 of 16: read omega (3 planes), write u, omega (6)
+    "inv_z_bs_t0": 9 / 16,     # kx tile 0 (with the symmetrize substitution)
     "inv_y": 12,           # 6 planes read + written
     "rows_cross": 9,       # read 6, write 3 (+3 real velocity in stage 1)
     "fwd_z": 6,            # 3 planes read + written

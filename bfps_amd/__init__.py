@@ -10,6 +10,8 @@ Python mirror of the reference's class surface over the C ABI
   (bfps/cpp/kspace.hpp:43-195)
 * ``particles_system``    <- ``abstract_particles_system`` built by
   ``particles_system_builder`` (bfps/cpp/particles/abstract_particles_system.hpp:12-55)
+* ``field``               <- ``field<rnumber, FFTW, ONE|THREE|THREExTHREE>``
+  with its in-place ``dft``/``ift``/``symmetrize`` (bfps/cpp/field.hpp:47-200)
 * ``NSVEparticles_step``  <- ``NSVEparticles::step`` (bfps/cpp/full_code/NSVEparticles.cpp:33-41)
 
 Host arrays use the reference's layouts (complex ``[ny][nz][nx//2+1][3]``,
@@ -22,7 +24,7 @@ import numpy as np
 
 from . import capi
 
-__all__ = ["vorticity_equation", "kspace", "particles_system", "NSVEparticles_step", "capi"]
+__all__ = ["vorticity_equation", "kspace", "field", "particles_system", "NSVEparticles_step", "capi"]
 
 
 class kspace:
@@ -204,6 +206,105 @@ class vorticity_equation:
         b = ctypes.c_size_t()
         capi.check(self._h, capi.load().bfpsh_device_bytes(self._h, ctypes.byref(b)), "device_bytes")
         return b.value
+
+
+class field:
+    """Device-resident ``field<rnumber, FFTW, fc>`` of a context
+    (bfps/cpp/field.hpp:47-200): ncomp = 1 (ONE), 3 (THREE) or 9
+    (THREExTHREE).  Transforms are the reference's unnormalised in-place
+    ``dft`` (r2c) / ``ift`` (c2r); ``real_space_representation`` tracks which
+    one the data is in, as the reference's member of that name."""
+
+    _FC = {"ONE": 1, "THREE": 3, "THREExTHREE": 9, 1: 1, 3: 3, 9: 9}
+
+    def __init__(self, fs, fc="THREE"):
+        self.fs = fs
+        self.ncomp = self._FC[fc]
+        self._id = ctypes.c_int(-1)
+        capi.check(fs.handle, capi.load().bfpsh_field_create(fs.handle, self.ncomp, ctypes.byref(self._id)),
+                   "field_create")
+        self.real_space_representation = False
+
+    @property
+    def id(self):
+        return self._id.value
+
+    def close(self):
+        if self._id.value >= 0 and self.fs.handle:
+            capi.check(self.fs.handle, capi.load().bfpsh_field_destroy(self.fs.handle, self._id.value),
+                       "field_destroy")
+        self._id.value = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rshape(self):
+        return (self.fs.nz, self.fs.ny, self.fs.nx, self.ncomp)
+
+    def cshape(self):
+        return self.fs.cshape(self.ncomp)
+
+    def put(self, data, real_space):
+        """Host data in the reference layout ([nz][ny][nx][nc] real or
+        [ny][nz][nx/2+1][nc] complex)."""
+        if real_space:
+            a = np.ascontiguousarray(np.asarray(data).reshape(self.rshape()), dtype=self.fs.rdtype)
+        else:
+            a = np.ascontiguousarray(np.asarray(data).reshape(self.cshape()), dtype=self.fs.cdtype)
+        capi.check(self.fs.handle, capi.load().bfpsh_field_put(self.fs.handle, self.id, int(bool(real_space)),
+                                                               a.ctypes.data), "field_put")
+        self.real_space_representation = bool(real_space)
+
+    def get(self):
+        if self.real_space_representation:
+            out = np.zeros(self.rshape(), dtype=self.fs.rdtype)
+        else:
+            out = np.zeros(self.cshape(), dtype=self.fs.cdtype)
+        capi.check(self.fs.handle, capi.load().bfpsh_field_get(self.fs.handle, self.id,
+                                                               int(self.real_space_representation),
+                                                               out.ctypes.data), "field_get")
+        return out
+
+    def get_rdata(self):
+        assert self.real_space_representation
+        return self.get()
+
+    def get_cdata(self):
+        assert not self.real_space_representation
+        return self.get()
+
+    def dft(self):
+        """field::dft (bfps/cpp/field.cpp:154-162)."""
+        capi.check(self.fs.handle, capi.load().bfpsh_field_dft(self.fs.handle, self.id), "field_dft")
+        self.real_space_representation = False
+
+    def ift(self):
+        """field::ift (bfps/cpp/field.cpp:144-152)."""
+        capi.check(self.fs.handle, capi.load().bfpsh_field_ift(self.fs.handle, self.id), "field_ift")
+        self.real_space_representation = True
+
+    def symmetrize(self):
+        """field::symmetrize (bfps/cpp/field.cpp:900-978)."""
+        capi.check(self.fs.handle, capi.load().bfpsh_field_symmetrize(self.fs.handle, self.id), "field_symmetrize")
+
+    def copy_from(self, which="vorticity"):
+        """Spectral copy of the solver's cvorticity or its Biot-Savart velocity
+        (bfps/cpp/full_code/NSVE.cpp:115,124)."""
+        sel = {"vorticity": capi.CVORTICITY, "velocity": capi.CVELOCITY}[which]
+        capi.check(self.fs.handle, capi.load().bfpsh_field_copy_from(self.fs.handle, self.id, sel),
+                   "field_copy_from")
+        self.real_space_representation = False
+
+    def cospectrum(self, other):
+        """kspace::cospectrum of two spectral 3-component fields, [shell][i][j]."""
+        out = np.zeros((self.fs.kk.nshells, 3, 3))
+        capi.check(self.fs.handle, capi.load().bfpsh_field_cospectrum(self.fs.handle, self.id, other.id,
+                                                                      out.ctypes.data_as(capi._DP)),
+                   "field_cospectrum")
+        return out
 
 
 class particles_system:

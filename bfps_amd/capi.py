@@ -37,6 +37,15 @@ SIGNATURES = {
     "bfpsh_velocity_rspace": (_I, [_P]),
     "bfpsh_gradient_rspace": (_I, [_P]),
     "bfpsh_cospectrum": (_I, [_P, _I, _I, _DP]),
+    "bfpsh_field_create": (_I, [_P, _I, _IP]),
+    "bfpsh_field_destroy": (_I, [_P, _I]),
+    "bfpsh_field_ift": (_I, [_P, _I]),
+    "bfpsh_field_dft": (_I, [_P, _I]),
+    "bfpsh_field_symmetrize": (_I, [_P, _I]),
+    "bfpsh_field_put": (_I, [_P, _I, _I, _P]),
+    "bfpsh_field_get": (_I, [_P, _I, _I, _P]),
+    "bfpsh_field_copy_from": (_I, [_P, _I, _I]),
+    "bfpsh_field_cospectrum": (_I, [_P, _I, _I, _DP]),
     "bfpsh_particles_create": (_I, [_P, _L, _I, _I, _I, _I]),
     "bfpsh_particles_put": (_I, [_P, _DP, _DP, _LP]),
     "bfpsh_particles_get": (_I, [_P, _DP, _DP, _LP, _IP]),

@@ -35,33 +35,50 @@ void build_twiddles(std::vector<unsigned char>& out, int n) {
 
 }  // namespace
 
-// host [ky][kz][kx=N/2+1][3]{re,im} <-> device planar [c][ky][kz][kx<N/2]
+// host [ky][kz][kx=N/2+1][nc]{re,im} <-> device planar [c][ky][kz][kx<N/2]
 template <typename T>
-static void host_to_planar(const T* h, T* p, int n) {
+static void host_to_planar(const T* h, T* p, int n, int nc = 3) {
     const int nxh = n / 2 + 1, nxs = n / 2;
     const size_t ps = size_t(n) * n * nxs;
     for (size_t yz = 0; yz < size_t(n) * n; yz++)
         for (int x = 0; x < nxs; x++)
-            for (int c = 0; c < 3; c++) {
-                p[(c * ps + yz * nxs + x) * 2 + 0] = h[((yz * nxh + x) * 3 + c) * 2 + 0];
-                p[(c * ps + yz * nxs + x) * 2 + 1] = h[((yz * nxh + x) * 3 + c) * 2 + 1];
+            for (int c = 0; c < nc; c++) {
+                p[(c * ps + yz * nxs + x) * 2 + 0] = h[((yz * nxh + x) * nc + c) * 2 + 0];
+                p[(c * ps + yz * nxs + x) * 2 + 1] = h[((yz * nxh + x) * nc + c) * 2 + 1];
             }
 }
 template <typename T>
-static void planar_to_host(const T* p, T* h, int n) {
+static void planar_to_host(const T* p, T* h, int n, int nc = 3) {
     const int nxh = n / 2 + 1, nxs = n / 2;
     const size_t ps = size_t(n) * n * nxs;
     for (size_t yz = 0; yz < size_t(n) * n; yz++)
         for (int x = 0; x < nxh; x++)
-            for (int c = 0; c < 3; c++) {
+            for (int c = 0; c < nc; c++) {
                 T re = 0, im = 0;
                 if (x < nxs) {
                     re = p[(c * ps + yz * nxs + x) * 2 + 0];
                     im = p[(c * ps + yz * nxs + x) * 2 + 1];
                 }
-                h[((yz * nxh + x) * 3 + c) * 2 + 0] = re;
-                h[((yz * nxh + x) * 3 + c) * 2 + 1] = im;
+                h[((yz * nxh + x) * nc + c) * 2 + 0] = re;
+                h[((yz * nxh + x) * nc + c) * 2 + 1] = im;
             }
+}
+// host real [z][y][x][nc] <-> device planar [c][y][z][x]
+template <typename T>
+static void host_real_to_planar(const T* h, T* p, int n, int nc) {
+    const size_t n3 = size_t(n) * n * n;
+    for (size_t y = 0; y < size_t(n); y++)
+        for (size_t z = 0; z < size_t(n); z++)
+            for (size_t x = 0; x < size_t(n); x++)
+                for (int c = 0; c < nc; c++) p[c * n3 + (y * n + z) * n + x] = h[((z * n + y) * n + x) * nc + c];
+}
+template <typename T>
+static void planar_to_host_real(const T* p, T* h, int n, int nc) {
+    const size_t n3 = size_t(n) * n * n;
+    for (size_t y = 0; y < size_t(n); y++)
+        for (size_t z = 0; z < size_t(n); z++)
+            for (size_t x = 0; x < size_t(n); x++)
+                for (int c = 0; c < nc; c++) h[((z * n + y) * n + x) * nc + c] = p[c * n3 + (y * n + z) * n + x];
 }
 
 #define CTX_CHECK()                                  \
@@ -158,6 +175,8 @@ int bfpsh_destroy(bfpsh_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         for (void* p : {ctx->v0, ctx->v1, ctx->v2, ctx->work, ctx->ru, ctx->rgrad, ctx->tw,
                         (void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab})
+            if (p) (void)hipFree(p);
+        for (void* p : ctx->fields)
             if (p) (void)hipFree(p);
         bfpsh_part_release(ctx);
         resolve_profile(ctx);
@@ -287,6 +306,118 @@ int bfpsh_cospectrum(bfpsh_ctx* ctx, int a, int b, double* out) {
     const void* pa = a == BFPSH_CVORTICITY ? ctx->v0 : ctx->work;
     const void* pb = b == BFPSH_CVORTICITY ? ctx->v0 : ctx->work;
     return PREC_CALL(cospectrum, ctx, pa, pb, out);
+}
+
+static bool field_ok(bfpsh_ctx* ctx, int id) {
+    return id >= 0 && id < int(ctx->fields.size()) && ctx->fields[id] != nullptr;
+}
+
+int bfpsh_field_create(bfpsh_ctx* ctx, int ncomp, int* id) {
+    CTX_CHECK();
+    if (!id || (ncomp != 1 && ncomp != 3 && ncomp != 9)) return fail(ctx, BFPSH_EINVAL, "ncomp must be 1, 3 or 9");
+    void* p = nullptr;
+    const size_t bytes = size_t(ncomp) * ctx->ps * 2 * ctx->es;
+    HIPCHK(hipMalloc(&p, bytes));
+    HIPCHK(hipMemsetAsync(p, 0, bytes, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->fields.push_back(p);
+    ctx->field_nc.push_back(ncomp);
+    ctx->bytes += bytes;
+    *id = int(ctx->fields.size()) - 1;
+    return BFPSH_OK;
+}
+
+int bfpsh_field_destroy(bfpsh_ctx* ctx, int id) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id)) return fail(ctx, BFPSH_EINVAL, "bad field id");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipFree(ctx->fields[id]));
+    ctx->bytes -= size_t(ctx->field_nc[id]) * ctx->ps * 2 * ctx->es;
+    ctx->fields[id] = nullptr;
+    return BFPSH_OK;
+}
+
+int bfpsh_field_ift(bfpsh_ctx* ctx, int id) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id)) return fail(ctx, BFPSH_EINVAL, "bad field id");
+    return PREC_CALL(field_ift, ctx, ctx->fields[id], ctx->field_nc[id]);
+}
+
+int bfpsh_field_dft(bfpsh_ctx* ctx, int id) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id)) return fail(ctx, BFPSH_EINVAL, "bad field id");
+    return PREC_CALL(field_dft, ctx, ctx->fields[id], ctx->field_nc[id]);
+}
+
+int bfpsh_field_symmetrize(bfpsh_ctx* ctx, int id) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id)) return fail(ctx, BFPSH_EINVAL, "bad field id");
+    const int n = ctx->n;
+    if (ctx->prec == BFPSH_FP32)
+        LAUNCH("symmetrize", (k_symmetrize<float><<<(n * n + 255) / 256, 256, 0, ctx->stream>>>((float2*)ctx->fields[id], n, ctx->field_nc[id])));
+    else
+        LAUNCH("symmetrize", (k_symmetrize<double><<<(n * n + 255) / 256, 256, 0, ctx->stream>>>((double2*)ctx->fields[id], n, ctx->field_nc[id])));
+    return BFPSH_OK;
+}
+
+int bfpsh_field_put(bfpsh_ctx* ctx, int id, int real_space, const void* host) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id) || !host) return fail(ctx, BFPSH_EINVAL, "bad field id or buffer");
+    const int nc = ctx->field_nc[id];
+    const size_t bytes = size_t(nc) * ctx->ps * 2 * ctx->es;
+    std::vector<unsigned char> tmp(bytes, 0);
+    if (real_space) {
+        if (ctx->prec == BFPSH_FP32) host_real_to_planar<float>((const float*)host, (float*)tmp.data(), ctx->n, nc);
+        else host_real_to_planar<double>((const double*)host, (double*)tmp.data(), ctx->n, nc);
+    } else {
+        if (ctx->prec == BFPSH_FP32) host_to_planar<float>((const float*)host, (float*)tmp.data(), ctx->n, nc);
+        else host_to_planar<double>((const double*)host, (double*)tmp.data(), ctx->n, nc);
+    }
+    HIPCHK(hipMemcpyAsync(ctx->fields[id], tmp.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BFPSH_OK;
+}
+
+int bfpsh_field_get(bfpsh_ctx* ctx, int id, int real_space, void* host) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id) || !host) return fail(ctx, BFPSH_EINVAL, "bad field id or buffer");
+    const int nc = ctx->field_nc[id];
+    const size_t bytes = size_t(nc) * ctx->ps * 2 * ctx->es;
+    std::vector<unsigned char> tmp(bytes);
+    HIPCHK(hipMemcpyAsync(tmp.data(), ctx->fields[id], bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (real_space) {
+        if (ctx->prec == BFPSH_FP32) planar_to_host_real<float>((const float*)tmp.data(), (float*)host, ctx->n, nc);
+        else planar_to_host_real<double>((const double*)tmp.data(), (double*)host, ctx->n, nc);
+    } else {
+        if (ctx->prec == BFPSH_FP32) planar_to_host<float>((const float*)tmp.data(), (float*)host, ctx->n, nc);
+        else planar_to_host<double>((const double*)tmp.data(), (double*)host, ctx->n, nc);
+    }
+    return BFPSH_OK;
+}
+
+int bfpsh_field_copy_from(bfpsh_ctx* ctx, int id, int which) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id) || ctx->field_nc[id] != 3) return fail(ctx, BFPSH_EINVAL, "needs a 3-component field");
+    const void* src;
+    if (which == BFPSH_CVORTICITY) {
+        src = ctx->v0;
+    } else if (which == BFPSH_CVELOCITY) {
+        int r = PREC_CALL(spectral_velocity, ctx);
+        if (r) return r;
+        src = ctx->work;
+    } else {
+        return fail(ctx, BFPSH_EINVAL, "bad field selector");
+    }
+    HIPCHK(hipMemcpyAsync(ctx->fields[id], src, 3 * ctx->ps * 2 * ctx->es, hipMemcpyDeviceToDevice, ctx->stream));
+    return BFPSH_OK;
+}
+
+int bfpsh_field_cospectrum(bfpsh_ctx* ctx, int id_a, int id_b, double* out) {
+    CTX_CHECK();
+    if (!out || !field_ok(ctx, id_a) || !field_ok(ctx, id_b) || ctx->field_nc[id_a] != 3 || ctx->field_nc[id_b] != 3)
+        return fail(ctx, BFPSH_EINVAL, "needs two 3-component fields");
+    return PREC_CALL(cospectrum, ctx, ctx->fields[id_a], ctx->fields[id_b], out);
 }
 
 int bfpsh_particles_create(bfpsh_ctx* ctx, int64_t np, int n, int m, int nsteps, int step_idx) {

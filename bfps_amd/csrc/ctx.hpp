@@ -51,6 +51,9 @@ struct bfpsh_ctx {
     double* d_rhs = nullptr;
     double* d_tab = nullptr;
     std::vector<int64_t> idx;
+    // standalone fields (field<rnumber, HIP, fc>): device buffer + ncomp per id
+    std::vector<void*> fields;
+    std::vector<int> field_nc;
     // spatial ordering of the tracers (a permutation, data stays in place)
     uint32_t* d_keys = nullptr;    // 2 x np
     uint32_t* d_perm = nullptr;    // 2 x np
@@ -180,6 +183,8 @@ struct OpsTable {
     int (*gradient_rspace)(bfpsh_ctx*);
     int (*spectral_velocity)(bfpsh_ctx*);
     int (*cospectrum)(bfpsh_ctx*, const void*, const void*, double*);
+    int (*field_ift)(bfpsh_ctx*, void*, int);
+    int (*field_dft)(bfpsh_ctx*, void*, int);
 };
 
 // tracer loop and sampling (particles.hip), dispatched on precision and

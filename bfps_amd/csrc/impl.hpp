@@ -14,9 +14,9 @@ struct Impl {
     // substitution, the remaining tiles without
     template <int MODE>
     static int bs_pass(bfpsh_ctx* ctx, const CT* src, CT* w, const CT* tw, const Geom& g, const char* name) {
-        LAUNCH(name, (k_inv_z_bs<T, N, MODE, true><<<dim3(1, N), CF::NT_BS, 0, ctx->stream>>>(src, w, tw, g, 0)));
-        if (NXH / B > 1)
-            LAUNCH(name, (k_inv_z_bs<T, N, MODE, false><<<dim3(NXH / B - 1, N), CF::NT_BS, 0, ctx->stream>>>(
+        LAUNCH("inv_z_bs_t0", (k_inv_z_bs<T, N, MODE, true><<<dim3(1, N), CF::NT_BS, 0, ctx->stream>>>(src, w, tw, g, 0)));
+        if (NXH / CF::B_BS > 1)
+            LAUNCH(name, (k_inv_z_bs<T, N, MODE, false><<<dim3(NXH / CF::B_BS - 1, N), CF::NT_BS, 0, ctx->stream>>>(
                               src, w, tw, g, 1)));
         return BFPSH_OK;
     }
@@ -42,12 +42,13 @@ struct Impl {
                                   w, ru, tw, T(double(N) * N * N))));
         LAUNCH("fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, 3), NT, 0, ctx->stream>>>(
                              w, size_t(NXH), size_t(N) * NXH, tw)));
+        const dim3 tiles3(NXH / CF::B_Y3, N);
         if (s == 1)
-            LAUNCH("fwd_y_update1", (k_fwd_y_update<T, N, 1><<<tiles, NT, 0, ctx->stream>>>(w, v0, v1, v1, tw, g, sp)));
+            LAUNCH("fwd_y_update1", (k_fwd_y_update<T, N, 1><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, v1, v1, tw, g, sp)));
         else if (s == 2)
-            LAUNCH("fwd_y_update2", (k_fwd_y_update<T, N, 2><<<tiles, NT, 0, ctx->stream>>>(w, v0, v1, v2, tw, g, sp)));
+            LAUNCH("fwd_y_update2", (k_fwd_y_update<T, N, 2><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, v1, v2, tw, g, sp)));
         else
-            LAUNCH("fwd_y_update3", (k_fwd_y_update<T, N, 3><<<tiles, NT, 0, ctx->stream>>>(w, v0, v2, v0, tw, g, sp)));
+            LAUNCH("fwd_y_update3", (k_fwd_y_update<T, N, 3><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, v2, v0, tw, g, sp)));
         return BFPSH_OK;
     }
 
@@ -120,6 +121,33 @@ struct Impl {
         LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 9), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw)));
         LAUNCH("rows_c2r9", (k_rows_c2r<T, N, 9><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->rgrad, tw)));
         return BFPSH_OK;
+    }
+
+    // field::ift / field::dft on a standalone NC-component field (in place):
+    // c2r = z pass, y pass (inverse), x rows; r2c = x rows, z pass, y pass
+    template <int NC>
+    static int field_ift_nc(bfpsh_ctx* ctx, void* data) {
+        CT* d = (CT*)data;
+        const CT* tw = (const CT*)ctx->tw;
+        LAUNCH("field_inv_z", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), size_t(N) * NXH, tw)));
+        LAUNCH("field_inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(N) * NXH, size_t(NXH), tw)));
+        LAUNCH("field_rows_c2r", (k_rows_c2r<T, N, NC><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(d, (T*)d, tw)));
+        return BFPSH_OK;
+    }
+    template <int NC>
+    static int field_dft_nc(bfpsh_ctx* ctx, void* data) {
+        CT* d = (CT*)data;
+        const CT* tw = (const CT*)ctx->tw;
+        LAUNCH("field_rows_r2c", (k_rows_r2c<T, N, NC><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>((const T*)d, d, tw)));
+        LAUNCH("field_fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), size_t(N) * NXH, tw)));
+        LAUNCH("field_fwd_y", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(N) * NXH, size_t(NXH), tw)));
+        return BFPSH_OK;
+    }
+    static int field_ift(bfpsh_ctx* ctx, void* data, int nc) {
+        return nc == 1 ? field_ift_nc<1>(ctx, data) : (nc == 3 ? field_ift_nc<3>(ctx, data) : field_ift_nc<9>(ctx, data));
+    }
+    static int field_dft(bfpsh_ctx* ctx, void* data, int nc) {
+        return nc == 1 ? field_dft_nc<1>(ctx, data) : (nc == 3 ? field_dft_nc<3>(ctx, data) : field_dft_nc<9>(ctx, data));
     }
 
     // cvelocity = symmetrize(BS(cvorticity)) into work planes 0..2

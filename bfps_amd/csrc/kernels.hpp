@@ -50,7 +50,17 @@ struct Cfg {
     }
     static constexpr int B = bcap();
     static constexpr int NT = G * B;
-    static constexpr int NT_BS = NT;
+    // Biot-Savart pass: three source components stay in registers; half the
+    // columns (when NT would exceed 512) keep it within a 256-VGPR budget
+    static constexpr int B_BS = (NT > 512 && B > 1) ? B / 2 : B;
+    static constexpr int NT_BS = G * B_BS;
+    // last forward pass with the update epilogue (3 components in registers)
+#ifdef BFPSH_Y3_NARROW
+    static constexpr int B_Y3 = B_BS;
+#else
+    static constexpr int B_Y3 = B;
+#endif
+    static constexpr int NT_Y3 = G * B_Y3;
     static constexpr int RP = (N * int(sizeof(C2<T>)) >= 8192) ? 1 : 2;  // row pairs per row block
     static constexpr int NT_ROW = N >= 256 ? 256 : (N >= 64 ? 128 : 64);
     static_assert(NT <= 1024, "tile threads");
@@ -85,10 +95,10 @@ template <int MODE> struct BsOut { static constexpr int NF = MODE == 0 ? 6 : (MO
 // symmetrize substitution there; the other tiles use MIRROR = false
 // (blockIdx.x offset by tile0) and carry no mirror code.
 template <typename T, int N, int MODE, bool MIRROR>
-__global__ void __launch_bounds__((Cfg<T, N>::NT))
+__global__ void __launch_bounds__((Cfg<T, N>::NT_BS))
 k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __restrict__ tw, Geom g, int tile0) {
     using CF = Cfg<T, N>;
-    constexpr int B = CF::B, NT = CF::NT, E = CF::E, NXH = CF::NXH;
+    constexpr int B = CF::B_BS, NT = CF::NT_BS, E = CF::E, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
     constexpr int NF = BsOut<MODE>::NF;
     using F = RegFft<T, N, B, B, NT, +1>;
@@ -106,16 +116,17 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
 #pragma unroll
         for (int c = 0; c < 3; c++) wr[c][e] = w[c * PS + off];
     }
+    // Velocity-type fields in a runtime loop (register arrays indexed by
+    // constants only), then the vorticity copies (MODE 0) as straight-line
+    // code: a runtime select between wr[0..2] would turn into a pointer into
+    // wr and force the array into scratch memory.
+    constexpr int NU = MODE == 0 ? 3 : NF;
 #pragma unroll 1
-    for (int f = 0; f < NF; f++) {
+    for (int f = 0; f < NU; f++) {
         C2<T> v[E];
 #pragma unroll
         for (int e = 0; e < E; e++) {
             const int n = F::in_row(grp, e);
-            if (MODE == 0 && f >= 3) {
-                v[e] = wr[f - 3][e];
-                continue;
-            }
             const double kz = kwave(n, N, g.dkz);
             const double k2 = kx * kx + kyv * kyv + kz * kz;
             const int cu = MODE == 2 ? f % 3 : f;
@@ -145,6 +156,15 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
         C2<T>* o = out + size_t(f) * PS;
 #pragma unroll
         for (int e = 0; e < E; e++) o[(size_t(ky) * N + F::out_row(grp, e)) * NXH + kxi] = v[e];
+    }
+    if constexpr (MODE == 0) {
+        sfor<0, 3>([&](auto cc) {
+            constexpr int c = cc;
+            F::run(wr[c], lds, tw, grp, b);
+            C2<T>* o = out + size_t(3 + c) * PS;
+#pragma unroll
+            for (int e = 0; e < E; e++) o[(size_t(ky) * N + F::out_row(grp, e)) * NXH + kxi] = wr[c][e];
+        });
     }
 }
 
@@ -180,8 +200,8 @@ k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __
 // NCOL columns at once: all global loads of a thread are issued before its
 // LDS stores.  Column col reads rows ra(col), rb(col) = base + off[col].
 template <typename T, int N, int LS, int NCOL, int NT>
-BFPSH_DEV void rows_unpack_inverse(C2<T>* lds, const C2<T>* __restrict__ const (&ra)[NCOL],
-                                   const C2<T>* __restrict__ const (&rb)[NCOL], int tid) {
+BFPSH_DEV void rows_unpack_inverse(C2<T>* lds, const C2<T>* const (&ra)[NCOL], const C2<T>* const (&rb)[NCOL],
+                                   int tid) {
     constexpr int H = N / 2;
     constexpr int PER = (H + NT - 1) / NT;
 #pragma unroll
@@ -207,8 +227,7 @@ BFPSH_DEV void rows_unpack_inverse(C2<T>* lds, const C2<T>* __restrict__ const (
     }
 }
 template <typename T, int N, int LS>
-BFPSH_DEV void rows_pack_forward(const C2<T>* lds, int col, C2<T>* __restrict__ ra, C2<T>* __restrict__ rb,
-                                 int tid, int nt) {
+BFPSH_DEV void rows_pack_forward(const C2<T>* lds, int col, C2<T>* ra, C2<T>* rb, int tid, int nt) {
     constexpr int H = N / 2;
     for (int k = tid; k < H; k += nt) {
         const C2<T> Z = lds[k * LS + col];
@@ -227,7 +246,8 @@ template <typename T, int N>
 __global__ void __launch_bounds__((Cfg<T, N>::NT_ROW))
 k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restrict__ tw, T npoints) {
     using CF = Cfg<T, N>;
-    constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = 6 * RP;
+    // LDS row stride: odd, so lanes striding by rows hit distinct banks
+    constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = 6 * RP + 1;
     constexpr size_t PS = size_t(N) * N * NXH;
     __shared__ C2<T> lds[N * LS];
     const int tid = threadIdx.x;
@@ -246,7 +266,7 @@ k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restri
         rows_unpack_inverse<T, N, LS, 6 * RP, NT>(lds, ra, rb, tid);
     }
     __syncthreads();
-    fft_tile<T, N, LS, LS, NT, CF::RMAX, +1>(lds, tw, tid);
+    fft_tile<T, N, 6 * RP, LS, NT, CF::RMAX, +1>(lds, tw, tid);
     for (int e = tid; e < N * RP; e += NT) {
         const int nn = e / RP, p = e % RP;
         C2<T> u[3], w[3];
@@ -281,40 +301,78 @@ k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restri
         }
 }
 
-// c2r x pass of NC planes into real planes out ([y][z][x] each)
+// c2r x pass of NC planes into real planes out ([y][z][x] each), components
+// in groups of up to 3.  out may equal data (in place: a real row occupies
+// the bytes of the spectral row with the same index, and a block reads all
+// of its rows before writing them).
 template <typename T, int N, int NC>
 __global__ void __launch_bounds__((Cfg<T, N>::NT_ROW))
-k_rows_c2r(const C2<T>* __restrict__ data, T* __restrict__ out, const C2<T>* __restrict__ tw) {
+k_rows_c2r(const C2<T>* data, T* out, const C2<T>* __restrict__ tw) {
     using CF = Cfg<T, N>;
-    constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = 3 * RP;
+    constexpr int NG = NC < 3 ? NC : 3;
+    constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = NG * RP + 1;
     constexpr size_t PS = size_t(N) * N * NXH;
+    static_assert(NC % NG == 0, "component groups");
     __shared__ C2<T> lds[N * LS];
     const int tid = threadIdx.x;
     const int q0 = blockIdx.x * RP;
-    for (int c0 = 0; c0 < NC; c0 += 3) {
+    for (int c0 = 0; c0 < NC; c0 += NG) {
         {
-            const C2<T>* ra[3 * RP];
-            const C2<T>* rb[3 * RP];
+            const C2<T>* ra[NG * RP];
+            const C2<T>* rb[NG * RP];
 #pragma unroll
-            for (int c = 0; c < 3; c++)
+            for (int c = 0; c < NG; c++)
 #pragma unroll
                 for (int p = 0; p < RP; p++) {
                     const size_t row = size_t(q0 + p) * 2;
                     ra[c * RP + p] = data + (c0 + c) * PS + row * NXH;
                     rb[c * RP + p] = data + (c0 + c) * PS + (row + 1) * NXH;
                 }
-            rows_unpack_inverse<T, N, LS, 3 * RP, NT>(lds, ra, rb, tid);
+            rows_unpack_inverse<T, N, LS, NG * RP, NT>(lds, ra, rb, tid);
         }
         __syncthreads();
-        fft_tile<T, N, LS, LS, NT, CF::RMAX, +1>(lds, tw, tid);
-        for (int e = tid; e < N * LS; e += NT) {
-            const int nn = e / LS, col = e % LS;
+        fft_tile<T, N, NG * RP, LS, NT, CF::RMAX, +1>(lds, tw, tid);
+        for (int e = tid; e < N * NG * RP; e += NT) {  // consecutive lanes: consecutive x
+            const int col = e / N, nn = e % N;
             const int c = col / RP, p = col % RP;
             const size_t row = size_t(q0 + p) * 2;
             const C2<T> v = lds[nn * LS + col];
             out[(c0 + c) * size_t(N) * N * N + row * N + nn] = v.x;
             out[(c0 + c) * size_t(N) * N * N + (row + 1) * N + nn] = v.y;
         }
+        __syncthreads();
+    }
+}
+
+// r2c x pass of NC real planes ([y][z][x]) into spectral rows (kx < N/2),
+// components in groups of up to 3; out may equal data (in place).
+template <typename T, int N, int NC>
+__global__ void __launch_bounds__((Cfg<T, N>::NT_ROW))
+k_rows_r2c(const T* data, C2<T>* out, const C2<T>* __restrict__ tw) {
+    using CF = Cfg<T, N>;
+    constexpr int NG = NC < 3 ? NC : 3;
+    constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = NG * RP + 1;
+    constexpr size_t PS = size_t(N) * N * NXH;
+    static_assert(NC % NG == 0, "component groups");
+    __shared__ C2<T> lds[N * LS];
+    const int tid = threadIdx.x;
+    const int q0 = blockIdx.x * RP;
+    for (int c0 = 0; c0 < NC; c0 += NG) {
+        for (int e = tid; e < N * NG * RP; e += NT) {  // consecutive lanes: consecutive x
+            const int col = e / N, nn = e % N;
+            const int c = col / RP, p = col % RP;
+            const size_t row = size_t(q0 + p) * 2;
+            lds[nn * LS + col] = mk<T>(data[(c0 + c) * size_t(N) * N * N + row * N + nn],
+                                       data[(c0 + c) * size_t(N) * N * N + (row + 1) * N + nn]);
+        }
+        __syncthreads();
+        fft_tile<T, N, NG * RP, LS, NT, CF::RMAX, -1>(lds, tw, tid);
+        for (int c = 0; c < NG; c++)
+            for (int p = 0; p < RP; p++) {
+                const size_t row = size_t(q0 + p) * 2;
+                rows_pack_forward<T, N, LS>(lds, c * RP + p, out + (c0 + c) * PS + row * NXH,
+                                            out + (c0 + c) * PS + (row + 1) * NXH, tid, NT);
+            }
         __syncthreads();
     }
 }
@@ -328,11 +386,11 @@ k_rows_c2r(const C2<T>* __restrict__ data, T* __restrict__ out, const C2<T>* __r
 // v0 = omega(t); vp = previous stage (stage 2: v1, stage 3: v2); dst receives
 // the new stage (stage 3 writes v0 in place and applies the final projection).
 template <typename T, int N, int STAGE>
-__global__ void __launch_bounds__((Cfg<T, N>::NT))
+__global__ void __launch_bounds__((Cfg<T, N>::NT_Y3))
 k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __restrict__ vp,
                C2<T>* dst, const C2<T>* __restrict__ tw, Geom g, StepParams sp) {
     using CF = Cfg<T, N>;
-    constexpr int B = CF::B, NT = CF::NT, E = CF::E, NXH = CF::NXH;
+    constexpr int B = CF::B_Y3, NT = CF::NT_Y3, E = CF::E, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
     using F = RegFft<T, N, B, B, NT, -1>;
     __shared__ C2<T> lds[N * B];

@@ -251,12 +251,17 @@ struct BrickCfg {
     static constexpr bool ok = W <= 16 && TILE_BYTES <= 80 * 1024;
 #endif
     static constexpr int LP = 16;
-    static constexpr int NT = 512;
+    static constexpr int NT = 256;
     static constexpr int GROUPS = NT / LP;
     // tracers per batch (their results are kept in LDS)
-    template <int NC> static constexpr int pb() { return NC <= 3 ? 4 * GROUPS : 2 * GROUPS; }
+    template <int NC> static constexpr int pb() {
+        return (NC <= 3 && sizeof(T) == 4) ? 8 * GROUPS : 4 * GROUPS;  // two bricks per CU fit in LDS
+    }
     template <int NC> static constexpr size_t lds_bytes(int deg) {
-        return (size_t(pb<NC>()) * NC + size_t(W) * (deg + 1)) * sizeof(double) + TILE_BYTES;
+        // res [PB][NC] + frac [PB][3] + weight table (doubles), cells [PB][3]
+        // (ints, padded to doubles), tile
+        return (size_t(pb<NC>()) * (NC + 3) + size_t(W) * (deg + 1) + (size_t(pb<NC>()) * 3 + 1) / 2) * sizeof(double) +
+               TILE_BYTES;
     }
 };
 
@@ -283,7 +288,11 @@ __global__ void k_brick_bounds(const uint32_t* __restrict__ skeys, int64_t np, u
 }
 
 // MODE 0: velocity + Adams-Bashforth into the history ring; MODE 1: write
-// the NC sampled components to out[p][NC]
+// the NC sampled components to out[p][NC].
+// Per batch of up to PB tracers: a prologue loads each tracer's position once
+// (thread q owns tracer q of the batch) and leaves its cell offset in the
+// brick and in-cell fractions in LDS; the component passes then read only
+// LDS, so no dependent global load sits inside them.
 template <typename T, int NB, int NC, int MODE>
 __global__ void __launch_bounds__((BrickCfg<T, NB>::NT))
 k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ rhs, double* __restrict__ out,
@@ -293,10 +302,13 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
     using BC = BrickCfg<T, NB>;
     constexpr int W = BC::W, LP = BC::LP, NT = BC::NT, PB = BC::template pb<NC>();
     constexpr int WX = BC::WX, WY = BC::WY, WZ = BC::WZ;
+    static_assert(PB <= NT, "one tracer per thread in the batch prologue");
     extern __shared__ double smem[];
     double* res = smem;                          // [PB][NC]
-    double* stab = res + PB * NC;                // weight table, derivative 0
-    T* tile = reinterpret_cast<T*>(stab + W * (deg + 1));
+    double* frac = res + PB * NC;                // [PB][3] in-cell positions
+    double* stab = frac + PB * 3;                // weight table, derivative 0
+    int* lcell = reinterpret_cast<int*>(stab + W * (deg + 1));  // [PB][3] cell in brick
+    T* tile = reinterpret_cast<T*>(lcell + ((PB * 3 + 1) / 2) * 2);
     const int64_t brick = xcd_block();
     if (brick >= int64_t(nbx) * nby * (pg.n / BC::KZ)) return;
     const uint32_t s0 = bstart[brick], e0 = bend[brick];
@@ -307,113 +319,143 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
     const int n = pg.n;
     const int bz = int(brick / (int64_t(nbx) * nby)), by = int((brick / nbx) % nby), bx = int(brick % nbx);
     const int x0 = bx * BC::KX - NB, y0 = by * BC::KY - NB, z0 = bz * BC::KZ - NB;
+    const int base[3] = {x0 + NB, y0 + NB, z0 + NB};
     const size_t ps = size_t(n) * n * n;
     for (uint32_t bs = s0; bs < e0; bs += PB) {
         const int cnt = int(min(uint32_t(PB), e0 - bs));
+        __syncthreads();  // previous batch done with frac / lcell / res
+        int64_t myp = -1;
+        double myx[3] = {0.0, 0.0, 0.0};
+        if (tid < cnt) {
+            myp = perm[bs + tid];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double sp = pos[myp * 3 + d];
+                myx[d] = sp;
+                // get_norm_pos_in_cell / pbc_field_layer
+                const double rep = floor(sp / pg.width[d]);
+                const double inbox = sp - rep * pg.width[d];
+                const double ci = floor(inbox / pg.h[d]);
+                frac[tid * 3 + d] = (inbox - ci * pg.h[d]) / pg.h[d];
+                lcell[tid * 3 + d] = cell_of(sp, pg.h[d], n) - base[d];
+            }
+        }
         for (int c = 0; c < NC; c++) {
-            __syncthreads();  // previous users of the tile / res are done
-            // tile fill in chunks of CH loads per thread, all issued before
-            // the chunk's LDS stores
+            __syncthreads();  // tile free; prologue visible
+            // tile fill in two halves; all loads of a half are issued before
+            // its LDS stores
             constexpr int PERT = (BC::TILE + NT - 1) / NT;
-            constexpr int CH = 12;
+            constexpr int CH = (PERT + 1) / 2;
+            const T* fc = f + c * ps;
+#pragma unroll
             for (int k0 = 0; k0 < PERT; k0 += CH) {
                 T vals[CH];
 #pragma unroll
                 for (int k = 0; k < CH; k++) {
                     const int i = tid + (k0 + k) * NT;
-                    if (i < BC::TILE) {
+                    if (k0 + k < PERT && i < BC::TILE) {
                         const int lx = i % WX, lz = (i / WX) % WZ, ly = i / (WX * WZ);
                         int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
                         gx = gx < 0 ? gx + n : (gx >= n ? gx - n : gx);
                         gy = gy < 0 ? gy + n : (gy >= n ? gy - n : gy);
                         gz = gz < 0 ? gz + n : (gz >= n ? gz - n : gz);
-                        vals[k] = f[c * ps + (size_t(gy) * n + gz) * n + gx];
+                        vals[k] = fc[(uint32_t(gy) * uint32_t(n) + uint32_t(gz)) * uint32_t(n) + uint32_t(gx)];
                     }
                 }
 #pragma unroll
                 for (int k = 0; k < CH; k++) {
                     const int i = tid + (k0 + k) * NT;
-                    if (i < BC::TILE) tile[i] = vals[k];
+                    if (k0 + k < PERT && i < BC::TILE) tile[i] = vals[k];
                 }
             }
             __syncthreads();
             for (int q = grp; q < cnt; q += NT / LP) {  // whole lane groups iterate together
-                const int64_t p = perm[bs + q];
-                // lane jx evaluates weight jx of each axis (get_norm_pos_in_cell)
+                // lane jx evaluates weight jx of each axis; the group shares them
                 double wv[3] = {0.0, 0.0, 0.0};
-                int lc[3];
-                const int base[3] = {x0 + NB, y0 + NB, z0 + NB};
+                if (jx < W) {
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const double sp = pos[p * 3 + d];
-                    const double rep = floor(sp / pg.width[d]);
-                    const double inbox = sp - rep * pg.width[d];
-                    const double ci = floor(inbox / pg.h[d]);
-                    const double xr = (inbox - ci * pg.h[d]) / pg.h[d];
-                    if (jx < W) {
+                    for (int d = 0; d < 3; d++) {
+                        const double xr = frac[q * 3 + d];
                         double a = 0.0;
                         for (int k = deg; k >= 0; k--) a = a * xr + stab[jx * (deg + 1) + k];
                         wv[d] = a;
                     }
-                    lc[d] = cell_of(sp, pg.h[d], n) - base[d];  // cell within the brick
                 }
-                // every lane needs all y and z weights of its tracer
-                double wy[W], wz[W];
+                double wz[W];
 #pragma unroll
-                for (int j = 0; j < W; j++) {
-                    wy[j] = __shfl(wv[1], j, LP);
-                    wz[j] = __shfl(wv[2], j, LP);
-                }
+                for (int j = 0; j < W; j++) wz[j] = __shfl(wv[2], j, LP);
+                const int lc0 = lcell[q * 3 + 0], lc1 = lcell[q * 3 + 1], lc2 = lcell[q * 3 + 2];
                 double acc = 0.0;
-                if (jx < W) {
-                    const T* tcol = tile + lc[0] + jx;
-                    for (int jz = 0; jz < W; jz++) {
-                        double col = 0.0;
+                const T* tcol = tile + lc0 + (jx < W ? jx : 0);
+                if constexpr (sizeof(T) == 4) {
+                    // FP32 fields: the y sums of FP32 values run in FP32,
+                    // the z sums and the result in FP64
+                    float wyf[W];
 #pragma unroll
-                        for (int jy = 0; jy < W; jy++)
-                            col = fma(wy[jy], double(tcol[((lc[1] + jy) * WZ + lc[2] + jz) * WX]), col);
-                        acc = fma(wz[jz], col, acc);
+                    for (int j = 0; j < W; j++) wyf[j] = __shfl(float(wv[1]), j, LP);
+                    if (jx < W) {
+                        for (int jz = 0; jz < W; jz++) {
+                            float col = 0.0f;
+#pragma unroll
+                            for (int jy = 0; jy < W; jy++)
+                                col = fmaf(wyf[jy], tcol[((lc1 + jy) * WZ + lc2 + jz) * WX], col);
+                            acc = fma(wz[jz], double(col), acc);
+                        }
                     }
-                    acc *= wv[0];
+                } else {
+                    double wy[W];
+#pragma unroll
+                    for (int j = 0; j < W; j++) wy[j] = __shfl(wv[1], j, LP);
+                    if (jx < W) {
+                        for (int jz = 0; jz < W; jz++) {
+                            double col = 0.0;
+#pragma unroll
+                            for (int jy = 0; jy < W; jy++)
+                                col = fma(wy[jy], double(tcol[((lc1 + jy) * WZ + lc2 + jz) * WX]), col);
+                            acc = fma(wz[jz], col, acc);
+                        }
+                    }
                 }
+                acc *= wv[0];
 #pragma unroll
                 for (int o = LP / 2; o > 0; o /= 2) acc += __shfl_xor(acc, o, LP);
                 if (jx == 0) res[q * NC + c] = acc;
             }
         }
         __syncthreads();
-        for (int q = tid; q < cnt; q += NT) {
-            const int64_t p = perm[bs + q];
+        if (tid < cnt) {
+            const int q = tid;
+            const int64_t p = myp;
             if (MODE == 1) {
 #pragma unroll
                 for (int c = 0; c < NC; c++) out[p * NC + c] = res[q * NC + c];
-                continue;
-            }
-            double r0[3] = {res[q * NC + 0], res[q * NC + 1], res[q * NC + 2]};
-            double* h0 = rhs + size_t(head) * np * 3 + p * 3;
-            if (add_head) {
-                r0[0] += h0[0];
-                r0[1] += h0[1];
-                r0[2] += h0[2];
-            }
-            h0[0] = r0[0];
-            h0[1] = r0[1];
-            h0[2] = r0[2];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                double r[6];
-                r[0] = r0[d];
-                for (int i = 1; i < order; i++) r[i] = rhs[size_t((head + i) % nsteps) * np * 3 + p * 3 + d];
-                double incr;
-                switch (order) {
-                case 1: incr = dt * r[0]; break;
-                case 2: incr = dt * (3. * r[0] - r[1]) / 2.; break;
-                case 3: incr = dt * (23. * r[0] - 16. * r[1] + 5. * r[2]) / 12.; break;
-                case 4: incr = dt * (55. * r[0] - 59. * r[1] + 37. * r[2] - 9. * r[3]) / 24.; break;
-                case 5: incr = dt * (1901. * r[0] - 2774. * r[1] + 2616. * r[2] - 1274. * r[3] + 251. * r[4]) / 720.; break;
-                default: incr = dt * (4277. * r[0] - 7923. * r[1] + 9982. * r[2] - 7298. * r[3] + 2877. * r[4] - 475. * r[5]) / 1440.; break;
+            } else {
+                double r0[3] = {res[q * NC + 0], res[q * NC + 1], res[q * NC + 2]};
+                double* h0 = rhs + size_t(head) * np * 3 + p * 3;
+                if (add_head) {
+                    r0[0] += h0[0];
+                    r0[1] += h0[1];
+                    r0[2] += h0[2];
                 }
-                pos[p * 3 + d] += incr;
+                h0[0] = r0[0];
+                h0[1] = r0[1];
+                h0[2] = r0[2];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    double r[6];
+                    r[0] = r0[d];
+                    for (int i = 1; i < order; i++) r[i] = rhs[size_t((head + i) % nsteps) * np * 3 + p * 3 + d];
+                    double incr;
+                    switch (order) {
+                    case 1: incr = dt * r[0]; break;
+                    case 2: incr = dt * (3. * r[0] - r[1]) / 2.; break;
+                    case 3: incr = dt * (23. * r[0] - 16. * r[1] + 5. * r[2]) / 12.; break;
+                    case 4: incr = dt * (55. * r[0] - 59. * r[1] + 37. * r[2] - 9. * r[3]) / 24.; break;
+                    case 5: incr = dt * (1901. * r[0] - 2774. * r[1] + 2616. * r[2] - 1274. * r[3] + 251. * r[4]) / 720.; break;
+                    default: incr = dt * (4277. * r[0] - 7923. * r[1] + 9982. * r[2] - 7298. * r[3] + 2877. * r[4] - 475. * r[5]) / 1440.; break;
+                    }
+                    pos[p * 3 + d] = myx[d] + incr;
+                }
             }
         }
     }

@@ -97,6 +97,34 @@ int bfpsh_gradient_rspace(bfpsh_ctx* ctx);
  * nshells*9 doubles, [shell][i][j]. */
 int bfpsh_cospectrum(bfpsh_ctx* ctx, int a, int b, double* out);
 
+/* Standalone device field of ncomp (1, 3 or 9) components with the
+ * reference's in-place transforms: replaces the construction of
+ * field<rnumber, FFTW, fc> (bfps/cpp/field.cpp:37-121).  *id receives the
+ * handle; the field starts at zero in the spectral representation.  Modes
+ * kx = nx/2 are not represented (they lie outside every dealiasing sphere;
+ * put ignores them, get returns zeros). */
+int bfpsh_field_create(bfpsh_ctx* ctx, int ncomp, int* id);
+/* Replaces ~field (bfps/cpp/field.cpp:123-142). */
+int bfpsh_field_destroy(bfpsh_ctx* ctx, int id);
+/* Replaces field::ift (bfps/cpp/field.cpp:144-152): unnormalised c2r in place. */
+int bfpsh_field_ift(bfpsh_ctx* ctx, int id);
+/* Replaces field::dft (bfps/cpp/field.cpp:154-162): unnormalised r2c in place. */
+int bfpsh_field_dft(bfpsh_ctx* ctx, int id);
+/* Replaces field::symmetrize (bfps/cpp/field.cpp:900-978). */
+int bfpsh_field_symmetrize(bfpsh_ctx* ctx, int id);
+/* Host copies in the reference layouts: real_space != 0 -> rnumber
+ * [nz][ny][nx][ncomp]; 0 -> rnumber {re, im} [ny][nz][nx/2+1][ncomp]
+ * (field::get_rdata / get_cdata, bfps/cpp/field.hpp:135-146). */
+int bfpsh_field_put(bfpsh_ctx* ctx, int id, int real_space, const void* host);
+int bfpsh_field_get(bfpsh_ctx* ctx, int id, int real_space, void* host);
+/* Replaces `*tmp_vec_field = fs->cvorticity->get_cdata()` and the same for
+ * cvelocity (bfps/cpp/full_code/NSVE.cpp:115,124): which = BFPSH_CVORTICITY
+ * or BFPSH_CVELOCITY into a 3-component field. */
+int bfpsh_field_copy_from(bfpsh_ctx* ctx, int id, int which);
+/* kspace::cospectrum (bfps/cpp/kspace.cpp:521-598) of two 3-component
+ * standalone fields (spectral); out nshells*9 doubles. */
+int bfpsh_field_cospectrum(bfpsh_ctx* ctx, int id_a, int id_b, double* out);
+
 /* Replaces particles_system_builder(...) (bfps/cpp/particles/particles_system_builder.hpp:238-257)
  * for one rank: np tracers, interpolation neighbours n in [1,10] and
  * smoothness m in [0,2] (particles_generic_interp.hpp:10-313), nsteps

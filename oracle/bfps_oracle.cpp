@@ -942,6 +942,16 @@ void orc_fft3d(int nx, int ny, int nz, int nc, int fp32, int inverse, const doub
     if (fp32) go(float()); else go(double());
 }
 
+// field::symmetrize (field.cpp:900-978) of a standalone complex field
+// [ny][nz][nx/2+1][nc] {re, im}, in place.
+void orc_symmetrize(int nx, int ny, int nz, int nc, double* data) {
+    Field<double> f(nx, ny, nz, nc);
+    const size_t cs = size_t(ny) * nz * (nx / 2 + 1) * nc * 2;
+    std::copy(data, data + cs, f.data.begin());
+    symmetrize(f);
+    std::copy(f.data.begin(), f.data.begin() + cs, data);
+}
+
 int orc_interp_table(int n, int m, double* out /* may be null */) {
     std::vector<double> tab;
     int deg = build_table(n, m, tab);

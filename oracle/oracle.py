@@ -62,6 +62,7 @@ def lib():
         L.orc_step.argtypes = [P, D]
         L.orc_cospectrum.argtypes = [P, I, I, dp]
         L.orc_fft3d.argtypes = [I, I, I, I, I, I, dp, dp]
+        L.orc_symmetrize.argtypes = [I, I, I, I, dp]
         L.orc_interp_table.argtypes = [I, I, dp]
         L.orc_interp_table.restype = I
         L.orc_part_new.restype = P
@@ -102,6 +103,14 @@ def fft3d_c2r(cplx, nx, fp32=False):
     ny, nz, nxh, nc = cplx.shape
     out = np.zeros((nz, ny, nx, nc), dtype=np.float64)
     lib().orc_fft3d(nx, ny, nz, nc, int(fp32), 1, _dp(cplx.view(np.float64)), _dp(out))
+    return out
+
+
+def symmetrize(cplx, nx):
+    """field::symmetrize of a complex field [ny][nz][nx//2+1][nc] (copy)."""
+    out = np.ascontiguousarray(cplx, dtype=np.complex128).copy()
+    ny, nz, nxh, nc = out.shape
+    lib().orc_symmetrize(nx, ny, nz, nc, _dp(out.view(np.float64)))
     return out
 
 

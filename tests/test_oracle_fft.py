@@ -46,3 +46,28 @@ def test_brute_force_dft_tiny(oracle):
             for kx in range(n // 2 + 1):
                 ph = np.exp(-2j * np.pi * (kx * x + ky * y + kz * z) / n)
                 assert abs((r[..., 0] * ph).sum() - c[ky, kz, kx, 0]) < 1e-12
+
+
+def test_symmetrize_pins(oracle):
+    """field::symmetrize (bfps/cpp/field.cpp:900-978): identity on the r2c of a
+    real field; on arbitrary data it makes the kx = 0 plane Hermitian for
+    every mode it writes and touches nothing with kx > 0."""
+    rng = np.random.default_rng(4)
+    for (nx, ny, nz, nc) in [(8, 8, 8, 3), (12, 10, 6, 1), (16, 16, 16, 9)]:
+        r = rng.standard_normal((nz, ny, nx, nc))
+        c = np.fft.rfftn(r, axes=(0, 1, 2)).transpose(1, 0, 2, 3)
+        s = oracle.symmetrize(c, nx)
+        assert np.abs(s - c).max() <= 1e-13 * np.abs(c).max()
+        a = rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)
+        s = oracle.symmetrize(a, nx)
+        assert np.array_equal(s[:, :, 1:], a[:, :, 1:])
+        assert s[0, 0, 0].imag.max() == 0 and np.array_equal(s[0, 0, 0].real, a[0, 0, 0].real)
+        for ky in range(ny):
+            for kz in range(nz):
+                my, mz = (-ky) % ny, (-kz) % nz
+                if ky > ny // 2 or (ky == 0 and kz > nz // 2):
+                    # written: conjugate of the mirror, which is never written
+                    assert np.array_equal(s[ky, kz, 0], np.conj(a[my, mz, 0]))
+                    assert np.array_equal(s[my, mz, 0], a[my, mz, 0])
+                elif not (ky == 0 and kz == 0):
+                    assert np.array_equal(s[ky, kz, 0], a[ky, kz, 0])
