@@ -219,7 +219,8 @@ class field:
 
     def __init__(self, fs, fc="THREE"):
         self.fs = fs
-        self.ncomp = self._FC[fc]
+        # anything else reaches the C ABI, which rejects it
+        self.ncomp = self._FC.get(fc, fc if isinstance(fc, int) else 0)
         self._id = ctypes.c_int(-1)
         capi.check(fs.handle, capi.load().bfpsh_field_create(fs.handle, self.ncomp, ctypes.byref(self._id)),
                    "field_create")
@@ -365,6 +366,15 @@ class particles_system:
         out = np.zeros((self.np, nc))
         rc = capi.load().bfpsh_particles_sample(self.fs.handle, sel, out.ctypes.data_as(capi._DP))
         capi.check(self.fs.handle, rc, "particles_sample")
+        return out
+
+    def sample_compute_field(self, f, out=None):
+        """sample_compute_field of a standalone field in real space
+        (bfps/cpp/particles/particles_system.hpp:143-170): [np][ncomp]."""
+        if out is None:
+            out = np.zeros((self.np, f.ncomp))
+        rc = capi.load().bfpsh_particles_sample_field(self.fs.handle, f.id, out.ctypes.data_as(capi._DP))
+        capi.check(self.fs.handle, rc, "particles_sample_field")
         return out
 
 

@@ -83,6 +83,24 @@ def build(jobs=None, sizes=SIZES, verbose=True, out=None, defines=()):
     return lib
 
 
+DRIVER_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "nsve_particles.cpp")
+DRIVER = os.path.join(os.path.dirname(HERE), "tests", "cpp", "nsve_particles")
+
+
+def build_driver(lib=None, out=None, verbose=True):
+    """Compile the C++ NSVEparticles-style driver (host code only, g++)
+    against include/bfps_hip.hpp and libbfps_hip.so."""
+    lib = lib or LIB
+    out = out or DRIVER
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I" + inc, DRIVER_SRC, "-o", out,
+           lib, "-Wl,-rpath," + os.path.dirname(os.path.abspath(lib))]
+    subprocess.check_call(cmd)
+    if verbose:
+        print("built %s" % out)
+    return out
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", "--jobs", type=int, default=None)
@@ -91,4 +109,6 @@ if __name__ == "__main__":
     ap.add_argument("-D", dest="defines", action="append", default=[], help="NAME=VALUE")
     a = ap.parse_args()
     sz = tuple(int(s) for s in a.sizes.split(",")) if a.sizes else SIZES
-    build(a.jobs, sz, out=a.out, defines=tuple(a.defines))
+    lib = build(a.jobs, sz, out=a.out, defines=tuple(a.defines))
+    if not a.out:
+        build_driver(lib)

@@ -51,6 +51,7 @@ SIGNATURES = {
     "bfpsh_particles_get": (_I, [_P, _DP, _DP, _LP, _IP]),
     "bfpsh_particles_complete_loop": (_I, [_P, _D]),
     "bfpsh_particles_sample": (_I, [_P, _I, _DP]),
+    "bfpsh_particles_sample_field": (_I, [_P, _I, _DP]),
     "bfpsh_nsve_particles_step": (_I, [_P, _D]),
     "bfpsh_synchronize": (_I, [_P]),
     "bfpsh_get_iteration": (_I, [_P, _IP]),

@@ -322,6 +322,7 @@ int bfpsh_field_create(bfpsh_ctx* ctx, int ncomp, int* id) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->fields.push_back(p);
     ctx->field_nc.push_back(ncomp);
+    ctx->field_real.push_back(0);
     ctx->bytes += bytes;
     *id = int(ctx->fields.size()) - 1;
     return BFPSH_OK;
@@ -340,12 +341,14 @@ int bfpsh_field_destroy(bfpsh_ctx* ctx, int id) {
 int bfpsh_field_ift(bfpsh_ctx* ctx, int id) {
     CTX_CHECK();
     if (!field_ok(ctx, id)) return fail(ctx, BFPSH_EINVAL, "bad field id");
+    ctx->field_real[id] = 1;
     return PREC_CALL(field_ift, ctx, ctx->fields[id], ctx->field_nc[id]);
 }
 
 int bfpsh_field_dft(bfpsh_ctx* ctx, int id) {
     CTX_CHECK();
     if (!field_ok(ctx, id)) return fail(ctx, BFPSH_EINVAL, "bad field id");
+    ctx->field_real[id] = 0;
     return PREC_CALL(field_dft, ctx, ctx->fields[id], ctx->field_nc[id]);
 }
 
@@ -375,6 +378,7 @@ int bfpsh_field_put(bfpsh_ctx* ctx, int id, int real_space, const void* host) {
     }
     HIPCHK(hipMemcpyAsync(ctx->fields[id], tmp.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->field_real[id] = real_space ? 1 : 0;
     return BFPSH_OK;
 }
 
@@ -410,6 +414,7 @@ int bfpsh_field_copy_from(bfpsh_ctx* ctx, int id, int which) {
         return fail(ctx, BFPSH_EINVAL, "bad field selector");
     }
     HIPCHK(hipMemcpyAsync(ctx->fields[id], src, 3 * ctx->ps * 2 * ctx->es, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->field_real[id] = 0;
     return BFPSH_OK;
 }
 
@@ -526,17 +531,13 @@ int bfpsh_particles_complete_loop(bfpsh_ctx* ctx, double dt) {
     return bfpsh_part_loop(ctx, dt);
 }
 
-int bfpsh_particles_sample(bfpsh_ctx* ctx, int which, double* out) {
-    CTX_CHECK();
-    if (!out) return fail(ctx, BFPSH_EINVAL, "null output");
-    if (which != BFPSH_RVELOCITY && which != BFPSH_RGRADIENT) return fail(ctx, BFPSH_EINVAL, "bad field selector");
-    if ((which == BFPSH_RVELOCITY && !ctx->ru) || (which == BFPSH_RGRADIENT && !ctx->rgrad))
-        return fail(ctx, BFPSH_ESTATE, "real field not computed");
+// sample `nc` real planes [c][y][z][x] at every tracer; out [np][nc] in
+// global-index order
+static int sample_to_host(bfpsh_ctx* ctx, const void* f, int nc, double* out) {
     if (ctx->np == 0) return BFPSH_OK;
-    const int nc = which == BFPSH_RVELOCITY ? 3 : 9;
     double* d_out = nullptr;
     HIPCHK(hipMalloc((void**)&d_out, ctx->np * nc * sizeof(double)));
-    int r = bfpsh_part_sample(ctx, which, d_out);
+    int r = bfpsh_part_sample(ctx, f, nc, d_out);
     if (r) { (void)hipFree(d_out); return r; }
     std::vector<double> tmp(ctx->np * nc);
     HIPCHK(hipMemcpyAsync(tmp.data(), d_out, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -548,6 +549,28 @@ int bfpsh_particles_sample(bfpsh_ctx* ctx, int which, double* out) {
     for (int64_t i = 0; i < ctx->np; i++)
         for (int c = 0; c < nc; c++) out[i * nc + c] = tmp[order[i] * nc + c];
     return BFPSH_OK;
+}
+
+int bfpsh_particles_sample(bfpsh_ctx* ctx, int which, double* out) {
+    CTX_CHECK();
+    if (!out) return fail(ctx, BFPSH_EINVAL, "null output");
+    if (which != BFPSH_RVELOCITY && which != BFPSH_RGRADIENT) return fail(ctx, BFPSH_EINVAL, "bad field selector");
+    if ((which == BFPSH_RVELOCITY && !ctx->ru) || (which == BFPSH_RGRADIENT && !ctx->rgrad))
+        return fail(ctx, BFPSH_ESTATE, "real field not computed");
+    if (ctx->nsteps == 0) return fail(ctx, BFPSH_ESTATE, "no tracers");
+    return sample_to_host(ctx, which == BFPSH_RVELOCITY ? ctx->ru : ctx->rgrad, which == BFPSH_RVELOCITY ? 3 : 9,
+                          out);
+}
+
+int bfpsh_particles_sample_field(bfpsh_ctx* ctx, int id, double* out) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id) || !out) return fail(ctx, BFPSH_EINVAL, "bad field id or buffer");
+    if (ctx->nsteps == 0) return fail(ctx, BFPSH_ESTATE, "no tracers");
+    // sample_compute_field of a one-component float field does nothing
+    // (bfps/cpp/particles/particles_system.hpp:147-150)
+    if (ctx->field_nc[id] == 1 && ctx->prec == BFPSH_FP32) return BFPSH_OK;
+    if (!ctx->field_real[id]) return fail(ctx, BFPSH_ESTATE, "field is not in real space (call ift first)");
+    return sample_to_host(ctx, ctx->fields[id], ctx->field_nc[id], out);
 }
 
 int bfpsh_nsve_particles_step(bfpsh_ctx* ctx, double dt) {

@@ -54,6 +54,7 @@ struct bfpsh_ctx {
     // standalone fields (field<rnumber, HIP, fc>): device buffer + ncomp per id
     std::vector<void*> fields;
     std::vector<int> field_nc;
+    std::vector<char> field_real;  // 1 when the field holds real-space data
     // spatial ordering of the tracers (a permutation, data stays in place)
     uint32_t* d_keys = nullptr;    // 2 x np
     uint32_t* d_perm = nullptr;    // 2 x np
@@ -190,7 +191,7 @@ struct OpsTable {
 // tracer loop and sampling (particles.hip), dispatched on precision and
 // neighbour count
 int bfpsh_part_loop(bfpsh_ctx* ctx, double dt);
-int bfpsh_part_sample(bfpsh_ctx* ctx, int which, double* d_out);
+int bfpsh_part_sample(bfpsh_ctx* ctx, const void* field, int nc, double* d_out);
 int bfpsh_part_release(bfpsh_ctx* ctx);
 
 #define BFPSH_FOR_SIZES(X) X(16) X(32) X(48) X(64) X(128) X(256) X(512) X(1024)

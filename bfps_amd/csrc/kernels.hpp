@@ -55,11 +55,7 @@ struct Cfg {
     static constexpr int B_BS = (NT > 512 && B > 1) ? B / 2 : B;
     static constexpr int NT_BS = G * B_BS;
     // last forward pass with the update epilogue (3 components in registers)
-#ifdef BFPSH_Y3_NARROW
-    static constexpr int B_Y3 = B_BS;
-#else
     static constexpr int B_Y3 = B;
-#endif
     static constexpr int NT_Y3 = G * B_Y3;
     static constexpr int RP = (N * int(sizeof(C2<T>)) >= 8192) ? 1 : 2;  // row pairs per row block
     static constexpr int NT_ROW = N >= 256 ? 256 : (N >= 64 ? 128 : 64);

@@ -426,10 +426,11 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
         if (tid < cnt) {
             const int q = tid;
             const int64_t p = myp;
-            if (MODE == 1) {
+            if constexpr (MODE == 1) {
 #pragma unroll
                 for (int c = 0; c < NC; c++) out[p * NC + c] = res[q * NC + c];
             } else {
+                static_assert(NC == 3, "the Adams-Bashforth epilogue moves 3 coordinates");
                 double r0[3] = {res[q * NC + 0], res[q * NC + 1], res[q * NC + 2]};
                 double* h0 = rhs + size_t(head) * np * 3 + p * 3;
                 if (add_head) {
@@ -583,6 +584,8 @@ int sample_nb(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
     if constexpr (BrickCfg<T, NB>::ok) {
         int r = sort_bricks<T, NB>(ctx);
         if (r) return r;
+        if constexpr (sizeof(T) == 8)
+            if (nc == 1) return launch_brick<T, NB, 1, 1>(ctx, f, d_out, 0.0, 0, "sample1");
         if (nc == 3) return launch_brick<T, NB, 3, 1>(ctx, f, d_out, 0.0, 0, "sample");
         return launch_brick<T, NB, 9, 1>(ctx, f, d_out, 0.0, 0, "sample9");
     }
@@ -591,6 +594,13 @@ int sample_nb(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
         if (r) return r;
     }
     const size_t lds = size_t(2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
+    if constexpr (sizeof(T) == 8)
+        if (nc == 1) {
+            LAUNCH("sample1", (k_sample<T, NB, 1><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
+                                  (const T*)f, ctx->d_pos, d_out, ctx->d_perm + ctx->np, ctx->np, ctx->d_tab, ctx->deg,
+                                  pgeom_d(ctx))));
+            return BFPSH_OK;
+        }
     if (nc == 3)
         LAUNCH("sample", (k_sample<T, NB, 3><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
                              (const T*)f, ctx->d_pos, d_out, ctx->d_perm + ctx->np, ctx->np, ctx->d_tab, ctx->deg,
@@ -667,9 +677,9 @@ int bfpsh_part_loop(bfpsh_ctx* ctx, double dt) {
     return BFPSH_OK;
 }
 
-int bfpsh_part_sample(bfpsh_ctx* ctx, int which, double* d_out) {
+int bfpsh_part_sample(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
     if (ctx->np == 0) return BFPSH_OK;
-    const void* f = which == BFPSH_RVELOCITY ? ctx->ru : ctx->rgrad;
-    const int nc = which == BFPSH_RVELOCITY ? 3 : 9;
+    if (nc != 3 && nc != 9 && !(nc == 1 && ctx->prec == BFPSH_FP64))
+        return fail(ctx, BFPSH_EINVAL, "sampling needs 3 or 9 components (or 1 in FP64)");
     return ctx->prec == BFPSH_FP32 ? sample_t<float>(ctx, f, nc, d_out) : sample_t<double>(ctx, f, nc, d_out);
 }

@@ -143,6 +143,12 @@ int bfpsh_particles_complete_loop(bfpsh_ctx* ctx, double dt);
  * velocity (3 comps, BFPSH_RVELOCITY) or gradient (9 comps, BFPSH_RGRADIENT);
  * out [np][ncomp], in the current (index-sorted) order. */
 int bfpsh_particles_sample(bfpsh_ctx* ctx, int which, double* out);
+/* Replaces particles_system::sample_compute_field(field<rnumber, FFTW, fc>)
+ * (bfps/cpp/particles/particles_system.hpp:143-170) for a standalone field
+ * in real space: out [np][ncomp] in global-index order.  As in the
+ * reference, a one-component FP32 field is not sampled (out is left
+ * untouched). */
+int bfpsh_particles_sample_field(bfpsh_ctx* ctx, int id, double* out);
 
 /* Replaces NSVEparticles::step (bfps/cpp/full_code/NSVEparticles.cpp:33-41):
  * velocity of omega(t) -> tracers completeLoop(dt) -> vorticity step(dt),

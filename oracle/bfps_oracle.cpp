@@ -29,10 +29,31 @@
 // Arithmetic types follow the reference: field data in rnumber (float or
 // double), wavenumbers / k^2 / factors in double, particle data in double.
 //
-// Parity status: FFT pinned against numpy.fft; solver pinned by the
-// reference's golden NSVEparticles checkpoint (32^3, 64 steps) and by closed
-// forms (Beltrami decay, linear decay); weights pinned by the reference's
-// compiled spline sources.
+// Parity status (each check lives under tests/):
+//   * 3D FFT: numpy.fft (pocketfft) on seven shapes incl. radix 3 and 5,
+//     brute-force DFT on 4^3 (test_oracle_fft.py)
+//   * symmetrize: identity on Hermitian data, per-mode write pattern
+//     (test_oracle_fft.py)
+//   * whole NSVEparticles pipeline, SMOOTH, linear forcing, n1m1, AB4 with
+//     its start-up ramp: the reference's golden checkpoint B32p1e4 (32^3,
+//     64 steps, restart at 32), FP32 and FP64 (test_oracle_fixture.py)
+//   * step / compute_velocity / gradient / forcing / projections /
+//     cospectrum / kshell: closed forms -- Beltrami decay and analytic u and
+//     grad u, RK3 stability polynomial of a forced single wave, Kolmogorov
+//     forcing = Simpson's rule, low_pass + divfree, Parseval, lattice counts
+//     (test_oracle_closed_forms.py)
+//   * interpolation weights: the reference's spline_n*.cpp / Lagrange_polys.cpp
+//     compiled from source, plus partition of unity and polynomial
+//     reproduction (test_interp_weights.py); gather + AB1..AB6: convergence
+//     to the analytic ABC field and order-k convergence of AB
+//     (test_oracle_closed_forms.py)
+//   * nonlinear term and dealiasing: the initial rate of a triad
+//     k1 + k2 -> k3 against its Fourier-coefficient closed form, with k3 where
+//     the SMOOTH filter is between 0.05 and 0.95, and across the TWO_THIRDS
+//     cut (test_oracle_closed_forms.py)
+//   * not reproducible bit for bit (last-bit effects, covered by
+//     tolerances): the reference's FFTW plan-dependent and -ffast-math
+//     rounding.
 #include <algorithm>
 #include <cmath>
 #include <complex>

@@ -173,3 +173,35 @@ def test_field_errors(B):
     assert lib.bfpsh_field_ift(fs.handle, i) != 0
     assert lib.bfpsh_field_destroy(fs.handle, i) != 0
     assert lib.bfpsh_field_dft(fs.handle, 12345) != 0
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_sample_standalone_fields(B, precision):
+    """sample_compute_field on standalone fields equals the tracer sampling
+    of the solver's own real-space velocity; a one-component float field is
+    left unsampled as in the reference."""
+    n, np_ = 32, 3000
+    say("case: sample standalone fields %s" % precision)
+    from bfps_amd import ic
+    fs = B.vorticity_equation("t", n, precision=precision)
+    fs.io_checkpoint(read=True, data=ic.solenoidal_vorticity(n, seed=4, k0=3.0))
+    ps = B.particles_system(fs, np_, 3, 1, 4)
+    ps.put(ic.uniform_positions(np_, seed=5))
+    fs.velocity_rspace()
+    ref = ps.sample("velocity")
+    fu = B.field(fs, 3)
+    fu.copy_from("velocity")
+    fu.ift()
+    got = ps.sample_compute_field(fu)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max() + (1e-6 if precision == "single" else 0)
+    with pytest.raises(B.capi.BfpshError):
+        fu.dft()
+        ps.sample_compute_field(fu)
+    f1 = B.field(fs, 1)
+    f1.put(fs.get_rfield(B.capi.RVELOCITY)[..., :1], real_space=True)
+    out = np.full((np_, 1), 7.0)
+    ps.sample_compute_field(f1, out)
+    if precision == "single":
+        assert np.all(out == 7.0)
+    else:
+        assert np.abs(out[:, 0] - ref[:, 0]).max() <= 1e-12 * np.abs(ref).max()
