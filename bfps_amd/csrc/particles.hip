@@ -342,10 +342,10 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
         }
         for (int c = 0; c < NC; c++) {
             __syncthreads();  // tile free; prologue visible
-            // tile fill in two halves; all loads of a half are issued before
-            // its LDS stores
+            // tile fill in chunks; all loads of a chunk are issued before its
+            // LDS stores
             constexpr int PERT = (BC::TILE + NT - 1) / NT;
-            constexpr int CH = (PERT + 1) / 2;
+            constexpr int CH = PERT < 16 ? PERT : 16;  // loads in flight per thread
             const T* fc = f + c * ps;
 #pragma unroll
             for (int k0 = 0; k0 < PERT; k0 += CH) {
@@ -381,12 +381,11 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
                         wv[d] = a;
                     }
                 }
-                double wz[W];
-#pragma unroll
-                for (int j = 0; j < W; j++) wz[j] = __shfl(wv[2], j, LP);
                 const int lc0 = lcell[q * 3 + 0], lc1 = lcell[q * 3 + 1], lc2 = lcell[q * 3 + 2];
                 double acc = 0.0;
-                const T* tcol = tile + lc0 + (jx < W ? jx : 0);
+                // column base of this lane (x = lc0 + jx) at (y = lc1, z = lc2)
+                const T* tcol = tile + lc0 + (jx < W ? jx : 0) + (lc1 * WZ + lc2) * WX;
+                constexpr int YS = WZ * WX;  // y stride of the tile
                 if constexpr (sizeof(T) == 4) {
                     // FP32 fields: the y sums of FP32 values run in FP32,
                     // the z sums and the result in FP64
@@ -395,11 +394,20 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
                     for (int j = 0; j < W; j++) wyf[j] = __shfl(float(wv[1]), j, LP);
                     if (jx < W) {
                         for (int jz = 0; jz < W; jz++) {
-                            float col = 0.0f;
+                            // z weight from lane jz; the W loads of the column
+                            // are issued together, then two FMA chains
+                            const double wzj = __shfl(wv[2], jz, LP);
+                            const T* tz = tcol + jz * WX;
+                            float t[W];
 #pragma unroll
-                            for (int jy = 0; jy < W; jy++)
-                                col = fmaf(wyf[jy], tcol[((lc1 + jy) * WZ + lc2 + jz) * WX], col);
-                            acc = fma(wz[jz], double(col), acc);
+                            for (int jy = 0; jy < W; jy++) t[jy] = tz[jy * YS];
+                            float c0 = 0.0f, c1 = 0.0f;
+#pragma unroll
+                            for (int jy = 0; jy < W; jy += 2) {
+                                c0 = fmaf(wyf[jy], t[jy], c0);
+                                if (jy + 1 < W) c1 = fmaf(wyf[jy + 1], t[jy + 1], c1);
+                            }
+                            acc = fma(wzj, double(c0 + c1), acc);
                         }
                     }
                 } else {
@@ -408,11 +416,18 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
                     for (int j = 0; j < W; j++) wy[j] = __shfl(wv[1], j, LP);
                     if (jx < W) {
                         for (int jz = 0; jz < W; jz++) {
-                            double col = 0.0;
+                            const double wzj = __shfl(wv[2], jz, LP);
+                            const T* tz = tcol + jz * WX;
+                            double t[W];
 #pragma unroll
-                            for (int jy = 0; jy < W; jy++)
-                                col = fma(wy[jy], double(tcol[((lc1 + jy) * WZ + lc2 + jz) * WX]), col);
-                            acc = fma(wz[jz], col, acc);
+                            for (int jy = 0; jy < W; jy++) t[jy] = double(tz[jy * YS]);
+                            double c0 = 0.0, c1 = 0.0;
+#pragma unroll
+                            for (int jy = 0; jy < W; jy += 2) {
+                                c0 = fma(wy[jy], t[jy], c0);
+                                if (jy + 1 < W) c1 = fma(wy[jy + 1], t[jy + 1], c1);
+                            }
+                            acc = fma(wzj, c0 + c1, acc);
                         }
                     }
                 }
