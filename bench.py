@@ -34,9 +34,10 @@ def step_bytes(n, es, with_tracer_u):
 
 
 # algorithmic bytes per launch of each kernel, in field volumes N^3 * es
+# ("inv_z_bs" is reported as one kernel: the kx tile holding kx = 0 runs as a
+# separate launch "inv_z_bs_t0" with the symmetrize mirror, merged below)
 KERNEL_VOLUMES = {
-    "inv_z_bs": 9 * 15 / 16,   # kx tiles 1..15 of 16: read omega (3 planes), write u, omega (6)
-    "inv_z_bs_t0": 9 / 16,     # kx tile 0 (with the symmetrize substitution)
+    "inv_z_bs": 9,         # read omega (3 planes), write u, omega after the z pass (6)
     "inv_y": 12,           # 6 planes read + written
     "rows_cross": 9,       # read 6, write 3 (+3 real velocity in stage 1)
     "fwd_z": 6,            # 3 planes read + written
@@ -44,6 +45,29 @@ KERNEL_VOLUMES = {
     "fwd_y_update2": 12,   # read N, v0, v1, write v2
     "fwd_y_update3": 12,   # read N, v0, v2, write v0
 }
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_512_fp32.txt")
+
+
+def merge_profile(prof):
+    """Fold the kx-tile-0 launch of the Biot-Savart pass into "inv_z_bs"."""
+    out = dict(prof)
+    if "inv_z_bs_t0" in out and "inv_z_bs" in out:
+        t0_ms, _ = out.pop("inv_z_bs_t0")
+        ms, cnt = out["inv_z_bs"]
+        out["inv_z_bs"] = (ms + t0_ms, cnt)
+    return out
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (FETCH_SIZE x calibration + WRITE_SIZE), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) == 2 and f[0] == kernel:
+            return float(f[1])
+    return None
 
 
 def tracer_bytes(np_, nsteps=4):
@@ -104,40 +128,81 @@ def run_precision(B, ic, args, precision, steps, warmup):
     return res
 
 
-def roofline(prof, n, es, steps):
-    """dominant kernel by total time; achieved = algorithmic bytes per launch
-    / average launch duration."""
-    best = None
-    for name, (ms, cnt) in prof.items():
-        if best is None or ms > best[1]:
-            best = (name, ms, cnt)
-    name, ms, cnt = best
-    vol = KERNEL_VOLUMES.get(name)
+def roofline(prof, n, es, np_, nb, default_case, steps):
+    """Roofline of the kernel with the largest share of the step: achieved =
+    algorithmic bytes per launch / average launch duration (HIP events on
+    the context stream)."""
+    prof = merge_profile(prof)
+    name, (ms, cnt) = max(prof.items(), key=lambda kv: kv[1][0])
     avg_s = ms / cnt / 1e3
-    if vol is None:
+    if name in KERNEL_VOLUMES:
+        bytes_per_launch = KERNEL_VOLUMES[name] * n ** 3 * es
+        note = "%g field volumes of %d^3 x %d B" % (KERNEL_VOLUMES[name], n, es)
+    elif name == "interp_ab":
+        # tracer state (144 B per tracer) + one read of the 3 real velocity volumes
+        bytes_per_launch = tracer_bytes(np_) + 3 * n ** 3 * es
+        note = ("tracer state 144 B x %d + 3 real velocity volumes; the kernel reads %d stencil "
+                "values per tracer from LDS" % (np_, 3 * (2 * nb + 2) ** 3))
+    else:
         return {"kernel": name, "avg_ms": ms / cnt}
-    bytes_per_launch = vol * n ** 3 * es
     ach = bytes_per_launch / avg_s
+    traffic = measured_traffic(name) if default_case else None
     return {"bound": "hbm", "kernel": name, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": ach / HBM_PEAK, "traffic": None, "avg_ms": ms / cnt, "bytes_per_launch": bytes_per_launch}
+            "frac": ach / HBM_PEAK, "traffic": traffic, "avg_ms": ms / cnt, "bytes_per_launch": bytes_per_launch,
+            "bytes_note": note, "launches_per_step": cnt / steps}
 
 
-def cpu_baseline(ic, n, threads):
-    """The oracle (CPU restatement, C++/OpenMP) on one full step of the same
-    512^3 FP32 field workload (no tracers), on `threads` host threads."""
+def cpu_baseline(ic, args, threads):
+    """The oracle (CPU restatement, C++/OpenMP) on `threads` host threads, on a
+    bounded sample of the same workload: one FP32 vorticity step and one
+    velocity transform at full size, and the tracer loop on 2^18 of the
+    tracers (scaled to the full count)."""
     from oracle import oracle as O
+    n = args.n
     O.build()
     O.set_threads(threads)
     s = O.Solver(n, precision="single")
     s.set_physics(0.0012, "linear", 0.5, 2.0, 4.0)
     s.set_cfield("vorticity", ic.solenoidal_vorticity(n, seed=1, k0=4.0))
     s.io_projection()
+    dt = 0.5 / n
     t0 = time.perf_counter()
-    s.step(0.5 / n)
-    dt = time.perf_counter() - t0
+    s.velocity_rspace()
+    t_vel = time.perf_counter() - t0
+    nsamp = min(args.np, 2 ** 18)
+    t_tr = 0.0
+    if nsamp:
+        p = O.Particles(s, nsamp, args.nb, args.m, 4, step_idx=1)
+        p.set_state(ic.uniform_positions(nsamp, seed=2))
+        t0 = time.perf_counter()
+        p.complete_loop(dt)
+        t_tr = (time.perf_counter() - t0) * args.np / nsamp
+        del p
+    t0 = time.perf_counter()
+    s.step(dt)
+    t_step = time.perf_counter() - t0
     del s
-    return {"value": n ** 3 / dt, "unit": "grid-pt*steps/s", "cores": threads, "kind": "port",
-            "sample": "1 step of the %d^3 FP32 vorticity step (no tracers) by the CPU oracle, %.1f s" % (n, dt)}
+    total = t_vel + t_tr + t_step
+    return {"value": n ** 3 / total, "unit": "grid-pt*steps/s", "cores": threads, "kind": "port",
+            "sample": ("one NSVEparticles step of the %d^3 FP32 workload by the CPU oracle: vorticity step %.1f s, "
+                       "velocity transform %.1f s, tracer loop on %d of %d tracers %.1f s scaled to %.1f s"
+                       % (n, t_step, t_vel, nsamp, args.np, t_tr * nsamp / max(args.np, 1), t_tr)),
+            "particle_interps_per_s": (args.np / t_tr) if t_tr else None}
+
+
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of known size: rocprofv3's FETCH_SIZE /
+    WRITE_SIZE of this copy calibrate the counters of the other kernels."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def main():
@@ -152,6 +217,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None)
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="finish with a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
@@ -162,6 +229,7 @@ def main():
 
     r32 = run_precision(B, ic, args, "single", args.steps, args.warmup)
     n = args.n
+    default_case = (n, args.np, args.nb, args.m) == (512, 2 ** 22, 6, 2)
     ms = r32["ms_per_step"]
     value = n ** 3 / (ms / 1e3)
     tb = tracer_bytes(args.np)
@@ -177,7 +245,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (fp64 factors, reductions, tracer positions)",
+        "dtype": "f32",
+        "precision_note": "f32 field storage, FFTs and cross product; f64 factors, reductions, weights, "
+                          "tracer positions and z-sums",
         "data": "synthetic: seeded solenoidal field E(k)~k^4 exp(-2(k/4)^2), seeded uniform tracers",
         "config": {"workload": "512^3 precision form: FP32 vorticity step + 2^22 tracers n6 m2 AB4",
                    "n": n, "nu": 0.0012, "dt": 0.5 / n, "tracers": args.np, "neighbours": args.nb,
@@ -186,8 +256,8 @@ def main():
         "gradient_sample_interps_per_s": args.np / r32["grad_sample_s"] if args.np else None,
         "hbm_frac_step": alg / (ms / 1e3) / HBM_PEAK,
         "algorithmic_bytes_per_step": alg,
-        "roofline": roofline(r32["profile"], n, 4, args.steps),
-        "kernel_ms": {k: v[0] / v[1] for k, v in r32["profile"].items()},
+        "roofline": roofline(r32["profile"], n, 4, args.np, args.nb, default_case, args.steps),
+        "kernel_ms": {k: v[0] / v[1] for k, v in merge_profile(r32["profile"]).items()},
         "wall_ms_per_step": r32["wall_ms_per_step"],
         "device_gb": r32["device_bytes"] / 1e9,
     }
@@ -197,13 +267,15 @@ def main():
         out["fp64"] = {"value": n ** 3 / (ms64 / 1e3), "ms_per_step": ms64,
                        "particle_interps_per_s": args.np / (ms64 / 1e3),
                        "hbm_frac_step": (step_bytes(n, 8, args.np > 0) + tb) / (ms64 / 1e3) / HBM_PEAK,
-                       "roofline": roofline(r64["profile"], n, 8, max(2, args.steps // 2)),
-                       "kernel_ms": {k: v[0] / v[1] for k, v in r64["profile"].items()}}
+                       "roofline": roofline(r64["profile"], n, 8, args.np, args.nb, False, max(2, args.steps // 2)),
+                       "kernel_ms": {k: v[0] / v[1] for k, v in merge_profile(r64["profile"]).items()}}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(ic, n, threads)
+        out["cpu_baseline"] = cpu_baseline(ic, args, threads)
     else:
         out["cpu_baseline"] = None
+    if args.copy_gib > 0:
+        copy_calibration(int(args.copy_gib * 2 ** 30))
     print(json.dumps(out), flush=True)
 
 

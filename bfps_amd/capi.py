@@ -12,7 +12,7 @@ LIB = os.environ.get("BFPSH_LIB") or os.path.join(HERE, "libbfps_hip.so")
 
 FP32, FP64 = 0, 1
 TWO_THIRDS, SMOOTH = 0, 1
-FORCING = {"none": 0, "linear": 1, "Kolmogorov": 2}
+FORCING = {"none": 0, "linear": 1, "Kolmogorov": 2, "fixed_energy_injection_rate": 3}
 CVORTICITY, CVELOCITY, RVELOCITY, RGRADIENT = 0, 1, 5, 6
 
 # every exported entry point: name -> (restype, argtypes)

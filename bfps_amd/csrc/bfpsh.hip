@@ -174,7 +174,7 @@ int bfpsh_destroy(bfpsh_ctx* ctx) {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         for (void* p : {ctx->v0, ctx->v1, ctx->v2, ctx->work, ctx->ru, ctx->rgrad, ctx->tw,
-                        (void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab})
+                        (void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab, (void*)ctx->d_famp})
             if (p) (void)hipFree(p);
         for (void* p : ctx->fields)
             if (p) (void)hipFree(p);
@@ -191,7 +191,11 @@ const char* bfpsh_last_error(const bfpsh_ctx* ctx) { return ctx ? ctx->err.c_str
 
 int bfpsh_set_physics(bfpsh_ctx* ctx, double nu, int forcing, double famplitude, double fk0, double fk1, int fmode) {
     if (!ctx) return BFPSH_EINVAL;
-    if (forcing < 0 || forcing > 2) return fail(ctx, BFPSH_EINVAL, "bad forcing type");
+    if (forcing < 0 || forcing > 3) return fail(ctx, BFPSH_EINVAL, "bad forcing type");
+    if (forcing == BFPSH_FORCING_FIXED_ENERGY && !ctx->d_famp) {
+        HIPCHK(hipMalloc((void**)&ctx->d_famp, 3 * sizeof(double)));
+        ctx->bytes += 3 * sizeof(double);
+    }
     if (forcing == BFPSH_FORCING_KOLMOGOROV && (fmode < 0 || fmode >= ctx->n))
         return fail(ctx, BFPSH_EINVAL, "fmode out of range");
     ctx->sp.nu = nu;

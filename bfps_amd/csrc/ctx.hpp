@@ -39,6 +39,7 @@ struct bfpsh_ctx {
     void* ru = nullptr;            // real velocity, 3 planes [y][z][x]
     void* rgrad = nullptr;         // real gradient, 9 planes
     void* tw = nullptr;            // twiddles exp(-2 pi i t / N), N complex
+    double* d_famp = nullptr;      // per-stage band amplitude (fixed energy-injection forcing)
     bool ru_valid = false;
     StepParams sp{0.1, 0.0, 1.0, 2.0, 4.0, 1, 1};
     int iteration = 0;

@@ -32,6 +32,11 @@ struct Impl {
         const CT* tw = (const CT*)ctx->tw;
         const CT* src = s == 1 ? v0 : (s == 2 ? v1 : v2);
         const dim3 tiles(NXH / B, N);
+        if (sp.forcing == BFPSH_FORCING_FIXED_ENERGY) {
+            sp.famp_dev = ctx->d_famp + (s - 1);
+            LAUNCH("band_amplitude", (k_band_amplitude<T, N><<<1, 256, 0, ctx->stream>>>(
+                                         src, ctx->d_famp + (s - 1), g, sp.famp, sp.fk0, sp.fk1)));
+        }
         {
             int r = bs_pass<0>(ctx, src, w, tw, g, "inv_z_bs");
             if (r) return r;

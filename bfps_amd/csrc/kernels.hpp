@@ -71,8 +71,9 @@ struct Geom {
 };
 struct StepParams {
     double nu, dt, famp, fk0, fk1;
-    int forcing;  // 0 none, 1 linear, 2 Kolmogorov
+    int forcing;  // 0 none, 1 linear, 2 Kolmogorov, 3 fixed energy-injection rate
     int fmode;
+    const double* famp_dev;  // forcing 3: the stage's band amplitude (device)
 };
 
 BFPSH_DEV double kwave(int i, int n, double dk) { return (i <= n / 2 ? i : i - n) * dk; }
@@ -429,13 +430,14 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
         cu[2] = mk<T>(T(-(kx * a[1].y - kyv * a[0].y)), T(kx * a[1].x - kyv * a[0].x));
         // forcing with the stage's source vorticity
         const C2<T>* srcv = STAGE == 1 ? v0 : vp;
-        if (sp.forcing == 1) {
+        if (sp.forcing == 1 || sp.forcing == 3) {
             const double kn = sqrt(k2);
             if (sp.fk0 <= kn && sp.fk1 >= kn) {
+                const double amp = sp.forcing == 1 ? sp.famp : *sp.famp_dev;
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     const C2<T> s = srcv[c * PS + off];
-                    cu[c] = mk<T>(T(cu[c].x + sp.famp * double(s.x) * 1.0), T(cu[c].y + sp.famp * double(s.y) * 1.0));
+                    cu[c] = mk<T>(T(cu[c].x + amp * double(s.x) * 1.0), T(cu[c].y + amp * double(s.y) * 1.0));
                 }
             }
         } else if (sp.forcing == 2) {
@@ -506,6 +508,49 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
 #pragma unroll
             for (int c = 0; c < 3; c++) dst[c * PS + off] = nv[c];
         }
+    }
+}
+
+// Fixed energy-injection-rate forcing (not a reference forcing type): the
+// linear band forcing with amplitude eps / (2 E_band), E_band = 1/2 sum over
+// fk0 <= |k| <= fk1 of nxmodes |omega_hat|^2 / k^2 (= 1/2 sum nxmodes |u_hat|^2
+// for solenoidal omega), so that the band gains energy at the rate eps.  One
+// block scans the box |k_i| <= fk1 and writes the amplitude to *amp.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_band_amplitude(const C2<T>* __restrict__ w, double* amp, Geom g, double eps,
+                                                        double fk0, double fk1) {
+    constexpr int NXH = N / 2;
+    constexpr size_t PS = size_t(N) * N * NXH;
+    __shared__ double part[256];
+    const int KX = min(NXH - 1, int(fk1 / g.dkx));
+    const int KY = min(N / 2 - 1, int(fk1 / g.dky)), KZ = min(N / 2 - 1, int(fk1 / g.dkz));
+    const int nyv = 2 * KY + 1, nzv = 2 * KZ + 1;
+    const int total = (KX + 1) * nyv * nzv;
+    double e = 0.0;
+    for (int t = threadIdx.x; t < total; t += 256) {
+        const int x = t % (KX + 1), zz = (t / (KX + 1)) % nzv - KZ, yy = t / ((KX + 1) * nzv) - KY;
+        const double kx = x * g.dkx, ky = yy * g.dky, kz = zz * g.dkz;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double kn = sqrt(k2);
+        if (!(fk0 <= kn && fk1 >= kn && k2 > 0)) continue;
+        const size_t off = (size_t(yy < 0 ? yy + N : yy) * N + (zz < 0 ? zz + N : zz)) * NXH + x;
+        double m = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const C2<T> v = w[c * PS + off];
+            m += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+        }
+        e += (x == 0 ? 1.0 : 2.0) * m / k2;
+    }
+    part[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double eb = 0.5 * part[0];
+        *amp = eb > 0 ? eps / (2 * eb) : 0.0;
     }
 }
 

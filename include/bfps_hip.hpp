@@ -445,6 +445,7 @@ class vorticity_equation {
         int f;
         if (!std::strcmp(forcing_type, "linear")) f = BFPSH_FORCING_LINEAR;
         else if (!std::strcmp(forcing_type, "Kolmogorov")) f = BFPSH_FORCING_KOLMOGOROV;
+        else if (!std::strcmp(forcing_type, "fixed_energy_injection_rate")) f = BFPSH_FORCING_FIXED_ENERGY;
         else if (!std::strcmp(forcing_type, "none") || !forcing_type[0]) f = BFPSH_FORCING_NONE;
         else throw error(std::string("unknown forcing_type ") + forcing_type);
         detail::check(box_.ctx, bfpsh_set_physics(box_.ctx, nu, f, famplitude, fk0, fk1, fmode), "set_physics");

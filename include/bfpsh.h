@@ -32,8 +32,13 @@ enum { BFPSH_FP32 = 0, BFPSH_FP64 = 1 };
 /* dealiasing: kspace_dealias_type (bfps/cpp/kspace.hpp:40) */
 enum { BFPSH_TWO_THIRDS = 0, BFPSH_SMOOTH = 1 };
 /* forcing_type strings of vorticity_equation::add_forcing
- * (bfps/cpp/vorticity_equation.cpp:241-288) */
-enum { BFPSH_FORCING_NONE = 0, BFPSH_FORCING_LINEAR = 1, BFPSH_FORCING_KOLMOGOROV = 2 };
+ * (bfps/cpp/vorticity_equation.cpp:241-288), plus a fixed energy-injection
+ * rate (no reference counterpart): the linear band forcing with amplitude
+ * famplitude / (2 E_band), E_band the kinetic energy of the band
+ * fk0 <= |k| <= fk1 of the stage's source field, so that the band gains
+ * energy at the rate famplitude. */
+enum { BFPSH_FORCING_NONE = 0, BFPSH_FORCING_LINEAR = 1, BFPSH_FORCING_KOLMOGOROV = 2,
+       BFPSH_FORCING_FIXED_ENERGY = 3 };
 /* field selectors */
 enum {
     BFPSH_CVORTICITY = 0, /* vorticity_equation::cvorticity (spectral) */

@@ -413,7 +413,11 @@ void symmetrize(Field<R>& f) {  // field.cpp:900-978, one rank
 // ------------------------------------------------------------------------ //
 // vorticity_equation (vorticity_equation.cpp:110-439)
 // ------------------------------------------------------------------------ //
-enum Forcing { F_NONE = 0, F_LINEAR = 1, F_KOLMOGOROV = 2 };
+// F_FIXED_ENERGY is not a reference forcing: the band 1.5 <= |k| <= 3.5
+// forcing of the 1024^3 workload, f = eps u_band / (2 E_band), i.e. the linear
+// forcing of the band with the amplitude that makes the injection rate
+// sum_band nxmodes Re(u* . f) equal eps (famplitude holds eps).
+enum Forcing { F_NONE = 0, F_LINEAR = 1, F_KOLMOGOROV = 2, F_FIXED_ENERGY = 3 };
 
 struct SolverBase {
     virtual ~SolverBase() {}
@@ -562,12 +566,32 @@ struct Solver : SolverBase {
             dst.cv(ci, 2, 0) = R(dst.cv(ci, 2, 0) - famplitude * factor / 2);
             return;
         }
+        double amp = famplitude;
+        if (forcing == F_FIXED_ENERGY) {
+            // E_band = 1/2 sum nxmodes |u|^2 with |u|^2 = |omega|^2 / k^2
+            // (serial: a reduction)
+            double e = 0.0;
+            for (int y = 0; y < kk.ny; y++)
+                for (int z = 0; z < int(nz); z++)
+                    for (int x = 0; x < int(nxh); x++) {
+                        const double k2 = kk.kx[x] * kk.kx[x] + kk.ky[y] * kk.ky[y] + kk.kz[z] * kk.kz[z];
+                        const double kn = std::sqrt(k2);
+                        if (!(fk0 <= kn && fk1 >= kn && k2 > 0)) continue;
+                        const size_t ci = (size_t(y) * nz + z) * nxh + x;
+                        double m = 0.0;
+                        for (int c = 0; c < 3; c++)
+                            for (int i = 0; i < 2; i++) m += double(vort.cv(ci, c, i)) * double(vort.cv(ci, c, i));
+                        e += (x == 0 ? 1.0 : 2.0) * m / k2;
+                    }
+            e *= 0.5;
+            amp = e > 0 ? famplitude / (2 * e) : 0.0;
+        }
         kk.cloop_k2([&](size_t ci, int x, int y, int z, double) {
             double kn = std::sqrt(kk.kx[x] * kk.kx[x] + kk.ky[y] * kk.ky[y] + kk.kz[z] * kk.kz[z]);
             if (fk0 <= kn && fk1 >= kn)
                 for (int c = 0; c < 3; c++)
                     for (int i = 0; i < 2; i++)
-                        dst.cv(ci, c, i) = R(dst.cv(ci, c, i) + famplitude * vort.cv(ci, c, i) * factor);
+                        dst.cv(ci, c, i) = R(dst.cv(ci, c, i) + amp * vort.cv(ci, c, i) * factor);
         });
     }
     // omega_nonlin (vorticity_equation.cpp:290-356)

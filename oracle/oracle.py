@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 FIELD = {"vorticity": 0, "velocity": 1, "v1": 2, "v2": 3, "rvorticity": 4,
          "rvelocity": 5, "gradient": 6}
-FORCING = {"none": 0, "linear": 1, "Kolmogorov": 2}
+FORCING = {"none": 0, "linear": 1, "Kolmogorov": 2, "fixed_energy_injection_rate": 3}
 
 _lib = None
 

@@ -137,7 +137,7 @@ def test_step_fp32_vs_oracle(B, oracle, n):
     assert e_gpu < 3 * e_orc + 1e-7
 
 
-@pytest.mark.parametrize("forcing", ["none", "Kolmogorov"])
+@pytest.mark.parametrize("forcing", ["none", "Kolmogorov", "fixed_energy_injection_rate"])
 def test_forcing_variants_fp64(B, oracle, forcing):
     n = 32
     say("case: forcing %s" % forcing)
@@ -310,3 +310,68 @@ def test_tracers_fp32_bricks(B, oracle, n, nb, m):
     sg, og = ps.sample("gradient"), op.sample("gradient")
     say("  gradient sample rel err %.3e" % (np.abs(sg - og).max() / np.abs(og).max()))
     assert np.abs(sg - og).max() < 1e-5 * np.abs(og).max()
+
+
+@pytest.mark.parametrize("precision,k,inband", [("double", (1, 2, 0), True), ("double", (2, -1, 3), True),
+                                                ("double", (3, 3, 3), False), ("single", (1, 2, 0), True)])
+def test_single_wave_rk3_polynomial(B, precision, k, inband):
+    """A single Fourier wave has a curl-free u x omega; with linear forcing on
+    its shell one IF-SSP-RK3 step multiplies it by
+    exp(-nu k^2 dt) (1 + a + a^2/2 + a^3/6), a = A dt."""
+    n, nu, dt, A = 32, 0.02, 0.05, 0.5
+    say("case: single wave %s %s" % (precision, k))
+    kv = np.array(k, float)
+    amp = np.array([0.3 + 0.1j, -0.2 + 0.4j, 0.25 - 0.3j])
+    amp = amp - kv * (kv @ amp) / (kv @ kv)
+    w0 = np.zeros((n, n, n // 2 + 1, 3), dtype=complex)
+    w0[k[1] % n, k[2] % n, k[0]] = amp
+    fs = B.vorticity_equation("t", n, precision=precision)
+    fs.nu, fs.famplitude, fs.forcing_type = nu, A, "linear"
+    fs.io_checkpoint(read=True, data=w0)
+    fs.step(dt)
+    a = A * dt if inband else 0.0
+    expect = w0 * np.exp(-nu * (kv @ kv) * dt) * (1 + a + a * a / 2 + a ** 3 / 6)
+    e = rel(fs.cvorticity, expect)
+    say("  rel", e)
+    assert e < (1e-13 if precision == "double" else 1e-6)
+
+
+def test_kolmogorov_from_rest_simpson(B):
+    n, nu, dt, A, fm = 32, 0.1, 0.05, 0.7, 2
+    say("case: Kolmogorov from rest")
+    fs = B.vorticity_equation("t", n, precision="double")
+    fs.nu, fs.famplitude, fs.forcing_type, fs.fmode = nu, A, "Kolmogorov", fm
+    fs.io_checkpoint(read=True, data=np.zeros((n, n, n // 2 + 1, 3), dtype=complex))
+    lam = nu * fm * fm * dt
+    S = (1 + np.exp(-lam)) / 6 + 2 * np.exp(-lam / 2) / 3
+    v = 0.0
+    expect = np.zeros((n, n, n // 2 + 1, 3), dtype=complex)
+    for _ in range(3):
+        fs.step(dt)
+        v = v * np.exp(-lam) - A / 2 * dt * S
+        expect[fm, 0, 0, 2] = v
+        expect[n - fm, 0, 0, 2] = v
+        assert np.abs(fs.cvorticity - expect).max() < 1e-15
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_fixed_energy_injection_vs_oracle(B, oracle, precision):
+    """Fixed energy-injection forcing on 1.5 <= |k| <= 3.5 of a random field:
+    GPU against the FP64 oracle after 4 steps, and the energy budget of a
+    single forced wave (dE/dt = eps - 2 nu k^2 E)."""
+    n = 48
+    say("case: fixed energy injection %s" % precision)
+    w0 = random_vorticity(n, seed=8, dtype=np.complex128)
+    fs = B.vorticity_equation("t", n, precision=precision)
+    fs.nu, fs.famplitude, fs.fk0, fs.fk1, fs.forcing_type = 0.02, 0.3, 1.5, 3.5, "fixed_energy_injection_rate"
+    fs.io_checkpoint(read=True, data=w0)
+    o = oracle.Solver(n, precision="double")
+    o.set_physics(0.02, "fixed_energy_injection_rate", 0.3, 1.5, 3.5)
+    o.set_cfield("vorticity", w0)
+    o.io_projection()
+    for _ in range(4):
+        fs.step(0.01)
+        o.step(0.01)
+    e = rel(fs.cvorticity, o.get_cfield("vorticity"))
+    say("  rel", e)
+    assert e < (1e-11 if precision == "double" else 2e-6)

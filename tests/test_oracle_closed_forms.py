@@ -320,3 +320,46 @@ def test_triad_nonlinear_term_and_filter(oracle, dealias, k1, k2):
     assert np.abs(got - rate).max() < 1e-6 * max(np.abs(rate).max(), 1e-3)
     if dealias == "SMOOTH":
         assert 0.05 < filt < 0.95   # the filter value matters for these triads
+
+
+def band_energy(w, k):
+    """1/2 sum nxmodes |omega|^2 / k^2 of a single stored wave."""
+    kv = np.array(k, float)
+    a = w[k[1] % w.shape[0], k[2] % w.shape[1], k[0]]
+    return 0.5 * (2 if k[0] > 0 else 1) * np.sum(np.abs(a) ** 2) / (kv @ kv)
+
+
+@pytest.mark.parametrize("nu", [0.0, 0.05])
+def test_fixed_energy_injection_single_wave(oracle, nu):
+    """Forcing with a fixed injection rate eps on the band: for a single wave
+    (no nonlinear transfer) dE/dt = eps - 2 nu k^2 E exactly, so
+    E(t) = eps/(2 nu k^2) + (E0 - eps/(2 nu k^2)) exp(-2 nu k^2 t)
+    (E0 + eps t when nu = 0); IF-RK3 follows it to O(dt^3)."""
+    n, eps, T = 16, 0.3, 0.2
+    k = (1, 1, 2)   # |k| = 2.449 inside 1.5 <= |k| <= 3.5
+    w0 = single_wave(n, k, [0.2 + 0.1j, -0.3j, 0.1])
+    k2 = float(np.dot(k, k))
+    errs = []
+    for nsteps in (10, 20):
+        dt = T / nsteps
+        s = oracle.Solver(n, precision="double")
+        s.set_physics(nu, "fixed_energy_injection_rate", eps, 1.5, 3.5)
+        s.set_cfield("vorticity", w0)
+        s.io_projection()
+        e0 = band_energy(s.get_cfield("vorticity"), k)
+        for _ in range(nsteps):
+            s.step(dt)
+        if nu == 0:
+            expect = e0 + eps * T
+        else:
+            einf = eps / (2 * nu * k2)
+            expect = einf + (e0 - einf) * np.exp(-2 * nu * k2 * T)
+        w = s.get_cfield("vorticity")
+        errs.append(abs(band_energy(w, k) - expect) / expect)
+        # the wave keeps its shape: only its amplitude changes
+        a0 = w0[k[1], k[2], k[0]]
+        a1 = w[k[1], k[2], k[0]]
+        assert np.abs(a1 / np.linalg.norm(a1) - a0 / np.linalg.norm(a0)).max() < 1e-12
+    # close to the exact energy, and converging at third order in dt
+    assert errs[0] < 1e-4, errs
+    assert 8 * 0.7 < errs[0] / errs[1] < 8 * 1.4, errs
