@@ -1,15 +1,24 @@
 """Benchmark: bfps vorticity-equation step + tracer loop on one MI355X.
 
-Workload (precision form): 512^3 periodic box, FP32 FFT / nonlinear
-arithmetic with FP64 integrating factors, reductions and tracer positions,
-nu = 0.0012, E(k) ~ k^4 exp(-2 (k/4)^2) initial field (total energy 1/2),
-linear band forcing 0.5 on 2 <= |k| <= 4, dt = 0.5/N, 2^22 tracers with
-6-neighbour smoothness-2 spline interpolation and AB4.  One step =
-NSVEparticles::step (tracer completeLoop with u(t), then the IF-SSP-RK3
-vorticity step).  Velocity-gradient sampling of the tracers (9 components)
-is timed separately.  The same workload in FP64 is reported beside it.
+Workloads (synthetic, seeded; E(k) ~ k^4 exp(-2 (k/k0)^2) initial field of
+total energy 1/2, uniform random tracers, AB4, dt = 0.5/N):
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fp64]
+  512   (default) precision form: 512^3, FP32 FFTs / nonlinear arithmetic with
+        FP64 integrating factors, reductions and tracer positions, nu = 0.0012,
+        linear band forcing 0.5 on 2 <= |k| <= 4, 2^22 tracers, 6-neighbour
+        smoothness-2 interpolation of the velocity; velocity-gradient
+        sampling timed separately.
+  1024  large single box: 1024^3 FP32, nu = 4e-4, fixed energy-injection rate
+        0.1 on 1.5 <= |k| <= 3.5, 2^24 tracers, 4-neighbour smoothness-1.
+
+One step = NSVEparticles::step (tracer completeLoop with u(t), then the
+IF-SSP-RK3 vorticity step).  The same workload in FP64 is timed beside it,
+and the accuracy leg runs FP32 and FP64 from the same seed for --accuracy
+steps and reports the FP32 error (vorticity L2, per-shell energy spectrum,
+tracer positions) against FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 512|1024]
+                    [--accuracy STEPS] [--no-cpu] [--no-fp64]
 
 Prints one JSON line.
 """
@@ -24,6 +33,14 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+
+WORKLOADS = {
+    "512": dict(n=512, nu=0.0012, forcing="linear", famp=0.5, fk0=2.0, fk1=4.0, np=2 ** 22, nb=6, m=2, k0=4.0,
+                name="512^3 precision form: FP32 vorticity step + 2^22 tracers n6 m2 AB4"),
+    "1024": dict(n=1024, nu=4e-4, forcing="fixed_energy_injection_rate", famp=0.1, fk0=1.5, fk1=3.5, np=2 ** 24,
+                 nb=4, m=1, k0=4.0,
+                 name="1024^3 FP32 vorticity step, fixed energy injection, + 2^24 tracers n4 m1 AB4"),
+}
 
 
 def step_bytes(n, es, with_tracer_u):
@@ -76,25 +93,25 @@ def tracer_bytes(np_, nsteps=4):
     return np_ * (48 + 24 * (nsteps - 1) + 24)
 
 
-def make_case(B, ic, n, precision, npart, nb, m, seed=1):
+def make_case(B, ic, wl, precision, seed=1):
+    import numpy as np
+    n = wl["n"]
     fs = B.vorticity_equation("bench", n, precision=precision)
-    fs.nu, fs.famplitude, fs.fk0, fs.fk1, fs.forcing_type = 0.0012, 0.5, 2.0, 4.0, "linear"
-    w = ic.solenoidal_vorticity(n, seed=seed, k0=4.0,
-                                dtype=__import__("numpy").complex64 if precision == "single" else complex)
+    fs.nu, fs.famplitude, fs.fk0, fs.fk1, fs.forcing_type = wl["nu"], wl["famp"], wl["fk0"], wl["fk1"], wl["forcing"]
+    w = ic.solenoidal_vorticity(n, seed=seed, k0=wl["k0"], dtype=np.complex64 if precision == "single" else complex)
     fs.io_checkpoint(read=True, data=w)
     del w
     ps = None
-    if npart:
-        ps = B.particles_system(fs, npart, nb, m, 4, step_idx=1)
-        ps.put(ic.uniform_positions(npart, seed=seed + 1))
+    if wl["np"]:
+        ps = B.particles_system(fs, wl["np"], wl["nb"], wl["m"], 4, step_idx=1)
+        ps.put(ic.uniform_positions(wl["np"], seed=seed + 1))
     return fs, ps
 
 
-def run_precision(B, ic, args, precision, steps, warmup):
-    n = args.n
-    fs, ps = make_case(B, ic, n, precision, args.np, args.nb, args.m)
+def run_precision(B, ic, wl, precision, steps, warmup, gradient=True):
+    fs, ps = make_case(B, ic, wl, precision)
+    n = wl["n"]
     dt = 0.5 / n
-    es = 4 if precision == "single" else 8
     fs._push_physics()
     for _ in range(warmup):
         B.NSVEparticles_step(fs, dt)
@@ -105,30 +122,24 @@ def run_precision(B, ic, args, precision, steps, warmup):
     wall = time.perf_counter() - t0
     prof = fs.profile_report()
     fs.profile(False)
-    # gradient sampling (velocity gradient, 9 components, at all tracers)
-    fs.synchronize()
-    t1 = time.perf_counter()
-    reps = 2
-    for _ in range(reps):
-        fs.gradient_rspace()
-        g = ps.sample("gradient") if ps is not None else None
-    fs.synchronize()
-    grad_s = (time.perf_counter() - t1) / reps
-    del g
-    mem = fs.device_bytes()
-    res = {
-        "ms_per_step": ms / steps,
-        "wall_ms_per_step": 1e3 * wall / steps,
-        "profile": prof,
-        "grad_sample_s": grad_s,
-        "device_bytes": mem,
-        "es": es,
-    }
+    grad_s = None
+    if gradient and ps is not None:
+        # velocity gradient (9 components) sampled at all tracers
+        fs.synchronize()
+        t1 = time.perf_counter()
+        reps = 2
+        for _ in range(reps):
+            fs.gradient_rspace()
+            ps.sample("gradient")
+        fs.synchronize()
+        grad_s = (time.perf_counter() - t1) / reps
+    res = {"ms_per_step": ms / steps, "wall_ms_per_step": 1e3 * wall / steps, "profile": prof,
+           "grad_sample_s": grad_s, "device_bytes": fs.device_bytes()}
     fs.close()
     return res
 
 
-def roofline(prof, n, es, np_, nb, default_case, steps):
+def roofline(prof, n, es, np_, nb, traffic_ok, steps):
     """Roofline of the kernel with the largest share of the step: achieved =
     algorithmic bytes per launch / average launch duration (HIP events on
     the context stream)."""
@@ -141,42 +152,82 @@ def roofline(prof, n, es, np_, nb, default_case, steps):
     elif name == "interp_ab":
         # tracer state (144 B per tracer) + one read of the 3 real velocity volumes
         bytes_per_launch = tracer_bytes(np_) + 3 * n ** 3 * es
-        note = ("tracer state 144 B x %d + 3 real velocity volumes; the kernel reads %d stencil "
+        note = ("tracer state 144 B x %d + 3 real velocity volumes; the kernel also reads %d stencil "
                 "values per tracer from LDS" % (np_, 3 * (2 * nb + 2) ** 3))
     else:
         return {"kernel": name, "avg_ms": ms / cnt}
     ach = bytes_per_launch / avg_s
-    traffic = measured_traffic(name) if default_case else None
+    traffic = measured_traffic(name) if traffic_ok else None
     return {"bound": "hbm", "kernel": name, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": ach / HBM_PEAK, "traffic": traffic, "avg_ms": ms / cnt, "bytes_per_launch": bytes_per_launch,
             "bytes_note": note, "launches_per_step": cnt / steps}
 
 
-def cpu_baseline(ic, args, threads):
+def accuracy(B, ic, wl, steps):
+    """FP32 against FP64 from the same seed after `steps` NSVEparticles steps:
+    relative L2 of the vorticity (half spectrum, nxmodes weights = real-space
+    L2), per-shell energy spectrum error, tracer position drift."""
+    import numpy as np
+    n = wl["n"]
+    dt = 0.5 / n
+    out = {}
+    for prec in ("double", "single"):
+        fs, ps = make_case(B, ic, wl, prec)
+        for _ in range(steps):
+            B.NSVEparticles_step(fs, dt)
+        w = fs.cvorticity.astype(np.complex128)
+        spec = fs.cospectrum("velocity", "velocity")
+        pos = ps.get()[0] if ps is not None else None
+        out[prec] = (w, np.trace(spec, axis1=1, axis2=2) / 2, pos)
+        fs.close()
+        del fs, ps
+    (w64, e64, p64), (w32, e32, p32) = out["double"], out["single"]
+    wt = np.ones(w64.shape[2])
+    wt[1:] = 2.0
+    d = np.abs(w32 - w64) ** 2
+    num = float(np.sqrt(np.einsum("abkc,k->", d, wt)))
+    den = float(np.sqrt(np.einsum("abkc,k->", np.abs(w64) ** 2, wt)))
+    del d
+    ok = e64 > 1e-12 * e64.max()
+    shell_err = np.abs(e32[ok] - e64[ok]) / e64[ok]
+    res = {"steps": steps, "reference": "the same workload in FP64 on the GPU (validated against the CPU oracle "
+                                        "up to 256^3 by the parity tests)",
+           "vorticity_rel_l2": num / den,
+           "energy_spectrum_rel_err_max": float(shell_err.max()),
+           "energy_spectrum_rel_err_median": float(np.median(shell_err))}
+    if p64 is not None:
+        dp = np.linalg.norm(p32 - p64, axis=1)
+        res["tracer_drift_max"] = float(dp.max())
+        res["tracer_drift_rms"] = float(np.sqrt(np.mean(dp ** 2)))
+        res["grid_spacing"] = 2 * np.pi / n
+    return res
+
+
+def cpu_baseline(ic, wl, threads):
     """The oracle (CPU restatement, C++/OpenMP) on `threads` host threads, on a
     bounded sample of the same workload: one FP32 vorticity step and one
     velocity transform at full size, and the tracer loop on 2^18 of the
     tracers (scaled to the full count)."""
     from oracle import oracle as O
-    n = args.n
+    n = wl["n"]
     O.build()
     O.set_threads(threads)
     s = O.Solver(n, precision="single")
-    s.set_physics(0.0012, "linear", 0.5, 2.0, 4.0)
-    s.set_cfield("vorticity", ic.solenoidal_vorticity(n, seed=1, k0=4.0))
+    s.set_physics(wl["nu"], wl["forcing"], wl["famp"], wl["fk0"], wl["fk1"])
+    s.set_cfield("vorticity", ic.solenoidal_vorticity(n, seed=1, k0=wl["k0"]))
     s.io_projection()
     dt = 0.5 / n
     t0 = time.perf_counter()
     s.velocity_rspace()
     t_vel = time.perf_counter() - t0
-    nsamp = min(args.np, 2 ** 18)
+    nsamp = min(wl["np"], 2 ** 18)
     t_tr = 0.0
     if nsamp:
-        p = O.Particles(s, nsamp, args.nb, args.m, 4, step_idx=1)
+        p = O.Particles(s, nsamp, wl["nb"], wl["m"], 4, step_idx=1)
         p.set_state(ic.uniform_positions(nsamp, seed=2))
         t0 = time.perf_counter()
         p.complete_loop(dt)
-        t_tr = (time.perf_counter() - t0) * args.np / nsamp
+        t_tr = (time.perf_counter() - t0) * wl["np"] / nsamp
         del p
     t0 = time.perf_counter()
     s.step(dt)
@@ -186,8 +237,8 @@ def cpu_baseline(ic, args, threads):
     return {"value": n ** 3 / total, "unit": "grid-pt*steps/s", "cores": threads, "kind": "port",
             "sample": ("one NSVEparticles step of the %d^3 FP32 workload by the CPU oracle: vorticity step %.1f s, "
                        "velocity transform %.1f s, tracer loop on %d of %d tracers %.1f s scaled to %.1f s"
-                       % (n, t_step, t_vel, nsamp, args.np, t_tr * nsamp / max(args.np, 1), t_tr)),
-            "particle_interps_per_s": (args.np / t_tr) if t_tr else None}
+                       % (n, t_step, t_vel, nsamp, wl["np"], t_tr * nsamp / max(wl["np"], 1), t_tr)),
+            "particle_interps_per_s": (wl["np"] / t_tr) if t_tr else None}
 
 
 def copy_calibration(nbytes):
@@ -210,10 +261,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--np", type=int, default=2 ** 22)
-    ap.add_argument("--nb", type=int, default=6)
-    ap.add_argument("--m", type=int, default=2)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="512")
+    ap.add_argument("--np", type=int, default=None, help="override the tracer count")
+    ap.add_argument("--accuracy", type=int, default=100, help="steps of the FP32-vs-FP64 leg (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None)
@@ -227,16 +277,18 @@ def main():
     import bfps_amd as B
     from bfps_amd import ic
 
-    r32 = run_precision(B, ic, args, "single", args.steps, args.warmup)
-    n = args.n
-    default_case = (n, args.np, args.nb, args.m) == (512, 2 ** 22, 6, 2)
+    wl = dict(WORKLOADS[args.workload])
+    default_case = args.workload == "512" and args.np is None
+    if args.np is not None:
+        wl["np"] = args.np
+    n, np_ = wl["n"], wl["np"]
+    r32 = run_precision(B, ic, wl, "single", args.steps, args.warmup)
     ms = r32["ms_per_step"]
-    value = n ** 3 / (ms / 1e3)
-    tb = tracer_bytes(args.np)
-    alg = step_bytes(n, 4, args.np > 0) + tb
+    tb = tracer_bytes(np_)
+    alg = step_bytes(n, 4, np_ > 0) + tb
     out = {
         "metric": "grid-pt*steps/s + particle interps/s, FP32 vs FP64 at 512^3, % HBM peak",
-        "value": value,
+        "value": n ** 3 / (ms / 1e3),
         "unit": "grid-pt*steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -248,30 +300,32 @@ def main():
         "dtype": "f32",
         "precision_note": "f32 field storage, FFTs and cross product; f64 factors, reductions, weights, "
                           "tracer positions and z-sums",
-        "data": "synthetic: seeded solenoidal field E(k)~k^4 exp(-2(k/4)^2), seeded uniform tracers",
-        "config": {"workload": "512^3 precision form: FP32 vorticity step + 2^22 tracers n6 m2 AB4",
-                   "n": n, "nu": 0.0012, "dt": 0.5 / n, "tracers": args.np, "neighbours": args.nb,
-                   "smoothness": args.m},
-        "particle_interps_per_s": args.np / (ms / 1e3),
-        "gradient_sample_interps_per_s": args.np / r32["grad_sample_s"] if args.np else None,
+        "data": "synthetic: seeded solenoidal field E(k)~k^4 exp(-2(k/%g)^2), seeded uniform tracers" % wl["k0"],
+        "config": {"workload": wl["name"], "n": n, "nu": wl["nu"], "dt": 0.5 / n, "forcing": wl["forcing"],
+                   "tracers": np_, "neighbours": wl["nb"], "smoothness": wl["m"]},
+        "particle_interps_per_s": np_ / (ms / 1e3),
+        "gradient_sample_interps_per_s": np_ / r32["grad_sample_s"] if r32["grad_sample_s"] else None,
         "hbm_frac_step": alg / (ms / 1e3) / HBM_PEAK,
         "algorithmic_bytes_per_step": alg,
-        "roofline": roofline(r32["profile"], n, 4, args.np, args.nb, default_case, args.steps),
+        "roofline": roofline(r32["profile"], n, 4, np_, wl["nb"], default_case, args.steps),
         "kernel_ms": {k: v[0] / v[1] for k, v in merge_profile(r32["profile"]).items()},
         "wall_ms_per_step": r32["wall_ms_per_step"],
         "device_gb": r32["device_bytes"] / 1e9,
     }
     if not args.no_fp64:
-        r64 = run_precision(B, ic, args, "double", max(2, args.steps // 2), 1)
+        s64 = max(2, args.steps // 2)
+        r64 = run_precision(B, ic, wl, "double", s64, 1, gradient=False)
         ms64 = r64["ms_per_step"]
         out["fp64"] = {"value": n ** 3 / (ms64 / 1e3), "ms_per_step": ms64,
-                       "particle_interps_per_s": args.np / (ms64 / 1e3),
-                       "hbm_frac_step": (step_bytes(n, 8, args.np > 0) + tb) / (ms64 / 1e3) / HBM_PEAK,
-                       "roofline": roofline(r64["profile"], n, 8, args.np, args.nb, False, max(2, args.steps // 2)),
+                       "particle_interps_per_s": np_ / (ms64 / 1e3),
+                       "hbm_frac_step": (step_bytes(n, 8, np_ > 0) + tb) / (ms64 / 1e3) / HBM_PEAK,
+                       "roofline": roofline(r64["profile"], n, 8, np_, wl["nb"], False, s64),
                        "kernel_ms": {k: v[0] / v[1] for k, v in merge_profile(r64["profile"]).items()}}
+    if args.accuracy > 0 and not args.no_fp64:
+        out["accuracy"] = accuracy(B, ic, wl, args.accuracy)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(ic, args, threads)
+        out["cpu_baseline"] = cpu_baseline(ic, wl, threads)
     else:
         out["cpu_baseline"] = None
     if args.copy_gib > 0:

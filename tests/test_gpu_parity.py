@@ -375,3 +375,61 @@ def test_fixed_energy_injection_vs_oracle(B, oracle, precision):
     e = rel(fs.cvorticity, o.get_cfield("vorticity"))
     say("  rel", e)
     assert e < (1e-11 if precision == "double" else 2e-6)
+
+
+def abc_modes(n, dtype):
+    """Spectral ABC vorticity (= velocity) u = (sin z + cos y, sin x + cos z,
+    sin y + cos x) in the reference layout, built mode by mode (6 waves)."""
+    w = np.zeros((n, n, n // 2 + 1, 3), dtype=dtype)
+    # cos(k.x) -> 1/2 at +-k, sin(k.x) -> -i/2 at +k (kx > 0 stored only)
+    w[1, 0, 0, 0] += 0.5          # cos y, +ky
+    w[n - 1, 0, 0, 0] += 0.5      # cos y, -ky
+    w[0, 1, 0, 0] += -0.5j        # sin z, +kz
+    w[0, n - 1, 0, 0] += 0.5j     # sin z, -kz
+    w[0, 0, 1, 1] += -0.5j        # sin x
+    w[0, 1, 0, 1] += 0.5          # cos z
+    w[0, n - 1, 0, 1] += 0.5
+    w[1, 0, 0, 2] += -0.5j        # sin y
+    w[n - 1, 0, 0, 2] += 0.5j
+    w[0, 0, 1, 2] += 0.5          # cos x
+    return w
+
+
+@pytest.mark.parametrize("n", [1024])
+def test_largest_grid_beltrami_and_tracers(B, n):
+    """The 1024^3 FP32 path: exact Beltrami decay of the six modes (and
+    nothing else), and tracer velocities against the analytic field."""
+    say("case: Beltrami + tracers n=%d single" % n)
+    w = abc_modes(n, np.complex64)
+    fs = B.vorticity_equation("big", n, precision="single")
+    fs.nu, fs.forcing_type = 0.1, "none"
+    fs.io_checkpoint(read=True, data=w)
+    np_ = 1 << 16
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0, 2 * np.pi, (np_, 3))
+    ps = B.particles_system(fs, np_, 4, 1, 4)
+    ps.put(pos)
+    fs.velocity_rspace()
+    u = ps.sample("velocity")
+    ue = np.stack([np.sin(pos[:, 2]) + np.cos(pos[:, 1]), np.sin(pos[:, 0]) + np.cos(pos[:, 2]),
+                   np.sin(pos[:, 1]) + np.cos(pos[:, 0])], -1)
+    eu = np.abs(u - ue).max()
+    say("  tracer velocity max err", eu)
+    assert eu < 1e-5
+    # dt = 0.5/N as in the workloads: nu k_max^2 dt / 2 stays ~6, so the
+    # stage-2 factor exp(+nu k^2 dt/2) of the IF-RK3 (stored in FP32, as in
+    # the reference's single mode) cannot overflow
+    dt, nsteps = 0.5 / n, 3
+    for _ in range(nsteps):
+        fs.step(dt)
+    got = fs.cvorticity
+    idx = np.nonzero(w)
+    exact = w[idx].astype(np.complex128) * np.exp(-0.1 * dt * nsteps)
+    e_modes = np.abs(got[idx] - exact).max()
+    got[idx] = 0
+    e_rest = np.abs(got).max()
+    say("  mode err %.3e, spurious max %.3e" % (e_modes, e_rest))
+    assert len(idx[0]) == 10
+    assert e_modes < 1e-6 and e_rest < 1e-6
+    del got
+    fs.close()
