@@ -174,7 +174,8 @@ int bfpsh_destroy(bfpsh_ctx* ctx) {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         for (void* p : {ctx->v0, ctx->v1, ctx->v2, ctx->work, ctx->ru, ctx->rgrad, ctx->tw,
-                        (void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab, (void*)ctx->d_famp})
+                        (void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab, (void*)ctx->d_famp,
+                        (void*)ctx->d_rank, (void*)ctx->d_samp})
             if (p) (void)hipFree(p);
         for (void* p : ctx->fields)
             if (p) (void)hipFree(p);
@@ -429,14 +430,36 @@ int bfpsh_field_cospectrum(bfpsh_ctx* ctx, int id_a, int id_b, double* out) {
     return PREC_CALL(cospectrum, ctx, ctx->fields[id_a], ctx->fields[id_b], out);
 }
 
+// Slots in global-index order, on the host (order) and as a rank per slot
+// on the device.  The storage order of tracers never changes (sorting for
+// locality only permutes an index array), so this is set once per put.
+static int set_order(bfpsh_ctx* ctx) {
+    ctx->order.resize(ctx->np);
+    std::iota(ctx->order.begin(), ctx->order.end(), int64_t(0));
+    std::stable_sort(ctx->order.begin(), ctx->order.end(),
+                     [&](int64_t a, int64_t b) { return ctx->idx[a] < ctx->idx[b]; });
+    if (ctx->np == 0) return BFPSH_OK;
+    std::vector<uint32_t> rank(ctx->np);
+    for (int64_t i = 0; i < ctx->np; i++) rank[ctx->order[i]] = uint32_t(i);
+    if (!ctx->d_rank) {
+        HIPCHK(hipMalloc((void**)&ctx->d_rank, ctx->np * sizeof(uint32_t)));
+        ctx->bytes += ctx->np * sizeof(uint32_t);
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d_rank, rank.data(), ctx->np * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BFPSH_OK;
+}
+
 int bfpsh_particles_create(bfpsh_ctx* ctx, int64_t np, int n, int m, int nsteps, int step_idx) {
     CTX_CHECK();
     if (np < 0 || n < 1 || n > BFPS_INTERP_NMAX || m < 0 || m > BFPS_INTERP_MMAX || nsteps < 1 || nsteps > 6)
         return fail(ctx, BFPSH_EINVAL, "bad tracer parameters (n in [1,10], m in [0,2], nsteps in [1,6])");
     if (2 * n + 2 > ctx->n) return fail(ctx, BFPSH_EINVAL, "interpolation stencil wider than the grid");
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (void* p : {(void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab})
+    for (void* p : {(void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab, (void*)ctx->d_rank, (void*)ctx->d_samp})
         if (p) HIPCHK(hipFree(p));
+    ctx->d_rank = nullptr;
+    ctx->d_samp = nullptr;
     {
         int r = bfpsh_part_release(ctx);
         if (r) return r;
@@ -465,7 +488,7 @@ int bfpsh_particles_create(bfpsh_ctx* ctx, int64_t np, int n, int m, int nsteps,
     ctx->idx.resize(np);
     std::iota(ctx->idx.begin(), ctx->idx.end(), int64_t(0));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    return BFPSH_OK;
+    return set_order(ctx);
 }
 
 int bfpsh_particles_put(bfpsh_ctx* ctx, const double* pos, const double* rhs, const int64_t* idx) {
@@ -487,8 +510,7 @@ int bfpsh_particles_put(bfpsh_ctx* ctx, const double* pos, const double* rhs, co
     ctx->sort_age = 1 << 30;  // positions replaced: re-sort before the next loop
     if (idx) std::copy(idx, idx + ctx->np, ctx->idx.begin());
     else std::iota(ctx->idx.begin(), ctx->idx.end(), int64_t(0));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    return BFPSH_OK;
+    return set_order(ctx);
 }
 
 int bfpsh_particles_get(bfpsh_ctx* ctx, double* pos, double* rhs, int64_t* idx, int* step_idx) {
@@ -496,9 +518,8 @@ int bfpsh_particles_get(bfpsh_ctx* ctx, double* pos, double* rhs, int64_t* idx, 
     if (step_idx) *step_idx = ctx->step_idx;
     if (ctx->np == 0) return BFPSH_OK;
     const int64_t np = ctx->np;
-    std::vector<int64_t> order(np);
-    std::iota(order.begin(), order.end(), int64_t(0));
-    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ctx->idx[a] < ctx->idx[b]; });
+    const std::vector<int64_t>& order = ctx->order;
+    if (int64_t(order.size()) != np) return fail(ctx, BFPSH_ESTATE, "tracer state not set (particles_put)");
     std::vector<double> tmp(np * 3);
     if (pos) {
         HIPCHK(hipMemcpyAsync(tmp.data(), ctx->d_pos, np * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -535,23 +556,34 @@ int bfpsh_particles_complete_loop(bfpsh_ctx* ctx, double dt) {
     return bfpsh_part_loop(ctx, dt);
 }
 
+// out[rank[p]][c] = in[p][c]: device rows into global-index order
+static __global__ void k_rank_rows(const double* __restrict__ in, double* __restrict__ out,
+                                   const uint32_t* __restrict__ rank, int64_t np, int nc) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= np * nc) return;
+    const int64_t p = i / nc;
+    out[int64_t(rank[p]) * nc + (i - p * nc)] = in[i];
+}
+
 // sample `nc` real planes [c][y][z][x] at every tracer; out [np][nc] in
 // global-index order
 static int sample_to_host(bfpsh_ctx* ctx, const void* f, int nc, double* out) {
     if (ctx->np == 0) return BFPSH_OK;
-    double* d_out = nullptr;
-    HIPCHK(hipMalloc((void**)&d_out, ctx->np * nc * sizeof(double)));
-    int r = bfpsh_part_sample(ctx, f, nc, d_out);
-    if (r) { (void)hipFree(d_out); return r; }
-    std::vector<double> tmp(ctx->np * nc);
-    HIPCHK(hipMemcpyAsync(tmp.data(), d_out, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (int64_t(ctx->order.size()) != ctx->np) return fail(ctx, BFPSH_ESTATE, "tracer state not set (particles_put)");
+    if (!ctx->d_samp) {
+        const size_t bytes = size_t(ctx->np) * 9 * 2 * sizeof(double);
+        HIPCHK(hipMalloc((void**)&ctx->d_samp, bytes));
+        ctx->bytes += bytes;
+    }
+    double* raw = ctx->d_samp;
+    double* ranked = ctx->d_samp + size_t(ctx->np) * 9;
+    int r = bfpsh_part_sample(ctx, f, nc, raw);
+    if (r) return r;
+    const int64_t tot = ctx->np * nc;
+    LAUNCH("rank_rows", (k_rank_rows<<<unsigned((tot + 255) / 256), 256, 0, ctx->stream>>>(raw, ranked, ctx->d_rank,
+                                                                                          ctx->np, nc)));
+    HIPCHK(hipMemcpyAsync(out, ranked, size_t(tot) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipFree(d_out));
-    std::vector<int64_t> order(ctx->np);
-    std::iota(order.begin(), order.end(), int64_t(0));
-    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return ctx->idx[a] < ctx->idx[b]; });
-    for (int64_t i = 0; i < ctx->np; i++)
-        for (int c = 0; c < nc; c++) out[i * nc + c] = tmp[order[i] * nc + c];
     return BFPSH_OK;
 }
 

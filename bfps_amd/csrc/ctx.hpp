@@ -52,6 +52,9 @@ struct bfpsh_ctx {
     double* d_rhs = nullptr;
     double* d_tab = nullptr;
     std::vector<int64_t> idx;
+    std::vector<int64_t> order;    // slots in global-index order (set by particles_put)
+    uint32_t* d_rank = nullptr;    // rank of each slot in global-index order
+    double* d_samp = nullptr;      // sampling output, 2 x np x 9 doubles (raw, ranked)
     // standalone fields (field<rnumber, HIP, fc>): device buffer + ncomp per id
     std::vector<void*> fields;
     std::vector<int> field_nc;

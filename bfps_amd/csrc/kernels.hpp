@@ -78,6 +78,18 @@ struct StepParams {
 
 BFPSH_DEV double kwave(int i, int n, double dk) { return (i <= n / 2 ? i : i - n) * dk; }
 
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
+// Remap a 2D grid so that each XCD walks a contiguous range of logical blocks
+// (x fastest): neighbouring x tiles, which share 128-byte lines when a tile
+// row is 64 bytes, then run on the same XCD back to back.
+BFPSH_DEV void xcd_tile(int& bx, int& by) {
+    const unsigned gx = gridDim.x, total = gridDim.x * gridDim.y;
+    unsigned p = blockIdx.x + blockIdx.y * gx;
+    if (total % 8u == 0) p = (p % 8u) * (total / 8u) + p / 8u;
+    bx = int(p % gx);
+    by = int(p / gx);
+}
+
 // ------------------------------------------------------------ K1 + first inverse pass
 // MODE 0: outputs u_hat (3), omega_hat (3)        [nonlinear term]
 // MODE 1: outputs u_hat (3)                        [velocity in real space]
@@ -102,8 +114,9 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
     __shared__ C2<T> lds[N * B];
     const int tid = threadIdx.x;
     const int b = tid % B, grp = tid / B;
-    const int kxi = (blockIdx.x + tile0) * B + b;
-    const int ky = blockIdx.y;
+    int tile, ky;
+    xcd_tile(tile, ky);
+    const int kxi = (tile + tile0) * B + b;
     const double kx = kxi * g.dkx, kyv = kwave(ky, N, g.dky);
     C2<T> wr[3][E];
 #pragma unroll

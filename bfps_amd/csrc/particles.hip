@@ -265,6 +265,25 @@ struct BrickCfg {
     }
 };
 
+// Visiting order of the bricks: each XCD walks a contiguous range of logical
+// blocks; logical blocks run through BG x BG x BG groups of bricks so that
+// the blocks resident on one XCD at a time share their halos through its L2
+// (the brick id itself stays x-fastest, as the bucket keys use it).
+#ifndef BFPSH_BRICK_GROUP
+#define BFPSH_BRICK_GROUP 4
+#endif
+__device__ __forceinline__ int64_t brick_of(int64_t logical, int nbx, int nby, int nbz) {
+    constexpr int BG = BFPSH_BRICK_GROUP;
+    if (BG <= 1 || nbx % BG || nby % BG || nbz % BG) return logical;
+    constexpr int64_t G3 = int64_t(BG) * BG * BG;
+    const int64_t g = logical / G3, w = logical % G3;
+    const int ngx = nbx / BG, ngy = nby / BG;
+    const int bx = int(g % ngx) * BG + int(w % BG);
+    const int by = int((g / ngx) % ngy) * BG + int((w / BG) % BG);
+    const int bz = int(g / (int64_t(ngx) * ngy)) * BG + int(w / (BG * BG));
+    return (int64_t(bz) * nby + by) * nbx + bx;
+}
+
 __global__ void k_brick_keys(const double* __restrict__ pos, int64_t np, PartGeomD pg, int kx, int ky, int kz,
                              int nbx, int nby, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -309,8 +328,9 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
     double* stab = frac + PB * 3;                // weight table, derivative 0
     int* lcell = reinterpret_cast<int*>(stab + W * (deg + 1));  // [PB][3] cell in brick
     T* tile = reinterpret_cast<T*>(lcell + ((PB * 3 + 1) / 2) * 2);
-    const int64_t brick = xcd_block();
-    if (brick >= int64_t(nbx) * nby * (pg.n / BC::KZ)) return;
+    const int nbz = pg.n / BC::KZ;
+    const int64_t brick = brick_of(xcd_block(), nbx, nby, nbz);
+    if (brick >= int64_t(nbx) * nby * nbz) return;
     const uint32_t s0 = bstart[brick], e0 = bend[brick];
     if (s0 >= e0) return;  // uniform over the workgroup
     const int tid = threadIdx.x;
@@ -345,7 +365,10 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
             // tile fill in chunks; all loads of a chunk are issued before its
             // LDS stores
             constexpr int PERT = (BC::TILE + NT - 1) / NT;
-            constexpr int CH = PERT < 16 ? PERT : 16;  // loads in flight per thread
+#ifndef BFPSH_FILL_CHUNK
+#define BFPSH_FILL_CHUNK 16
+#endif
+            constexpr int CH = PERT < BFPSH_FILL_CHUNK ? PERT : BFPSH_FILL_CHUNK;  // loads in flight per thread
             const T* fc = f + c * ps;
 #pragma unroll
             for (int k0 = 0; k0 < PERT; k0 += CH) {
