@@ -18,7 +18,10 @@ steps and reports the FP32 error (vorticity L2, per-shell energy spectrum,
 tracer positions) against FP64.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 512|1024]
-                    [--accuracy STEPS] [--no-cpu] [--no-fp64]
+                    [--accuracy STEPS] [--ensemble-steps K] [--no-cpu] [--no-fp64]
+
+The ensemble leg (96 boxes of 32^3..128^3, batched vs box by box) is
+reported under "ensemble".
 
 Prints one JSON line.
 """
@@ -33,6 +36,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+LDS_PEAK = 128 * 256 * 2.4e9  # B/s: 128 B/clk per CU, 256 CUs, 2.4 GHz
 
 WORKLOADS = {
     "512": dict(n=512, nu=0.0012, forcing="linear", famp=0.5, fk0=2.0, fk1=4.0, np=2 ** 22, nb=6, m=2, k0=4.0,
@@ -158,9 +162,21 @@ def roofline(prof, n, es, np_, nb, traffic_ok, steps):
         return {"kernel": name, "avg_ms": ms / cnt}
     ach = bytes_per_launch / avg_s
     traffic = measured_traffic(name) if traffic_ok else None
-    return {"bound": "hbm", "kernel": name, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": ach / HBM_PEAK, "traffic": traffic, "avg_ms": ms / cnt, "bytes_per_launch": bytes_per_launch,
-            "bytes_note": note, "launches_per_step": cnt / steps}
+    out = {"bound": "hbm", "kernel": name, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": ach / HBM_PEAK, "traffic": traffic, "avg_ms": ms / cnt, "bytes_per_launch": bytes_per_launch,
+           "bytes_note": note, "launches_per_step": cnt / steps}
+    if name == "interp_ab":
+        # the tracer kernel is bound by LDS bandwidth: every tracer reads its
+        # (2n+2)^3 stencil of each component from the brick tile in LDS, and
+        # each brick's tile (with its 2n+1 halo) is written once per component
+        w = 2 * nb + 2
+        kx = 16 if es == 4 else 8
+        tile = (kx + w - 1) * (kx + w - 1) * (8 + w - 1)
+        nbricks = n ** 3 // (kx * kx * 8)
+        lds_bytes = np_ * 3 * w ** 3 * es + nbricks * 3 * tile * es
+        out["lds"] = {"bytes_per_launch": lds_bytes, "achieved": lds_bytes / avg_s / 1e9, "peak": LDS_PEAK / 1e9,
+                      "unit": "GB/s", "frac": lds_bytes / avg_s / LDS_PEAK}
+    return out
 
 
 def accuracy(B, ic, wl, steps):
@@ -201,6 +217,52 @@ def accuracy(B, ic, wl, steps):
         res["tracer_drift_rms"] = float(np.sqrt(np.mean(dp ** 2)))
         res["grid_spacing"] = 2 * np.pi / n
     return res
+
+
+ENSEMBLE = [(32, 32), (32, 48), (24, 64), (8, 128)]  # (boxes, N)
+
+
+def ensemble_bench(B, ic, steps, warmup=2, seed=5):
+    """96 independent FP32 boxes (32 x 32^3, 32 x 48^3, 24 x 64^3, 8 x 128^3),
+    each with its own seed, nu ~ U[0.01, 0.05], k0 in {2, 3, 4}, linear band
+    forcing, dt = 0.5/N, and log-uniform [1e3, 1e5] tracers (n1 m1 AB4, the
+    reference's defaults).  The batched form (one graph launch of batched field
+    kernels per step + the tracer loop of each box) against the same step issued
+    box by box."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = [n for cnt, n in ENSEMBLE for _ in range(cnt)]
+    ens = B.ensemble(sizes, precision="single")
+    total_np = 0
+    parts = []
+    for i, n in enumerate(sizes):
+        fs = ens.boxes[i]
+        fs.nu = float(rng.uniform(0.01, 0.05))
+        fs.famplitude, fs.fk0, fs.fk1, fs.forcing_type = 0.5, 2.0, 4.0, "linear"
+        fs.io_checkpoint(read=True, data=ic.solenoidal_vorticity(n, seed=seed * 1000 + i,
+                                                                 k0=float(rng.choice([2.0, 3.0, 4.0]))))
+        npi = int(round(10 ** rng.uniform(3, 5)))
+        total_np += npi
+        ps = B.particles_system(fs, npi, 1, 1, 4, step_idx=1)
+        ps.put(ic.uniform_positions(npi, seed=seed * 1000 + 500 + i))
+        parts.append(ps)
+    dt = np.array([0.5 / n for n in sizes])
+    for _ in range(warmup):
+        ens.step(dt)
+    ens.synchronize()
+    ms_b = ens.time_steps(steps, dt)
+    for _ in range(warmup):
+        ens.step_separate(dt)
+    ens.synchronize()
+    ms_s = ens.time_steps_separate(steps, dt)
+    points = sum(n ** 3 for n in sizes)
+    ens.close()
+    return {"boxes": len(sizes), "grid_points": points, "tracers": total_np, "steps": steps,
+            "batched_ms_per_step": ms_b / steps, "per_box_ms_per_step": ms_s / steps,
+            "batched_boxes_steps_per_s": len(sizes) * steps / (ms_b / 1e3),
+            "per_box_boxes_steps_per_s": len(sizes) * steps / (ms_s / 1e3),
+            "batched_grid_pt_steps_per_s": points * steps / (ms_b / 1e3),
+            "speedup": ms_s / ms_b}
 
 
 def cpu_baseline(ic, wl, threads):
@@ -264,6 +326,8 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="512")
     ap.add_argument("--np", type=int, default=None, help="override the tracer count")
     ap.add_argument("--accuracy", type=int, default=100, help="steps of the FP32-vs-FP64 leg (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=20,
+                    help="steps of the 96-box ensemble leg (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None)
@@ -323,6 +387,8 @@ def main():
                        "kernel_ms": {k: v[0] / v[1] for k, v in merge_profile(r64["profile"]).items()}}
     if args.accuracy > 0 and not args.no_fp64:
         out["accuracy"] = accuracy(B, ic, wl, args.accuracy)
+    if args.ensemble_steps > 0:
+        out["ensemble"] = ensemble_bench(B, ic, args.ensemble_steps)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(ic, wl, threads)

@@ -13,6 +13,8 @@ Python mirror of the reference's class surface over the C ABI
 * ``field``               <- ``field<rnumber, FFTW, ONE|THREE|THREExTHREE>``
   with its in-place ``dft``/``ift``/``symmetrize`` (bfps/cpp/field.hpp:47-200)
 * ``NSVEparticles_step``  <- ``NSVEparticles::step`` (bfps/cpp/full_code/NSVEparticles.cpp:33-41)
+* ``ensemble``            -- many independent boxes advanced together (one
+  graph launch of batched field kernels per step); no reference counterpart
 
 Host arrays use the reference's layouts (complex ``[ny][nz][nx//2+1][3]``,
 real ``[nz][ny][nx][ncomp]``).  Everything runs on the GPU through
@@ -24,7 +26,7 @@ import numpy as np
 
 from . import capi
 
-__all__ = ["vorticity_equation", "kspace", "field", "particles_system", "NSVEparticles_step", "capi"]
+__all__ = ["vorticity_equation", "kspace", "field", "particles_system", "NSVEparticles_step", "ensemble", "capi"]
 
 
 class kspace:
@@ -56,7 +58,7 @@ class vorticity_equation:
     """
 
     def __init__(self, name, nx, ny=None, nz=None, dkx=1.0, dky=1.0, dkz=1.0,
-                 precision="single", dealias="SMOOTH", device=0):
+                 precision="single", dealias="SMOOTH", device=0, _handle=None):
         ny = nx if ny is None else ny
         nz = nx if nz is None else nz
         self.name = name
@@ -65,16 +67,21 @@ class vorticity_equation:
         self.rdtype = np.float32 if precision == "single" else np.float64
         self.cdtype = np.complex64 if precision == "single" else np.complex128
         lib = capi.load()
-        h = ctypes.c_void_p()
-        rc = lib.bfpsh_create(ctypes.byref(h), nx, ny, nz, dkx, dky, dkz,
-                              capi.FP32 if precision == "single" else capi.FP64,
-                              capi.SMOOTH if dealias == "SMOOTH" else capi.TWO_THIRDS, device)
+        # _handle: a box context owned by an ensemble (not destroyed here)
+        self._owned = _handle is None
+        if _handle is not None:
+            h = ctypes.c_void_p(_handle)
+        else:
+            h = ctypes.c_void_p()
+            rc = lib.bfpsh_create(ctypes.byref(h), nx, ny, nz, dkx, dky, dkz,
+                                  capi.FP32 if precision == "single" else capi.FP64,
+                                  capi.SMOOTH if dealias == "SMOOTH" else capi.TWO_THIRDS, device)
+            if rc != 0:
+                msg = lib.bfpsh_last_error(h)
+                lib.bfpsh_destroy(h)
+                self._h = None
+                raise capi.BfpshError("bfpsh_create failed (%d): %s" % (rc, msg.decode() if msg else ""))
         self._h = h
-        if rc != 0:
-            msg = lib.bfpsh_last_error(h)
-            lib.bfpsh_destroy(h)
-            self._h = None
-            raise capi.BfpshError("bfpsh_create failed (%d): %s" % (rc, msg.decode() if msg else ""))
         self.kk = kspace(h, nx, ny, nz, dkx, dky, dkz)
         # physical parameters, initialised as the reference constructor does
         # (bfps/cpp/vorticity_equation.cpp:149-155)
@@ -93,7 +100,8 @@ class vorticity_equation:
 
     def close(self):
         if getattr(self, "_h", None):
-            capi.load().bfpsh_destroy(self._h)
+            if getattr(self, "_owned", True):
+                capi.load().bfpsh_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -382,3 +390,91 @@ def NSVEparticles_step(fs, dt):
     """NSVEparticles::step: tracers advance with u(t), then the fluid step."""
     fs._push_physics()
     capi.check(fs.handle, capi.load().bfpsh_nsve_particles_step(fs.handle, float(dt)), "nsve_particles_step")
+
+
+class ensemble:
+    """Independent boxes of sizes ``sizes`` (box 2 pi, one precision) advanced
+    together.  ``boxes[i]`` is a ``vorticity_equation`` over box i's context:
+    set its physics, fields and tracers as for a single box; ``step`` then
+    runs one NSVEparticles step of every box (box i with ``dt[i]``)."""
+
+    def __init__(self, sizes, precision="single", dealias="SMOOTH", device=0):
+        lib = capi.load()
+        sizes = [int(n) for n in sizes]
+        h = ctypes.c_void_p()
+        arr = (ctypes.c_int * len(sizes))(*sizes)
+        rc = lib.bfpsh_ensemble_create(ctypes.byref(h), len(sizes), arr,
+                                       capi.FP32 if precision == "single" else capi.FP64,
+                                       capi.SMOOTH if dealias == "SMOOTH" else capi.TWO_THIRDS, device)
+        if rc != 0:
+            msg = lib.bfpsh_ensemble_last_error(h)
+            lib.bfpsh_ensemble_destroy(h)
+            raise capi.BfpshError("bfpsh_ensemble_create failed (%d): %s" % (rc, msg.decode() if msg else ""))
+        self._h = h
+        self.sizes = sizes
+        self.boxes = []
+        for i, n in enumerate(sizes):
+            b = ctypes.c_void_p()
+            self._check(lib.bfpsh_ensemble_box(h, i, ctypes.byref(b)), "ensemble_box")
+            self.boxes.append(vorticity_equation("box%d" % i, n, precision=precision, dealias=dealias,
+                                                 _handle=b.value))
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = capi.load().bfpsh_ensemble_last_error(self._h)
+            raise capi.BfpshError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def _dt(self, dt):
+        d = np.broadcast_to(np.asarray(dt, dtype=np.float64), (len(self.sizes),))
+        return np.ascontiguousarray(d)
+
+    def step(self, dt, with_particles=True):
+        for b in self.boxes:
+            b._push_physics()
+        d = self._dt(dt)
+        self._check(capi.load().bfpsh_ensemble_step(self._h, d.ctypes.data_as(capi._DP), int(bool(with_particles))),
+                    "ensemble_step")
+
+    def time_steps(self, nsteps, dt, with_particles=True):
+        for b in self.boxes:
+            b._push_physics()
+        d = self._dt(dt)
+        ms = ctypes.c_double()
+        self._check(capi.load().bfpsh_ensemble_time_steps(self._h, int(nsteps), d.ctypes.data_as(capi._DP),
+                                                          int(bool(with_particles)), ctypes.byref(ms)),
+                    "ensemble_time_steps")
+        return ms.value
+
+    def step_separate(self, dt, with_particles=True):
+        """The same step issued box by box (no batching, no graph)."""
+        for b in self.boxes:
+            b._push_physics()
+        d = self._dt(dt)
+        self._check(capi.load().bfpsh_ensemble_step_separate(self._h, d.ctypes.data_as(capi._DP),
+                                                             int(bool(with_particles))), "ensemble_step_separate")
+
+    def time_steps_separate(self, nsteps, dt, with_particles=True):
+        for b in self.boxes:
+            b._push_physics()
+        d = self._dt(dt)
+        ms = ctypes.c_double()
+        self._check(capi.load().bfpsh_ensemble_time_steps_separate(
+            self._h, int(nsteps), d.ctypes.data_as(capi._DP), int(bool(with_particles)), ctypes.byref(ms)),
+            "ensemble_time_steps_separate")
+        return ms.value
+
+    def synchronize(self):
+        self._check(capi.load().bfpsh_ensemble_synchronize(self._h), "ensemble_synchronize")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for b in self.boxes:
+                b._h = None
+            capi.load().bfpsh_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -53,6 +53,15 @@ SIGNATURES = {
     "bfpsh_particles_sample": (_I, [_P, _I, _DP]),
     "bfpsh_particles_sample_field": (_I, [_P, _I, _DP]),
     "bfpsh_nsve_particles_step": (_I, [_P, _D]),
+    "bfpsh_ensemble_create": (_I, [ctypes.POINTER(_P), _I, _IP, _I, _I, _I]),
+    "bfpsh_ensemble_destroy": (_I, [_P]),
+    "bfpsh_ensemble_last_error": (ctypes.c_char_p, [_P]),
+    "bfpsh_ensemble_box": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "bfpsh_ensemble_step": (_I, [_P, _DP, _I]),
+    "bfpsh_ensemble_time_steps": (_I, [_P, _I, _DP, _I, _DP]),
+    "bfpsh_ensemble_step_separate": (_I, [_P, _DP, _I]),
+    "bfpsh_ensemble_time_steps_separate": (_I, [_P, _I, _DP, _I, _DP]),
+    "bfpsh_ensemble_synchronize": (_I, [_P]),
     "bfpsh_synchronize": (_I, [_P]),
     "bfpsh_get_iteration": (_I, [_P, _IP]),
     "bfpsh_set_iteration": (_I, [_P, _I]),

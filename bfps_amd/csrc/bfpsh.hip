@@ -94,6 +94,15 @@ extern "C" {
 
 int bfpsh_create(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, double dkz, int precision,
                  int dealias, int device) {
+    return create_ctx(out, nx, ny, nz, dkx, dky, dkz, precision, dealias, device, nullptr, nullptr);
+}
+
+}  // extern "C"
+
+// A context; with `stream` it runs on that (shared) stream, with `slab` its
+// v0 | v1 | v2 | work (6 planes) | ru buffers are the 18 planes at `slab`.
+int create_ctx(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, double dkz, int precision,
+               int dealias, int device, hipStream_t stream, void* slab) {
     if (!out) return BFPSH_EINVAL;
     *out = nullptr;
     bfpsh_ctx* ctx = new bfpsh_ctx();
@@ -115,7 +124,12 @@ int bfpsh_create(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky
     ctx->dkz = dkz;
     ctx->es = precision == BFPSH_FP32 ? 4 : 8;
     HIPCHK(hipSetDevice(device));
-    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    if (stream) {
+        ctx->stream = stream;
+        ctx->own_stream = false;
+    } else {
+        HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    }
     // kspace geometry (bfps/cpp/kspace.cpp:56-166), cubic grid
     const int n = nx, nxh = n / 2 + 1;
     double kMx, kMy, kMz;
@@ -152,10 +166,24 @@ int bfpsh_create(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky
     // device state
     ctx->ps = size_t(n) * n * (n / 2);
     const size_t vbytes = 3 * ctx->ps * 2 * ctx->es;
-    for (void** p : {&ctx->v0, &ctx->v1, &ctx->v2}) {
-        HIPCHK(hipMalloc(p, vbytes));
-        HIPCHK(hipMemsetAsync(*p, 0, vbytes, ctx->stream));
-        ctx->bytes += vbytes;
+    if (slab) {
+        // 18 planes: v0 (3), v1 (3), v2 (3), work (6), real velocity (3); the
+        // owner clears the slab
+        char* b = (char*)slab;
+        const size_t pb = ctx->ps * 2 * ctx->es;
+        ctx->slab = slab;
+        ctx->v0 = b;
+        ctx->v1 = b + 3 * pb;
+        ctx->v2 = b + 6 * pb;
+        ctx->work = b + 9 * pb;
+        ctx->work_planes = 6;
+        ctx->ru = b + 15 * pb;
+    } else {
+        for (void** p : {&ctx->v0, &ctx->v1, &ctx->v2}) {
+            HIPCHK(hipMalloc(p, vbytes));
+            HIPCHK(hipMemsetAsync(*p, 0, vbytes, ctx->stream));
+            ctx->bytes += vbytes;
+        }
     }
     std::vector<unsigned char> tw;
     if (precision == BFPSH_FP32) build_twiddles<float>(tw, n); else build_twiddles<double>(tw, n);
@@ -168,6 +196,8 @@ int bfpsh_create(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky
     return BFPSH_OK;
 }
 
+extern "C" {
+
 int bfpsh_destroy(bfpsh_ctx* ctx) {
     if (!ctx) return BFPSH_OK;
     if (ctx->stream) {
@@ -176,13 +206,13 @@ int bfpsh_destroy(bfpsh_ctx* ctx) {
         for (void* p : {ctx->v0, ctx->v1, ctx->v2, ctx->work, ctx->ru, ctx->rgrad, ctx->tw,
                         (void*)ctx->d_pos, (void*)ctx->d_rhs, (void*)ctx->d_tab, (void*)ctx->d_famp,
                         (void*)ctx->d_rank, (void*)ctx->d_samp})
-            if (p) (void)hipFree(p);
+            if (p && !in_slab(ctx, p)) (void)hipFree(p);
         for (void* p : ctx->fields)
             if (p) (void)hipFree(p);
         bfpsh_part_release(ctx);
         resolve_profile(ctx);
         for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
-        (void)hipStreamDestroy(ctx->stream);
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;
     return BFPSH_OK;
@@ -362,9 +392,9 @@ int bfpsh_field_symmetrize(bfpsh_ctx* ctx, int id) {
     if (!field_ok(ctx, id)) return fail(ctx, BFPSH_EINVAL, "bad field id");
     const int n = ctx->n;
     if (ctx->prec == BFPSH_FP32)
-        LAUNCH("symmetrize", (k_symmetrize<float><<<(n * n + 255) / 256, 256, 0, ctx->stream>>>((float2*)ctx->fields[id], n, ctx->field_nc[id])));
+        LAUNCH("symmetrize", (k_symmetrize<float><<<(n * n + 255) / 256, 256, 0, ctx->stream>>>((float2*)ctx->fields[id], n, ctx->field_nc[id], 0)));
     else
-        LAUNCH("symmetrize", (k_symmetrize<double><<<(n * n + 255) / 256, 256, 0, ctx->stream>>>((double2*)ctx->fields[id], n, ctx->field_nc[id])));
+        LAUNCH("symmetrize", (k_symmetrize<double><<<(n * n + 255) / 256, 256, 0, ctx->stream>>>((double2*)ctx->fields[id], n, ctx->field_nc[id], 0)));
     return BFPSH_OK;
 }
 
@@ -650,6 +680,291 @@ int bfpsh_time_steps(bfpsh_ctx* ctx, int nsteps, double dt, int with_particles, 
     *ms = t;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    return BFPSH_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ ensembles
+// Independent boxes advanced together: boxes of one size form a class whose
+// 18-plane buffers (v0 | v1 | v2 | work | ru) sit in one slab, so each field
+// kernel of a step covers the whole class in one launch (an extra grid
+// dimension over the boxes, per-box physics from a device table).  The field
+// launches of all classes are captured once as a HIP graph and replayed each
+// step; the tracers of each box follow with the stage-1 velocity.
+struct bfpsh_ensemble {
+    int device = 0, prec = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<bfpsh_ctx*> boxes;
+    struct Cls {
+        int n = 0;
+        std::vector<int> members;
+        void* slab = nullptr;
+        size_t bstride = 0;            // complex elements between boxes
+        StepParams* d_sp = nullptr;    // per-box parameters
+        double* d_famp = nullptr;      // 3 band amplitudes per box
+        std::vector<StepParams> sp;    // host copy of d_sp
+    };
+    std::vector<Cls> cls;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool captured_ru = false;          // the graph writes the real velocity
+};
+
+static int efail(bfpsh_ensemble* e, int code, const std::string& msg) {
+    e->err = msg;
+    return code;
+}
+#define EHIPCHK(x)                                                              \
+    do {                                                                        \
+        hipError_t e_ = (x);                                                    \
+        if (e_ != hipSuccess) return efail(e, BFPSH_EHIP, hipGetErrorString(e_)); \
+    } while (0)
+
+static void ensemble_free(bfpsh_ensemble* e) {
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (bfpsh_ctx* c : e->boxes)
+        if (c) bfpsh_destroy(c);
+    for (auto& c : e->cls) {
+        for (void* p : {c.slab, (void*)c.d_sp, (void*)c.d_famp})
+            if (p) (void)hipFree(p);
+    }
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+// Upload the per-box parameters (physics of each box context, its dt);
+// returns whether anything changed.
+static int ensemble_params(bfpsh_ensemble* e, const double* dt, bool* changed) {
+    *changed = false;
+    for (auto& c : e->cls) {
+        bool diff = false;
+        for (size_t j = 0; j < c.members.size(); j++) {
+            StepParams p = e->boxes[c.members[j]]->sp;
+            p.dt = dt[c.members[j]];
+            p.famp_dev = c.d_famp + 3 * j;
+            if (std::memcmp(&p, &c.sp[j], sizeof(StepParams)) != 0) {
+                c.sp[j] = p;
+                diff = true;
+            }
+        }
+        if (diff) {
+            EHIPCHK(hipStreamSynchronize(e->stream));
+            EHIPCHK(hipMemcpy(c.d_sp, c.sp.data(), c.sp.size() * sizeof(StepParams), hipMemcpyHostToDevice));
+            *changed = true;
+        }
+    }
+    return BFPSH_OK;
+}
+
+static int ensemble_capture(bfpsh_ensemble* e, bool with_ru) {
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    // no per-launch profiling events inside the graph
+    std::vector<bool> prof(e->boxes.size());
+    for (size_t i = 0; i < e->boxes.size(); i++) {
+        prof[i] = e->boxes[i]->prof;
+        e->boxes[i]->prof = false;
+    }
+    EHIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = BFPSH_OK;
+    for (auto& c : e->cls) {
+        bfpsh_ctx* b0 = e->boxes[c.members[0]];
+        BatchDesc d;
+        d.nbox = int(c.members.size());
+        d.bstride = c.bstride;
+        d.spb = c.d_sp;
+        d.any_fixed = false;
+        for (const StepParams& p : c.sp) d.any_fixed |= p.forcing == BFPSH_FORCING_FIXED_ENERGY;
+        d.v0 = b0->v0;
+        d.v1 = b0->v1;
+        d.v2 = b0->v2;
+        d.w = b0->work;
+        d.ru = with_ru ? b0->ru : nullptr;
+        rc = ops_for(b0)->batch_step(b0, d);
+        if (rc) {
+            e->err = b0->err;
+            break;
+        }
+    }
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    for (size_t i = 0; i < e->boxes.size(); i++) e->boxes[i]->prof = prof[i];
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess) return efail(e, BFPSH_EHIP, hipGetErrorString(ce));
+    e->graph = g;
+    EHIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    e->captured_ru = with_ru;
+    return BFPSH_OK;
+}
+
+extern "C" {
+
+int bfpsh_ensemble_create(bfpsh_ensemble** out, int nboxes, const int* sizes, int precision, int dealias,
+                          int device) {
+    if (!out) return BFPSH_EINVAL;
+    bfpsh_ensemble* e = new bfpsh_ensemble();
+    *out = e;
+    if (nboxes < 1 || !sizes) return efail(e, BFPSH_EINVAL, "need at least one box");
+    if (precision != BFPSH_FP32 && precision != BFPSH_FP64) return efail(e, BFPSH_EINVAL, "bad precision");
+    for (int i = 0; i < nboxes; i++)
+        if (!supported_size(sizes[i])) return efail(e, BFPSH_EINVAL, "unsupported box size");
+    e->device = device;
+    e->prec = precision;
+    EHIPCHK(hipSetDevice(device));
+    EHIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    const size_t es = precision == BFPSH_FP32 ? 4 : 8;
+    e->boxes.assign(nboxes, nullptr);
+    std::vector<int> order(nboxes);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sizes[a] < sizes[b]; });
+    for (int i : order) {
+        if (e->cls.empty() || e->cls.back().n != sizes[i]) {
+            e->cls.emplace_back();
+            e->cls.back().n = sizes[i];
+        }
+        e->cls.back().members.push_back(i);
+    }
+    for (auto& c : e->cls) {
+        const size_t ps = size_t(c.n) * c.n * (c.n / 2);
+        const size_t box_bytes = 18 * ps * 2 * es;
+        const size_t nb = c.members.size();
+        c.bstride = 18 * ps;
+        EHIPCHK(hipMalloc(&c.slab, nb * box_bytes));
+        EHIPCHK(hipMemsetAsync(c.slab, 0, nb * box_bytes, e->stream));
+        EHIPCHK(hipMalloc((void**)&c.d_sp, nb * sizeof(StepParams)));
+        EHIPCHK(hipMalloc((void**)&c.d_famp, 3 * nb * sizeof(double)));
+        EHIPCHK(hipMemsetAsync(c.d_famp, 0, 3 * nb * sizeof(double), e->stream));
+        c.sp.assign(nb, StepParams{});
+        for (size_t j = 0; j < nb; j++) {
+            bfpsh_ctx* ctx = nullptr;
+            int r = create_ctx(&ctx, c.n, c.n, c.n, 1.0, 1.0, 1.0, precision, dealias, device, e->stream,
+                               (char*)c.slab + j * box_bytes);
+            e->boxes[c.members[j]] = ctx;
+            if (r) return efail(e, r, ctx ? ctx->err : "box creation failed");
+            ctx->d_famp = c.d_famp + 3 * j;  // shared with the batch table (not freed by the box)
+            ctx->force_coop = true;
+            ctx->sort_every = 16;
+            // the first step's upload always differs from this placeholder
+            c.sp[j].nu = -1.0;
+        }
+    }
+    EHIPCHK(hipStreamSynchronize(e->stream));
+    return BFPSH_OK;
+}
+
+int bfpsh_ensemble_destroy(bfpsh_ensemble* e) {
+    if (!e) return BFPSH_OK;
+    (void)hipSetDevice(e->device);
+    for (auto& c : e->cls)
+        for (size_t j = 0; j < c.members.size(); j++) {
+            bfpsh_ctx* ctx = e->boxes[c.members[j]];
+            if (ctx) ctx->d_famp = nullptr;  // owned by the class table
+        }
+    ensemble_free(e);
+    return BFPSH_OK;
+}
+
+const char* bfpsh_ensemble_last_error(const bfpsh_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int bfpsh_ensemble_box(bfpsh_ensemble* e, int i, bfpsh_ctx** box) {
+    if (!e || !box) return BFPSH_EINVAL;
+    if (i < 0 || i >= int(e->boxes.size())) return efail(e, BFPSH_EINVAL, "box index out of range");
+    *box = e->boxes[i];
+    return BFPSH_OK;
+}
+
+int bfpsh_ensemble_step(bfpsh_ensemble* e, const double* dt, int with_particles) {
+    if (!e || !dt) return BFPSH_EINVAL;
+    EHIPCHK(hipSetDevice(e->device));
+    bool changed = false;
+    int r = ensemble_params(e, dt, &changed);
+    if (r) return r;
+    bool any_part = false;
+    for (bfpsh_ctx* c : e->boxes) any_part |= with_particles && c->nsteps > 0 && c->np > 0;
+    if (!e->exec || e->captured_ru != any_part) {
+        r = ensemble_capture(e, any_part);
+        if (r) return r;
+    }
+    EHIPCHK(hipGraphLaunch(e->exec, e->stream));
+    for (size_t i = 0; i < e->boxes.size(); i++) {
+        bfpsh_ctx* c = e->boxes[i];
+        if (any_part && with_particles && c->nsteps > 0 && c->np > 0) {
+            r = bfpsh_part_loop(c, dt[i]);
+            if (r) return efail(e, r, c->err);
+        }
+        c->ru_valid = false;
+        c->iteration++;
+    }
+    return BFPSH_OK;
+}
+
+int bfpsh_ensemble_step_separate(bfpsh_ensemble* e, const double* dt, int with_particles) {
+    if (!e || !dt) return BFPSH_EINVAL;
+    EHIPCHK(hipSetDevice(e->device));
+    for (size_t i = 0; i < e->boxes.size(); i++) {
+        bfpsh_ctx* ctx = e->boxes[i];
+        ctx->sp.dt = dt[i];
+        int r = PREC_CALL(step, ctx, dt[i], with_particles && ctx->nsteps > 0 && ctx->np > 0);
+        if (r) return efail(e, r, ctx->err);
+    }
+    return BFPSH_OK;
+}
+
+int bfpsh_ensemble_time_steps(bfpsh_ensemble* e, int nsteps, const double* dt, int with_particles, double* ms) {
+    if (!e || !ms || nsteps < 0) return BFPSH_EINVAL;
+    EHIPCHK(hipSetDevice(e->device));
+    hipEvent_t a, b;
+    EHIPCHK(hipEventCreate(&a));
+    EHIPCHK(hipEventCreate(&b));
+    EHIPCHK(hipEventRecord(a, e->stream));
+    for (int i = 0; i < nsteps; i++) {
+        int r = bfpsh_ensemble_step(e, dt, with_particles);
+        if (r) return r;
+    }
+    EHIPCHK(hipEventRecord(b, e->stream));
+    EHIPCHK(hipEventSynchronize(b));
+    float t = 0;
+    EHIPCHK(hipEventElapsedTime(&t, a, b));
+    *ms = t;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return BFPSH_OK;
+}
+
+int bfpsh_ensemble_time_steps_separate(bfpsh_ensemble* e, int nsteps, const double* dt, int with_particles,
+                                       double* ms) {
+    if (!e || !ms || nsteps < 0) return BFPSH_EINVAL;
+    EHIPCHK(hipSetDevice(e->device));
+    hipEvent_t a, b;
+    EHIPCHK(hipEventCreate(&a));
+    EHIPCHK(hipEventCreate(&b));
+    EHIPCHK(hipEventRecord(a, e->stream));
+    for (int i = 0; i < nsteps; i++) {
+        int r = bfpsh_ensemble_step_separate(e, dt, with_particles);
+        if (r) return r;
+    }
+    EHIPCHK(hipEventRecord(b, e->stream));
+    EHIPCHK(hipEventSynchronize(b));
+    float t = 0;
+    EHIPCHK(hipEventElapsedTime(&t, a, b));
+    *ms = t;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return BFPSH_OK;
+}
+
+int bfpsh_ensemble_synchronize(bfpsh_ensemble* e) {
+    if (!e) return BFPSH_EINVAL;
+    EHIPCHK(hipStreamSynchronize(e->stream));
     return BFPSH_OK;
 }
 

@@ -28,6 +28,9 @@ struct bfpsh_ctx {
     std::vector<double> kshell;
     std::vector<int64_t> nshell;
     hipStream_t stream = nullptr;
+    bool own_stream = true;        // false: the stream of an ensemble
+    void* slab = nullptr;          // ensemble member: v0|v1|v2|work(6)|ru live here
+    bool force_coop = false;       // tracers: global-gather kernel, no per-step sort
     // device buffers (element size es = sizeof(rnumber))
     size_t es = 4;
     size_t ps = 0;                 // complex elements per spectral plane
@@ -157,12 +160,21 @@ inline Geom geom(const bfpsh_ctx* ctx) {
     return g;
 }
 
+// true when p lives in the context's ensemble slab (not freed by the context)
+inline bool in_slab(const bfpsh_ctx* ctx, const void* p) {
+    if (!ctx->slab || !p) return false;
+    const char* s = (const char*)ctx->slab;
+    return (const char*)p >= s && (const char*)p < s + 18 * ctx->ps * 2 * ctx->es;
+}
+
 inline int ensure_work(bfpsh_ctx* ctx, int planes) {
     if (ctx->work_planes >= planes) return BFPSH_OK;
     if (ctx->work) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        HIPCHK(hipFree(ctx->work));
-        ctx->bytes -= ctx->work_planes * ctx->ps * 2 * ctx->es;
+        if (!in_slab(ctx, ctx->work)) {
+            HIPCHK(hipFree(ctx->work));
+            ctx->bytes -= ctx->work_planes * ctx->ps * 2 * ctx->es;
+        }
     }
     HIPCHK(hipMalloc(&ctx->work, planes * ctx->ps * 2 * ctx->es));
     ctx->bytes += planes * ctx->ps * 2 * ctx->es;
@@ -181,6 +193,16 @@ inline int ensure_real(bfpsh_ctx* ctx, void** buf, int planes) {
 
 
 // per-(precision, N) entry points, filled by impl_inst.hip
+// A batch of equal boxes for one launch sequence: box i's v0 | v1 | v2 |
+// work | ru sit i * bstride complex elements after box 0's.
+struct BatchDesc {
+    int nbox;
+    size_t bstride;
+    const StepParams* spb;  // per-box parameters (device)
+    bool any_fixed;
+    void *v0, *v1, *v2, *w, *ru;
+};
+
 struct OpsTable {
     int (*step)(bfpsh_ctx*, double, bool);
     int (*load_projection)(bfpsh_ctx*);
@@ -190,7 +212,11 @@ struct OpsTable {
     int (*cospectrum)(bfpsh_ctx*, const void*, const void*, double*);
     int (*field_ift)(bfpsh_ctx*, void*, int);
     int (*field_dft)(bfpsh_ctx*, void*, int);
+    int (*batch_step)(bfpsh_ctx*, const BatchDesc&);
 };
+
+int create_ctx(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, double dkz, int precision,
+               int dealias, int device, hipStream_t stream, void* slab);
 
 // tracer loop and sampling (particles.hip), dispatched on precision and
 // neighbour count

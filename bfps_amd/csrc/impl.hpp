@@ -10,50 +10,89 @@ struct Impl {
     static constexpr int B = CF::B, NT = CF::NT, NXH = CF::NXH;
     typedef C2<T> CT;
 
+    // A batch of equal boxes (same N, precision, dk, dealiasing) whose
+    // v0 | v1 | v2 | work | ru buffers sit bstride complex elements apart;
+    // one context is a batch of one with bstride = 0.
+    struct Batch {
+        int nbox = 1;
+        size_t bstride = 0;
+        const StepParams* spb = nullptr;  // per-box parameters (device), or null
+        bool any_fixed = false;           // some box uses the fixed-injection forcing
+        CT *v0, *v1, *v2, *w;
+        T* ru;
+        double* famp = nullptr;           // single box: its 3 band amplitudes
+        StepParams sp;
+        Geom g;
+    };
+    static Batch single(bfpsh_ctx* ctx, double dt) {
+        Batch b;
+        b.v0 = (CT*)ctx->v0;
+        b.v1 = (CT*)ctx->v1;
+        b.v2 = (CT*)ctx->v2;
+        b.w = (CT*)ctx->work;
+        b.ru = nullptr;
+        b.sp = ctx->sp;
+        b.sp.dt = dt;
+        b.famp = ctx->d_famp;
+        b.sp.famp_dev = ctx->d_famp;
+        b.any_fixed = b.sp.forcing == BFPSH_FORCING_FIXED_ENERGY;
+        b.g = geom(ctx);
+        return b;
+    }
+
     // Biot-Savart + first inverse pass: kx tile 0 with the symmetrize
     // substitution, the remaining tiles without
     template <int MODE>
-    static int bs_pass(bfpsh_ctx* ctx, const CT* src, CT* w, const CT* tw, const Geom& g, const char* name) {
-        LAUNCH("inv_z_bs_t0", (k_inv_z_bs<T, N, MODE, true><<<dim3(1, N), CF::NT_BS, 0, ctx->stream>>>(src, w, tw, g, 0)));
+    static int bs_pass(bfpsh_ctx* ctx, const CT* src, CT* w, const CT* tw, const Geom& g, const char* name,
+                       int nbox = 1, size_t bstride = 0) {
+        LAUNCH("inv_z_bs_t0", (k_inv_z_bs<T, N, MODE, true><<<dim3(1, N, nbox), CF::NT_BS, 0, ctx->stream>>>(
+                                  src, w, tw, g, 0, bstride)));
         if (NXH / CF::B_BS > 1)
-            LAUNCH(name, (k_inv_z_bs<T, N, MODE, false><<<dim3(NXH / CF::B_BS - 1, N), CF::NT_BS, 0, ctx->stream>>>(
-                              src, w, tw, g, 1)));
+            LAUNCH(name, (k_inv_z_bs<T, N, MODE, false><<<dim3(NXH / CF::B_BS - 1, N, nbox), CF::NT_BS, 0, ctx->stream>>>(
+                              src, w, tw, g, 1, bstride)));
         return BFPSH_OK;
     }
 
-    static int stage(bfpsh_ctx* ctx, int s, double dt, T* ru) {
-        const Geom g = geom(ctx);
-        StepParams sp = ctx->sp;
-        sp.dt = dt;
-        CT* v0 = (CT*)ctx->v0;
-        CT* v1 = (CT*)ctx->v1;
-        CT* v2 = (CT*)ctx->v2;
-        CT* w = (CT*)ctx->work;
+    // RK stage s of every box of the batch; ru (box 0 address) receives the
+    // real velocity of the stage-1 source when non-null
+    static int stage(bfpsh_ctx* ctx, int s, const Batch& bt, T* ru) {
+        const Geom& g = bt.g;
+        const int nb = bt.nbox;
+        const size_t bs = bt.bstride;
+        CT* v0 = bt.v0;
+        CT* v1 = bt.v1;
+        CT* v2 = bt.v2;
+        CT* w = bt.w;
         const CT* tw = (const CT*)ctx->tw;
         const CT* src = s == 1 ? v0 : (s == 2 ? v1 : v2);
-        const dim3 tiles(NXH / B, N);
-        if (sp.forcing == BFPSH_FORCING_FIXED_ENERGY) {
-            sp.famp_dev = ctx->d_famp + (s - 1);
-            LAUNCH("band_amplitude", (k_band_amplitude<T, N><<<1, 256, 0, ctx->stream>>>(
-                                         src, ctx->d_famp + (s - 1), g, sp.famp, sp.fk0, sp.fk1)));
+        if (bt.any_fixed) {
+            if (bt.spb)
+                LAUNCH("band_amplitude", (k_band_amplitude<T, N><<<nb, 256, 0, ctx->stream>>>(
+                                             src, nullptr, g, 0.0, 0.0, 0.0, bs, bt.spb, s)));
+            else
+                LAUNCH("band_amplitude", (k_band_amplitude<T, N><<<1, 256, 0, ctx->stream>>>(
+                                             src, bt.famp + (s - 1), g, bt.sp.famp, bt.sp.fk0, bt.sp.fk1, 0, nullptr, s)));
         }
         {
-            int r = bs_pass<0>(ctx, src, w, tw, g, "inv_z_bs");
+            int r = bs_pass<0>(ctx, src, w, tw, g, "inv_z_bs", nb, bs);
             if (r) return r;
         }
-        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 6), NT, 0, ctx->stream>>>(
-                             w, size_t(N) * NXH, size_t(NXH), tw)));
-        LAUNCH("rows_cross", (k_rows_cross<T, N><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(
-                                  w, ru, tw, T(double(N) * N * N))));
-        LAUNCH("fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, 3), NT, 0, ctx->stream>>>(
-                             w, size_t(NXH), size_t(N) * NXH, tw)));
-        const dim3 tiles3(NXH / CF::B_Y3, N);
+        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 6 * nb), NT, 0, ctx->stream>>>(
+                             w, size_t(N) * NXH, size_t(NXH), tw, 6, bs)));
+        LAUNCH("rows_cross", (k_rows_cross<T, N><<<dim3(N * N / 2 / CF::RP, nb), CF::NT_ROW, 0, ctx->stream>>>(
+                                  w, ru, tw, T(double(N) * N * N), bs)));
+        LAUNCH("fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, 3 * nb), NT, 0, ctx->stream>>>(
+                             w, size_t(NXH), size_t(N) * NXH, tw, 3, bs)));
+        const dim3 tiles3(NXH / CF::B_Y3, N, nb);
         if (s == 1)
-            LAUNCH("fwd_y_update1", (k_fwd_y_update<T, N, 1><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, v1, v1, tw, g, sp)));
+            LAUNCH("fwd_y_update1", (k_fwd_y_update<T, N, 1><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(
+                                        w, v0, v1, v1, tw, g, bt.sp, bs, bt.spb)));
         else if (s == 2)
-            LAUNCH("fwd_y_update2", (k_fwd_y_update<T, N, 2><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, v1, v2, tw, g, sp)));
+            LAUNCH("fwd_y_update2", (k_fwd_y_update<T, N, 2><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(
+                                        w, v0, v1, v2, tw, g, bt.sp, bs, bt.spb)));
         else
-            LAUNCH("fwd_y_update3", (k_fwd_y_update<T, N, 3><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, v2, v0, tw, g, sp)));
+            LAUNCH("fwd_y_update3", (k_fwd_y_update<T, N, 3><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(
+                                        w, v0, v2, v0, tw, g, bt.sp, bs, bt.spb)));
         return BFPSH_OK;
     }
 
@@ -66,15 +105,16 @@ struct Impl {
             if (r) return r;
             ru = (T*)ctx->ru;
         }
-        r = stage(ctx, 1, dt, ru);
+        const Batch bt = single(ctx, dt);
+        r = stage(ctx, 1, bt, ru);
         if (r) return r;
         if (particles) {
             r = bfpsh_part_loop(ctx, dt);
             if (r) return r;
         }
-        r = stage(ctx, 2, dt, nullptr);
+        r = stage(ctx, 2, bt, nullptr);
         if (r) return r;
-        r = stage(ctx, 3, dt, nullptr);
+        r = stage(ctx, 3, bt, nullptr);
         if (r) return r;
         r = symmetrize(ctx, ctx->v0, 3);
         if (r) return r;
@@ -83,10 +123,39 @@ struct Impl {
         return BFPSH_OK;
     }
 
+    static int batch_step_desc(bfpsh_ctx* ctx, const BatchDesc& d) {
+        Batch bt;
+        bt.nbox = d.nbox;
+        bt.bstride = d.bstride;
+        bt.spb = d.spb;
+        bt.any_fixed = d.any_fixed;
+        bt.v0 = (CT*)d.v0;
+        bt.v1 = (CT*)d.v1;
+        bt.v2 = (CT*)d.v2;
+        bt.w = (CT*)d.w;
+        bt.ru = (T*)d.ru;
+        bt.sp = ctx->sp;
+        bt.g = geom(ctx);
+        return batch_step(ctx, bt);
+    }
+
+    // The field part of one step of a whole batch (the tracers of each box
+    // are advanced separately with the stage-1 real velocity left in ru)
+    static int batch_step(bfpsh_ctx* ctx, const Batch& bt) {
+        for (int s = 1; s <= 3; s++) {
+            int r = stage(ctx, s, bt, s == 1 ? bt.ru : nullptr);
+            if (r) return r;
+        }
+        const int nthreads = 256;
+        LAUNCH("symmetrize", (k_symmetrize<T><<<dim3((N * N + nthreads - 1) / nthreads, bt.nbox), nthreads, 0,
+                                                 ctx->stream>>>(bt.v0, N, 3, bt.bstride)));
+        return BFPSH_OK;
+    }
+
     static int symmetrize(bfpsh_ctx* ctx, void* v, int nc) {
         const int nthreads = 256;
         const int nblocks = (N * N + nthreads - 1) / nthreads;
-        LAUNCH("symmetrize", (k_symmetrize<T><<<nblocks, nthreads, 0, ctx->stream>>>((CT*)v, N, nc)));
+        LAUNCH("symmetrize", (k_symmetrize<T><<<nblocks, nthreads, 0, ctx->stream>>>((CT*)v, N, nc, 0)));
         return BFPSH_OK;
     }
 
@@ -108,7 +177,7 @@ struct Impl {
         const CT* tw = (const CT*)ctx->tw;
         r = bs_pass<1>(ctx, (const CT*)ctx->v0, w, tw, geom(ctx), "inv_z_bs_u");
         if (r) return r;
-        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 3), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw)));
+        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 3), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw, 3, 0)));
         LAUNCH("rows_c2r3", (k_rows_c2r<T, N, 3><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->ru, tw)));
         ctx->ru_valid = true;
         return BFPSH_OK;
@@ -123,7 +192,7 @@ struct Impl {
         const CT* tw = (const CT*)ctx->tw;
         r = bs_pass<2>(ctx, (const CT*)ctx->v0, w, tw, geom(ctx), "inv_z_bs_grad");
         if (r) return r;
-        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 9), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw)));
+        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 9), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw, 9, 0)));
         LAUNCH("rows_c2r9", (k_rows_c2r<T, N, 9><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->rgrad, tw)));
         return BFPSH_OK;
     }
@@ -134,8 +203,8 @@ struct Impl {
     static int field_ift_nc(bfpsh_ctx* ctx, void* data) {
         CT* d = (CT*)data;
         const CT* tw = (const CT*)ctx->tw;
-        LAUNCH("field_inv_z", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), size_t(N) * NXH, tw)));
-        LAUNCH("field_inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(N) * NXH, size_t(NXH), tw)));
+        LAUNCH("field_inv_z", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), size_t(N) * NXH, tw, NC, 0)));
+        LAUNCH("field_inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(N) * NXH, size_t(NXH), tw, NC, 0)));
         LAUNCH("field_rows_c2r", (k_rows_c2r<T, N, NC><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(d, (T*)d, tw)));
         return BFPSH_OK;
     }
@@ -144,8 +213,8 @@ struct Impl {
         CT* d = (CT*)data;
         const CT* tw = (const CT*)ctx->tw;
         LAUNCH("field_rows_r2c", (k_rows_r2c<T, N, NC><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>((const T*)d, d, tw)));
-        LAUNCH("field_fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), size_t(N) * NXH, tw)));
-        LAUNCH("field_fwd_y", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(N) * NXH, size_t(NXH), tw)));
+        LAUNCH("field_fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), size_t(N) * NXH, tw, NC, 0)));
+        LAUNCH("field_fwd_y", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(N) * NXH, size_t(NXH), tw, NC, 0)));
         return BFPSH_OK;
     }
     static int field_ift(bfpsh_ctx* ctx, void* data, int nc) {

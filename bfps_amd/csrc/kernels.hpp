@@ -73,7 +73,7 @@ struct StepParams {
     double nu, dt, famp, fk0, fk1;
     int forcing;  // 0 none, 1 linear, 2 Kolmogorov, 3 fixed energy-injection rate
     int fmode;
-    const double* famp_dev;  // forcing 3: the stage's band amplitude (device)
+    const double* famp_dev;  // forcing 3: band amplitudes of the 3 stages (device)
 };
 
 BFPSH_DEV double kwave(int i, int n, double dk) { return (i <= n / 2 ? i : i - n) * dk; }
@@ -105,8 +105,12 @@ template <int MODE> struct BsOut { static constexpr int NF = MODE == 0 ? 6 : (MO
 // (blockIdx.x offset by tile0) and carry no mirror code.
 template <typename T, int N, int MODE, bool MIRROR>
 __global__ void __launch_bounds__((Cfg<T, N>::NT_BS))
-k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __restrict__ tw, Geom g, int tile0) {
+k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __restrict__ tw, Geom g, int tile0,
+           size_t bstride) {
     using CF = Cfg<T, N>;
+    // box blockIdx.z of a batch of equal boxes bstride elements apart
+    w += blockIdx.z * bstride;
+    out += blockIdx.z * bstride;
     constexpr int B = CF::B_BS, NT = CF::NT_BS, E = CF::E, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
     constexpr int NF = BsOut<MODE>::NF;
@@ -179,11 +183,13 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
 }
 
 // ------------------------------------------------------------ generic strided pass (in place)
-// element (n, b) of tile (blockIdx.x, blockIdx.y) in plane blockIdx.z at
-//   base + blockIdx.z * PS + blockIdx.y * ostride + n * estride + kx0 + b
+// element (n, b) of tile (blockIdx.x, blockIdx.y) of plane p of box q
+// (blockIdx.z = q * planes + p) at
+//   base + q * bstride + p * PS + blockIdx.y * ostride + n * estride + kx0 + b
 template <typename T, int N, int SIGN>
 __global__ void __launch_bounds__((Cfg<T, N>::NT))
-k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __restrict__ tw) {
+k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __restrict__ tw, int planes,
+       size_t bstride) {
     using CF = Cfg<T, N>;
     constexpr int B = CF::B, NT = CF::NT, E = CF::E, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
@@ -191,7 +197,9 @@ k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __
     __shared__ C2<T> lds[N * B];
     const int tid = threadIdx.x;
     const int b = tid % B, grp = tid / B;
-    C2<T>* base = data + blockIdx.z * PS + blockIdx.y * ostride + blockIdx.x * B + b;
+    // blockIdx.z = box * planes + plane (boxes bstride elements apart)
+    const int box = blockIdx.z / planes, plane = blockIdx.z % planes;
+    C2<T>* base = data + box * bstride + plane * PS + blockIdx.y * ostride + blockIdx.x * B + b;
     C2<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; e++) v[e] = base[F::in_row(grp, e) * estride];
@@ -254,12 +262,15 @@ BFPSH_DEV void rows_pack_forward(const C2<T>* lds, int col, C2<T>* ra, C2<T>* rb
 //      velocity ru (3 real planes [y][z][x]) for the tracers.
 template <typename T, int N>
 __global__ void __launch_bounds__((Cfg<T, N>::NT_ROW))
-k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restrict__ tw, T npoints) {
+k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restrict__ tw, T npoints, size_t bstride) {
     using CF = Cfg<T, N>;
     // LDS row stride: odd, so lanes striding by rows hit distinct banks
     constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = 6 * RP + 1;
     constexpr size_t PS = size_t(N) * N * NXH;
     __shared__ C2<T> lds[N * LS];
+    // box blockIdx.y of a batch (complex planes and real velocity move together)
+    data += blockIdx.y * bstride;
+    if (ru) ru += blockIdx.y * 2 * bstride;
     const int tid = threadIdx.x;
     const int q0 = blockIdx.x * RP;  // first row pair; rows (y, z) flattened: row = y*N + z
     {
@@ -398,8 +409,16 @@ k_rows_r2c(const T* data, C2<T>* out, const C2<T>* __restrict__ tw) {
 template <typename T, int N, int STAGE>
 __global__ void __launch_bounds__((Cfg<T, N>::NT_Y3))
 k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __restrict__ vp,
-               C2<T>* dst, const C2<T>* __restrict__ tw, Geom g, StepParams sp) {
+               C2<T>* dst, const C2<T>* __restrict__ tw, Geom g, StepParams sp0, size_t bstride,
+               const StepParams* __restrict__ spb) {
     using CF = Cfg<T, N>;
+    // box blockIdx.z of a batch: its fields bstride elements apart and, when
+    // spb is given, its own physics parameters
+    data += blockIdx.z * bstride;
+    v0 += blockIdx.z * bstride;
+    vp += blockIdx.z * bstride;
+    dst += blockIdx.z * bstride;
+    const StepParams sp = spb ? spb[blockIdx.z] : sp0;
     constexpr int B = CF::B_Y3, NT = CF::NT_Y3, E = CF::E, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
     using F = RegFft<T, N, B, B, NT, -1>;
@@ -446,7 +465,7 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
         if (sp.forcing == 1 || sp.forcing == 3) {
             const double kn = sqrt(k2);
             if (sp.fk0 <= kn && sp.fk1 >= kn) {
-                const double amp = sp.forcing == 1 ? sp.famp : *sp.famp_dev;
+                const double amp = sp.forcing == 1 ? sp.famp : sp.famp_dev[STAGE - 1];
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     const C2<T> s = srcv[c * PS + off];
@@ -531,7 +550,19 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
 // block scans the box |k_i| <= fk1 and writes the amplitude to *amp.
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_band_amplitude(const C2<T>* __restrict__ w, double* amp, Geom g, double eps,
-                                                        double fk0, double fk1) {
+                                                        double fk0, double fk1, size_t bstride,
+                                                        const StepParams* __restrict__ spb, int stage) {
+    // box blockIdx.x of a batch: its source field, its own rate, band and
+    // amplitude slot (spb[box].famp_dev + stage - 1)
+    if (spb) {
+        const StepParams& p = spb[blockIdx.x];
+        if (p.forcing != 3) return;
+        w += blockIdx.x * bstride;
+        amp = const_cast<double*>(p.famp_dev) + (stage - 1);
+        eps = p.famp;
+        fk0 = p.fk0;
+        fk1 = p.fk1;
+    }
     constexpr int NXH = N / 2;
     constexpr size_t PS = size_t(N) * N * NXH;
     __shared__ double part[256];
@@ -572,7 +603,8 @@ __global__ void __launch_bounds__(256) k_band_amplitude(const C2<T>* __restrict_
 // Im(0,0,0) = 0; (0, kz) = conj(0, N-kz) for N/2 < kz < N;
 // (ky, kz) = conj(N-ky, (N-kz) mod N) for N/2 < ky < N.
 template <typename T>
-__global__ void k_symmetrize(C2<T>* v, int n, int nc) {
+__global__ void k_symmetrize(C2<T>* v, int n, int nc, size_t bstride) {
+    v += blockIdx.y * bstride;  // box blockIdx.y of a batch
     const int nxh = n / 2;
     const size_t ps = size_t(n) * n * nxh;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;

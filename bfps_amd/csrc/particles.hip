@@ -602,10 +602,12 @@ int launch_brick(bfpsh_ctx* ctx, const void* f, double* d_out, double dt, int or
 template <typename T, int NB>
 int loop_nb(bfpsh_ctx* ctx, double dt, int order) {
     if constexpr (BrickCfg<T, NB>::ok) {
+        if (ctx->force_coop) goto coop;
         int r = sort_bricks<T, NB>(ctx);
         if (r) return r;
         return launch_brick<T, NB, 3, 0>(ctx, ctx->ru, nullptr, dt, order, "interp_ab");
     }
+coop:
     if (ctx->sort_age >= ctx->sort_every) {
         int r = sort_tracers(ctx);
         if (r) return r;

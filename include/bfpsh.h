@@ -160,6 +160,29 @@ int bfpsh_particles_sample_field(bfpsh_ctx* ctx, int id, double* out);
  * with the tracer velocity taken from the first RK stage (same data). */
 int bfpsh_nsve_particles_step(bfpsh_ctx* ctx, double dt);
 
+/* Ensembles: many independent boxes (sizes[i]^3, one precision, box 2 pi,
+ * per-box physics and tracers) advanced together.  Each NSVEparticles::step
+ * (bfps/cpp/full_code/NSVEparticles.cpp:33-41) of every box runs as one HIP
+ * graph launch of batched field kernels (one launch per kernel and box size)
+ * followed by the tracer loop of each box.  The reference runs one box per
+ * job; the box contexts returned by bfpsh_ensemble_box accept every bfpsh_*
+ * call above (physics, fields, tracers) and are owned by the ensemble. */
+typedef struct bfpsh_ensemble bfpsh_ensemble;
+int bfpsh_ensemble_create(bfpsh_ensemble** out, int nboxes, const int* sizes, int precision, int dealias,
+                          int device);
+int bfpsh_ensemble_destroy(bfpsh_ensemble* e);
+const char* bfpsh_ensemble_last_error(const bfpsh_ensemble* e);
+int bfpsh_ensemble_box(bfpsh_ensemble* e, int i, bfpsh_ctx** box);
+/* One step of every box, box i with time step dt[i]. */
+int bfpsh_ensemble_step(bfpsh_ensemble* e, const double* dt, int with_particles);
+int bfpsh_ensemble_time_steps(bfpsh_ensemble* e, int nsteps, const double* dt, int with_particles, double* ms);
+/* The same step issued box by box (one launch sequence per box, no batching
+ * or graph): the per-box baseline the batched form is measured against. */
+int bfpsh_ensemble_step_separate(bfpsh_ensemble* e, const double* dt, int with_particles);
+int bfpsh_ensemble_time_steps_separate(bfpsh_ensemble* e, int nsteps, const double* dt, int with_particles,
+                                       double* ms);
+int bfpsh_ensemble_synchronize(bfpsh_ensemble* e);
+
 int bfpsh_synchronize(bfpsh_ctx* ctx);
 /* Current iteration counter (vorticity_equation::iteration). */
 int bfpsh_get_iteration(bfpsh_ctx* ctx, int* it);
