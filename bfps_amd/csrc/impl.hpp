@@ -84,15 +84,18 @@ struct Impl {
         LAUNCH("fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, 3 * nb), NT, 0, ctx->stream>>>(
                              w, size_t(NXH), size_t(N) * NXH, tw, 3, bs)));
         const dim3 tiles3(NXH / CF::B_Y3, N, nb);
-        if (s == 1)
-            LAUNCH("fwd_y_update1", (k_fwd_y_update<T, N, 1><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(
-                                        w, v0, v1, v1, tw, g, bt.sp, bs, bt.spb)));
-        else if (s == 2)
-            LAUNCH("fwd_y_update2", (k_fwd_y_update<T, N, 2><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(
-                                        w, v0, v1, v2, tw, g, bt.sp, bs, bt.spb)));
-        else
-            LAUNCH("fwd_y_update3", (k_fwd_y_update<T, N, 3><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(
-                                        w, v0, v2, v0, tw, g, bt.sp, bs, bt.spb)));
+        const char* nm = s == 1 ? "fwd_y_update1" : (s == 2 ? "fwd_y_update2" : "fwd_y_update3");
+        const CT* vp = s == 1 ? v1 : (s == 2 ? v1 : v2);
+        CT* dst = s == 1 ? v1 : (s == 2 ? v2 : v0);
+        if (bt.spb) {
+            if (s == 1) LAUNCH(nm, (k_fwd_y_update<T, N, 1, true><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, bs, bt.spb)));
+            else if (s == 2) LAUNCH(nm, (k_fwd_y_update<T, N, 2, true><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, bs, bt.spb)));
+            else LAUNCH(nm, (k_fwd_y_update<T, N, 3, true><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, bs, bt.spb)));
+        } else {
+            if (s == 1) LAUNCH(nm, (k_fwd_y_update<T, N, 1, false><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, 0, nullptr)));
+            else if (s == 2) LAUNCH(nm, (k_fwd_y_update<T, N, 2, false><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, 0, nullptr)));
+            else LAUNCH(nm, (k_fwd_y_update<T, N, 3, false><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, 0, nullptr)));
+        }
         return BFPSH_OK;
     }
 

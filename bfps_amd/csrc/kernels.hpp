@@ -406,19 +406,22 @@ k_rows_r2c(const T* data, C2<T>* out, const C2<T>* __restrict__ tw) {
 // data: planes 0..2 hold the output of the z forward pass ([y][kz][kx]).
 // v0 = omega(t); vp = previous stage (stage 2: v1, stage 3: v2); dst receives
 // the new stage (stage 3 writes v0 in place and applies the final projection).
-template <typename T, int N, int STAGE>
+template <typename T, int N, int STAGE, bool BATCH>
 __global__ void __launch_bounds__((Cfg<T, N>::NT_Y3))
 k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __restrict__ vp,
                C2<T>* dst, const C2<T>* __restrict__ tw, Geom g, StepParams sp0, size_t bstride,
                const StepParams* __restrict__ spb) {
     using CF = Cfg<T, N>;
-    // box blockIdx.z of a batch: its fields bstride elements apart and, when
-    // spb is given, its own physics parameters
-    data += blockIdx.z * bstride;
-    v0 += blockIdx.z * bstride;
-    vp += blockIdx.z * bstride;
-    dst += blockIdx.z * bstride;
-    const StepParams sp = spb ? spb[blockIdx.z] : sp0;
+    // BATCH: box blockIdx.z of a batch, its fields bstride elements apart and
+    // its own physics parameters in spb (a separate instantiation keeps the
+    // single-box kernel's registers free of the batch bookkeeping)
+    if constexpr (BATCH) {
+        data += blockIdx.z * bstride;
+        v0 += blockIdx.z * bstride;
+        vp += blockIdx.z * bstride;
+        dst += blockIdx.z * bstride;
+    }
+    const StepParams sp = BATCH ? spb[blockIdx.z] : sp0;
     constexpr int B = CF::B_Y3, NT = CF::NT_Y3, E = CF::E, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
     using F = RegFft<T, N, B, B, NT, -1>;
