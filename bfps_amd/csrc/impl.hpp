@@ -7,7 +7,7 @@
 template <typename T, int N>
 struct Impl {
     using CF = Cfg<T, N>;
-    static constexpr int B = CF::B, NT = CF::NT, NXH = CF::NXH;
+    static constexpr int B = CF::B, NT = CF::NT_PASS, NXH = CF::NXH;
     typedef C2<T> CT;
 
     // A batch of equal boxes (same N, precision, dk, dealiasing) whose

@@ -56,7 +56,20 @@ struct Cfg {
     static constexpr int NT_BS = G * B_BS;
     // last forward pass with the update epilogue (3 components in registers)
     static constexpr int B_Y3 = B;
-    static constexpr int NT_Y3 = G * B_Y3;
+    // elements per thread of the plain strided pass (measured at 512^3 FP32:
+    // 16 elements x 512 threads beat 8 x 1024) and of the last forward pass
+    // (E; the register budget of its epilogue favours more threads)
+#ifndef BFPSH_E_PASS
+#define BFPSH_E_PASS 16
+#endif
+#ifndef BFPSH_E_Y3
+#define BFPSH_E_Y3 0
+#endif
+    static constexpr int pick_e(int e) { return (e > 0 && N % e == 0 && (N / e) * B <= 1024) ? e : E; }
+    static constexpr int E_PASS = pick_e(sizeof(T) == 4 && N == 512 ? BFPSH_E_PASS : 0);
+    static constexpr int NT_PASS = (N / E_PASS) * B;
+    static constexpr int E_Y3 = pick_e(sizeof(T) == 4 && N == 512 ? BFPSH_E_Y3 : 0);
+    static constexpr int NT_Y3 = (N / E_Y3) * B_Y3;
     static constexpr int RP = (N * int(sizeof(C2<T>)) >= 8192) ? 1 : 2;  // row pairs per row block
     static constexpr int NT_ROW = N >= 256 ? 256 : (N >= 64 ? 128 : 64);
     static_assert(NT <= 1024, "tile threads");
@@ -187,11 +200,11 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
 // (blockIdx.z = q * planes + p) at
 //   base + q * bstride + p * PS + blockIdx.y * ostride + n * estride + kx0 + b
 template <typename T, int N, int SIGN>
-__global__ void __launch_bounds__((Cfg<T, N>::NT))
+__global__ void __launch_bounds__((Cfg<T, N>::NT_PASS))
 k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __restrict__ tw, int planes,
        size_t bstride) {
     using CF = Cfg<T, N>;
-    constexpr int B = CF::B, NT = CF::NT, E = CF::E, NXH = CF::NXH;
+    constexpr int B = CF::B, NT = CF::NT_PASS, E = CF::E_PASS, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
     using F = RegFft<T, N, B, B, NT, SIGN>;
     __shared__ C2<T> lds[N * B];
@@ -422,7 +435,7 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
         dst += blockIdx.z * bstride;
     }
     const StepParams sp = BATCH ? spb[blockIdx.z] : sp0;
-    constexpr int B = CF::B_Y3, NT = CF::NT_Y3, E = CF::E, NXH = CF::NXH;
+    constexpr int B = CF::B_Y3, NT = CF::NT_Y3, E = CF::E_Y3, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
     using F = RegFft<T, N, B, B, NT, -1>;
     __shared__ C2<T> lds[N * B];
