@@ -79,8 +79,12 @@ struct Impl {
         }
         LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 6 * nb), NT, 0, ctx->stream>>>(
                              w, size_t(N) * NXH, size_t(NXH), tw, 6, bs)));
-        LAUNCH("rows_cross", (k_rows_cross<T, N><<<dim3(N * N / 2 / CF::RP, nb), CF::NT_ROW, 0, ctx->stream>>>(
-                                  w, ru, tw, T(double(N) * N * N), bs)));
+        if constexpr (RowsW<T, N>::ok && !BFPSH_ROWS_TILE)
+            LAUNCH("rows_cross", (k_rows_cross_w<T, N><<<dim3(N * N / 2 / RowsW<T, N>::RPB, nb), RowsW<T, N>::NT, 0,
+                                                         ctx->stream>>>(w, ru, tw, T(double(N) * N * N), bs)));
+        else
+            LAUNCH("rows_cross", (k_rows_cross<T, N><<<dim3(N * N / 2 / CF::RP, nb), CF::NT_ROW, 0, ctx->stream>>>(
+                                      w, ru, tw, T(double(N) * N * N), bs)));
         LAUNCH("fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, 3 * nb), NT, 0, ctx->stream>>>(
                              w, size_t(NXH), size_t(N) * NXH, tw, 3, bs)));
         const dim3 tiles3(NXH / CF::B_Y3, N, nb);

@@ -20,6 +20,19 @@
 #ifndef BFPSH_TILE_BYTES
 #define BFPSH_TILE_BYTES 128
 #endif
+// register-resident x pass (k_rows_cross_w): minimum waves per SIMD and
+// LDS padding (one slot every 2^BFPSH_RW_PAD rows); measured at 512^3 FP32,
+// 3 waves with padding every 16 rows beat 2 waves with padding every 8
+#ifndef BFPSH_RW_WAVES
+#define BFPSH_RW_WAVES 3
+#endif
+#ifndef BFPSH_RW_PAD
+#define BFPSH_RW_PAD 4
+#endif
+// 1: x-axis rows through the LDS-tile FFT for every size (A/B switch)
+#ifndef BFPSH_ROWS_TILE
+#define BFPSH_ROWS_TILE 0
+#endif
 
 namespace bfpsh {
 
@@ -333,6 +346,188 @@ k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restri
             rows_pack_forward<T, N, LS>(lds, c * RP + p, data + c * PS + row * NXH,
                                         data + c * PS + (row + 1) * NXH, tid, NT);
         }
+}
+
+// Register-resident form of k_rows_cross (FP32, N <= 512): G = N / 8 threads
+// own one row pair and hold 8 values of each of the six components, so the
+// cross product is formed in registers; the radix exchanges of each
+// component go through an LDS region private to the row pair (with padding
+// slots against bank conflicts), and the pairing of Z[k] with Z[N-k] in the packed
+// rows is a lane shuffle.  A row pair lives inside one wave (G <= 64), so the
+// exchanges need wave-level ordering only, no workgroup barrier.  Element e
+// of a thread holds row g + G sigma(e) at the input of the FFT and row
+// g + G tau(e) at its output (g = thread index within the row pair).
+template <typename T, int N>
+struct RowsW {
+    static constexpr int E = 8, G = N / E, NT = 256, RPB = NT / G, H = E / 2;
+    static constexpr int PAD = BFPSH_RW_PAD;  // one LDS slot of padding every 2^PAD rows
+    static constexpr int SL = N + (N >> PAD);  // LDS slots per row pair
+    static constexpr bool ok = sizeof(T) == 4 && N >= 64 && N <= 512 && (N & (N - 1)) == 0;
+    using P = PlanE<N, E>;
+    static constexpr int S = P::nstages;
+    static constexpr int sigma(int e) {
+        const int R = P::radix(0);
+        return e / R + (e % R) * (E / R);
+    }
+    static constexpr int tau(int e) {
+        const int R = P::radix(S - 1);
+        return e / R + (e % R) * (E / R);
+    }
+    static constexpr int tau_inv(int m) {
+        for (int e = 0; e < E; e++)
+            if (tau(e) == m) return e;
+        return -1;
+    }
+    static BFPSH_DEV int slot(int row) { return row + (row >> PAD); }
+};
+
+BFPSH_DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// FFT of NCG components of a row pair at once (their exchanges share the
+// wave-level ordering points): stage inputs in v, outputs in v
+template <typename T, int N, int SIGN, int NCG>
+BFPSH_DEV void rows_w_fft(C2<T> (&v)[NCG][8], C2<T>* reg, const C2<T>* __restrict__ tw, int g) {
+    using RW = RowsW<T, N>;
+    using F = RegFft<T, N, 1, 1, RW::G, SIGN>;
+    // an opaque copy of the twiddle pointer: the twiddles are re-read (from
+    // cache) per group instead of being held in registers across all of them
+    const C2<T>* twl = tw;
+    asm volatile("" : "+s"(twl));
+    // likewise the LDS slot addresses are recomputed per group
+    int gl = g;
+    asm volatile("" : "+v"(gl));
+    sfor<0, RW::S>([&](auto s) {
+#pragma unroll
+        for (int c = 0; c < NCG; c++) F::template compute_stage<s>(v[c], twl, gl);
+        if constexpr (s + 1 < RW::S) {
+#pragma unroll
+            for (int c = 0; c < NCG; c++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) reg[c * RW::SL + RW::slot(F::out_row_s(s, gl, e))] = v[c][e];
+            wave_sync();
+#pragma unroll
+            for (int c = 0; c < NCG; c++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) v[c][e] = reg[c * RW::SL + RW::slot(F::in_row_s(s + 1, gl, e))];
+            wave_sync();
+        }
+    });
+}
+
+template <typename T>
+BFPSH_DEV C2<T> shfl2(C2<T> a, int src) {
+    return mk<T>(__shfl(a.x, src, 64), __shfl(a.y, src, 64));
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256, BFPSH_RW_WAVES)
+k_rows_cross_w(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restrict__ tw, T npoints,
+               size_t bstride) {
+    using RW = RowsW<T, N>;
+    constexpr int E = RW::E, G = RW::G, H = RW::H, RPB = RW::RPB, NXH = N / 2;
+    constexpr size_t PS = size_t(N) * N * NXH;
+    static_assert(RW::ok && 64 % G == 0, "row pair inside one wave");
+    __shared__ C2<T> lds[RPB * 3 * RW::SL];
+    data += blockIdx.y * bstride;
+    if (ru) ru += blockIdx.y * 2 * bstride;
+    const int tid = threadIdx.x;
+    const int rp = tid / G, g = tid % G;
+    const int partner = (tid & 63) - g + (G - g) % G;  // lane holding index N - k
+    C2<T>* reg = lds + rp * 3 * RW::SL;
+    const size_t q = size_t(blockIdx.x) * RPB + rp;  // rows 2q, 2q + 1 (row = y N + z)
+    const size_t ra = 2 * q * NXH, rb = ra + NXH;
+    C2<T> v[6][E];
+    // unpack (as rows_unpack_inverse): Z[k] = A[k] + i B[k], Z[N/2] = 0,
+    // Z[N-k] = conj(A[k]) + i conj(B[k]); rows k = g + G m, m < E / 2.  The
+    // loads of omega are issued before the FFTs of u.
+    C2<T> A[3][H], B[3][H];
+    auto load = [&](int c0) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int m = 0; m < H; m++) {
+                A[c][m] = data[(c0 + c) * PS + ra + g + G * m];
+                B[c][m] = data[(c0 + c) * PS + rb + g + G * m];
+            }
+    };
+    auto unpack = [&](C2<T> (&vv)[3][E]) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            C2<T> z[E];
+#pragma unroll
+            for (int m = 0; m < H; m++) {
+                z[m] = mk<T>(A[c][m].x - B[c][m].y, A[c][m].y + B[c][m].x);
+                if (m == 0 && g == 0) z[m] = mk<T>(A[c][m].x, B[c][m].x);
+            }
+#pragma unroll
+            for (int m = H; m < E; m++) {
+                C2<T> a = shfl2<T>(A[c][E - 1 - m], partner), b = shfl2<T>(B[c][E - 1 - m], partner);
+                if (g == 0) {
+                    if (m == H) {
+                        a = mk<T>(T(0), T(0));
+                        b = a;
+                    } else {
+                        a = A[c][(E - m) % H];
+                        b = B[c][(E - m) % H];
+                    }
+                }
+                z[m] = mk<T>(a.x + b.y, b.x - a.y);
+            }
+#pragma unroll
+            for (int e = 0; e < E; e++) vv[c][e] = z[RW::sigma(e)];
+        }
+    };
+    C2<T> (&vu)[3][E] = *reinterpret_cast<C2<T>(*)[3][E]>(&v[0][0]);
+    C2<T> (&vw)[3][E] = *reinterpret_cast<C2<T>(*)[3][E]>(&v[3][0]);
+    load(0);
+    unpack(vu);
+    load(3);
+    rows_w_fft<T, N, +1, 3>(vu, reg, tw, g);
+    unpack(vw);
+    rows_w_fft<T, N, +1, 3>(vw, reg, tw, g);
+    if (ru) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int x = g + G * RW::tau(e);
+                ru[c * size_t(N) * N * N + 2 * q * N + x] = v[c][e].x;
+                ru[c * size_t(N) * N * N + (2 * q + 1) * N + x] = v[c][e].y;
+            }
+    }
+    // (u x omega) / N^3 on rows g + G tau(e), reordered to the forward FFT's
+    // input rows g + G sigma(e)
+    C2<T> f[3][E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int eo = RW::tau_inv(RW::sigma(e));
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            const C2<T> u1 = v[c1][eo], u2 = v[c2][eo], w1 = v[3 + c1][eo], w2 = v[3 + c2][eo];
+            const T re = u1.x * w2.x - u2.x * w1.x;
+            const T im = u1.y * w2.y - u2.y * w1.y;
+            f[c][e] = mk<T>(re / npoints, im / npoints);
+        }
+    }
+    rows_w_fft<T, N, -1, 3>(f, reg, tw, g);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        // pack (as rows_pack_forward): A[k] = (Z[k] + conj Z[N-k]) / 2,
+        // B[k] = (Z[k] - conj Z[N-k]) / (2 i), k = g + G m, m < E / 2
+#pragma unroll
+        for (int m = 0; m < H; m++) {
+            const C2<T> Z = f[c][RW::tau_inv(m)];
+            C2<T> M = shfl2<T>(f[c][RW::tau_inv(E - 1 - m)], partner);
+            if (g == 0) M = f[c][RW::tau_inv((E - m) % E)];
+            data[c * PS + ra + g + G * m] = mk<T>(T(0.5) * (Z.x + M.x), T(0.5) * (Z.y - M.y));
+            data[c * PS + rb + g + G * m] = mk<T>(T(0.5) * (Z.y + M.y), T(-0.5) * (Z.x - M.x));
+        }
+    }
 }
 
 // c2r x pass of NC planes into real planes out ([y][z][x] each), components
