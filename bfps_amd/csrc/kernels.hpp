@@ -65,8 +65,13 @@ struct Cfg {
     static constexpr int NT = G * B;
     // Biot-Savart pass: three source components stay in registers; half the
     // columns (when NT would exceed 512) keep it within a 256-VGPR budget
-    static constexpr int B_BS = (NT > 512 && B > 1) ? B / 2 : B;
-    static constexpr int NT_BS = G * B_BS;
+#ifndef BFPSH_E_BS
+#define BFPSH_E_BS 0
+#endif
+    static constexpr int E_BS = (sizeof(T) == 4 && N == 512 && BFPSH_E_BS > 0 && N % BFPSH_E_BS == 0) ? BFPSH_E_BS : E;
+    static constexpr int G_BS = N / E_BS;
+    static constexpr int B_BS = (G_BS * B > 512 && B > 1) ? B / 2 : B;
+    static constexpr int NT_BS = G_BS * B_BS;
     // last forward pass with the update epilogue (3 components in registers)
     static constexpr int B_Y3 = B;
     // elements per thread of the plain strided pass (measured at 512^3 FP32:
@@ -137,7 +142,7 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
     // box blockIdx.z of a batch of equal boxes bstride elements apart
     w += blockIdx.z * bstride;
     out += blockIdx.z * bstride;
-    constexpr int B = CF::B_BS, NT = CF::NT_BS, E = CF::E, NXH = CF::NXH;
+    constexpr int B = CF::B_BS, NT = CF::NT_BS, E = CF::E_BS, NXH = CF::NXH;
     constexpr size_t PS = size_t(N) * N * NXH;
     constexpr int NF = BsOut<MODE>::NF;
     using F = RegFft<T, N, B, B, NT, +1>;
@@ -163,10 +168,20 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
     constexpr int NU = MODE == 0 ? 3 : NF;
 #pragma unroll 1
     for (int f = 0; f < NU; f++) {
+        // an opaque copy of the group index: the per-element wavenumbers are
+        // recomputed per output field instead of being held across the loop
+        int gq = grp;
+        asm volatile("" : "+v"(gq));
+        // likewise the source values: products of them with the wavenumbers
+        // would otherwise be hoisted (in FP64) out of the loop
+#pragma unroll
+        for (int e = 0; e < E; e++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) asm volatile("" : "+v"(wr[c][e].x), "+v"(wr[c][e].y));
         C2<T> v[E];
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const int n = F::in_row(grp, e);
+            const int n = F::in_row(gq, e);
             const double kz = kwave(n, N, g.dkz);
             const double k2 = kx * kx + kyv * kyv + kz * kz;
             const int cu = MODE == 2 ? f % 3 : f;
@@ -192,10 +207,10 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
                 v[e] = u;
             }
         }
-        F::run(v, lds, tw, grp, b);
+        F::run(v, lds, tw, gq, b);
         C2<T>* o = out + size_t(f) * PS;
 #pragma unroll
-        for (int e = 0; e < E; e++) o[(size_t(ky) * N + F::out_row(grp, e)) * NXH + kxi] = v[e];
+        for (int e = 0; e < E; e++) o[(size_t(ky) * N + F::out_row(gq, e)) * NXH + kxi] = v[e];
     }
     if constexpr (MODE == 0) {
         sfor<0, 3>([&](auto cc) {
