@@ -253,14 +253,23 @@ struct BrickCfg {
     static constexpr int LP = 16;
     static constexpr int NT = 256;
     static constexpr int GROUPS = NT / LP;
-    // tracers per batch (their results are kept in LDS)
-    template <int NC> static constexpr int pb() {
-        return (NC <= 3 && sizeof(T) == 4) ? 8 * GROUPS : 4 * GROUPS;  // two bricks per CU fit in LDS
+    // tracers per batch, sized so that two bricks per CU fit in LDS.  The
+    // velocity + Adams-Bashforth form (MODE 0) keeps a batch's results in
+    // LDS for its epilogue; sampling (MODE 1) writes them straight to the
+    // output and keeps only the tracers' slots, so its FP32 batches hold a
+    // whole brick's tracers (each batch refills the tiles of all components)
+    template <int NC, int MODE> static constexpr int pb() {
+        if (MODE == 1) return sizeof(T) == 4 ? NT : 4 * GROUPS;
+        return (NC <= 3 && sizeof(T) == 4) ? 8 * GROUPS : 4 * GROUPS;
     }
-    template <int NC> static constexpr size_t lds_bytes(int deg) {
-        // res [PB][NC] + frac [PB][3] + weight table (doubles), cells [PB][3]
-        // (ints, padded to doubles), tile
-        return (size_t(pb<NC>()) * (NC + 3) + size_t(W) * (deg + 1) + (size_t(pb<NC>()) * 3 + 1) / 2) * sizeof(double) +
+    template <int NC, int MODE> static constexpr size_t res_doubles() {
+        return MODE == 0 ? size_t(pb<NC, MODE>()) * NC : (size_t(pb<NC, MODE>()) + 1) / 2;
+    }
+    template <int NC, int MODE> static constexpr size_t lds_bytes(int deg) {
+        // results [PB][NC] (or slots [PB], uint32) + frac [PB][3] + weight
+        // table (doubles), cells [PB][3] (ints, padded to doubles), tile
+        constexpr size_t P = pb<NC, MODE>();
+        return (res_doubles<NC, MODE>() + P * 3 + size_t(W) * (deg + 1) + (P * 3 + 1) / 2) * sizeof(double) +
                TILE_BYTES;
     }
 };
@@ -319,12 +328,13 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
         int nbx, int nby, int64_t np, int nsteps, int head, int order, int add_head, double dt,
         const double* __restrict__ tab, int deg, PartGeomD pg) {
     using BC = BrickCfg<T, NB>;
-    constexpr int W = BC::W, LP = BC::LP, NT = BC::NT, PB = BC::template pb<NC>();
+    constexpr int W = BC::W, LP = BC::LP, NT = BC::NT, PB = BC::template pb<NC, MODE>();
     constexpr int WX = BC::WX, WY = BC::WY, WZ = BC::WZ;
     static_assert(PB <= NT, "one tracer per thread in the batch prologue");
     extern __shared__ double smem[];
-    double* res = smem;                          // [PB][NC]
-    double* frac = res + PB * NC;                // [PB][3] in-cell positions
+    double* res = smem;                          // MODE 0: [PB][NC]
+    uint32_t* lslot = reinterpret_cast<uint32_t*>(smem);  // MODE 1: [PB] tracer slots
+    double* frac = res + BC::template res_doubles<NC, MODE>();  // [PB][3] in-cell positions
     double* stab = frac + PB * 3;                // weight table, derivative 0
     int* lcell = reinterpret_cast<int*>(stab + W * (deg + 1));  // [PB][3] cell in brick
     T* tile = reinterpret_cast<T*>(lcell + ((PB * 3 + 1) / 2) * 2);
@@ -348,6 +358,7 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
         double myx[3] = {0.0, 0.0, 0.0};
         if (tid < cnt) {
             myp = perm[bs + tid];
+            if constexpr (MODE == 1) lslot[tid] = uint32_t(myp);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const double sp = pos[myp * 3 + d];
@@ -457,17 +468,19 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
                 acc *= wv[0];
 #pragma unroll
                 for (int o = LP / 2; o > 0; o /= 2) acc += __shfl_xor(acc, o, LP);
-                if (jx == 0) res[q * NC + c] = acc;
+                if (jx == 0) {
+                    if constexpr (MODE == 1)
+                        out[size_t(lslot[q]) * NC + c] = acc;
+                    else
+                        res[q * NC + c] = acc;
+                }
             }
         }
         __syncthreads();
         if (tid < cnt) {
             const int q = tid;
             const int64_t p = myp;
-            if constexpr (MODE == 1) {
-#pragma unroll
-                for (int c = 0; c < NC; c++) out[p * NC + c] = res[q * NC + c];
-            } else {
+            if constexpr (MODE == 0) {
                 static_assert(NC == 3, "the Adams-Bashforth epilogue moves 3 coordinates");
                 double r0[3] = {res[q * NC + 0], res[q * NC + 1], res[q * NC + 2]};
                 double* h0 = rhs + size_t(head) * np * 3 + p * 3;
@@ -588,7 +601,7 @@ int launch_brick(bfpsh_ctx* ctx, const void* f, double* d_out, double dt, int or
     const int n = ctx->n;
     const int nbx = n / BC::KX, nby = n / BC::KY;
     const int64_t nb = int64_t(nbx) * nby * (n / BC::KZ);
-    const size_t lds = BC::template lds_bytes<NC>(ctx->deg);
+    const size_t lds = BC::template lds_bytes<NC, MODE>(ctx->deg);
     auto kern = k_brick<T, NB, NC, MODE>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     const unsigned grid = unsigned((nb + 7) / 8 * 8);
