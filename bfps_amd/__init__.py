@@ -165,6 +165,22 @@ class vorticity_equation:
     def gradient_rspace(self):
         capi.check(self._h, capi.load().bfpsh_gradient_rspace(self._h), "gradient_rspace")
 
+    def compute_pressure(self, pressure):
+        """vorticity_equation::compute_pressure
+        (bfps/cpp/vorticity_equation.cpp:441-520) of the velocity of
+        cvorticity into a 1-component field (spectral)."""
+        capi.check(self.handle, capi.load().bfpsh_compute_pressure(self.handle, pressure.id), "compute_pressure")
+        pressure.real_space_representation = False
+
+    def compute_Lagrangian_acceleration(self, acceleration):
+        """vorticity_equation::compute_Lagrangian_acceleration
+        (bfps/cpp/vorticity_equation.cpp:522-578) into a 3-component field
+        (spectral)."""
+        self._push_physics()
+        capi.check(self.handle, capi.load().bfpsh_compute_lagrangian_acceleration(self.handle, acceleration.id),
+                   "compute_lagrangian_acceleration")
+        acceleration.real_space_representation = False
+
     def get_rfield(self, which=capi.RVELOCITY):
         nc = 3 if which == capi.RVELOCITY else 9
         out = np.zeros((self.nz, self.ny, self.nx, nc), dtype=self.rdtype)
@@ -306,6 +322,45 @@ class field:
         capi.check(self.fs.handle, capi.load().bfpsh_field_copy_from(self.fs.handle, self.id, sel),
                    "field_copy_from")
         self.real_space_representation = False
+
+    def normalize(self):
+        """field::normalize (bfps/cpp/field.cpp:891-898): data /= npoints."""
+        capi.check(self.fs.handle, capi.load().bfpsh_field_normalize(self.fs.handle, self.id), "field_normalize")
+
+    def compute_rspace_stats(self, max_estimate, nbins=256):
+        """field::compute_rspace_stats (bfps/cpp/field.cpp:661-850) without the
+        HDF5 writes: (moments [10][nvals], histograms [nbins][nvals]) with
+        nvals = 4 for a vector field (the magnitude last), else ncomp.
+        max_estimate holds nvals values (the reference's max_estimate_vector)."""
+        nvals = 4 if self.ncomp == 3 else self.ncomp
+        me = np.ascontiguousarray(max_estimate, dtype=np.float64).reshape(nvals)
+        mom = np.zeros((10, nvals))
+        hist = np.zeros((nbins, nvals), dtype=np.int64)
+        capi.check(self.fs.handle, capi.load().bfpsh_field_rspace_stats(
+            self.fs.handle, self.id, int(nbins), me.ctypes.data_as(capi._DP), mom.ctypes.data_as(capi._DP),
+            hist.ctypes.data_as(capi._LP)), "field_rspace_stats")
+        return mom, hist
+
+    def compute_stats(self, max_estimate, nbins=256):
+        """field::compute_stats (bfps/cpp/field.cpp:980-1035) of a vector
+        field, returning what the reference writes: {"moments", "histograms",
+        "cospectrum"}.  max_estimate is the scalar of the reference's call;
+        the magnitude's range is sqrt(3) times it.  A real-space field is
+        left spectral and normalised, a spectral one real, as the reference
+        leaves them."""
+        assert self.ncomp == 3
+        me = [max_estimate] * 3 + [max_estimate * np.sqrt(3.0)]
+        out = {}
+        if self.real_space_representation:
+            out["moments"], out["histograms"] = self.compute_rspace_stats(me, nbins)
+            self.dft()
+            self.normalize()
+            out["cospectrum"] = self.cospectrum(self)
+        else:
+            out["cospectrum"] = self.cospectrum(self)
+            self.ift()
+            out["moments"], out["histograms"] = self.compute_rspace_stats(me, nbins)
+        return out
 
     def cospectrum(self, other):
         """kspace::cospectrum of two spectral 3-component fields, [shell][i][j]."""

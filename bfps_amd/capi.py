@@ -46,6 +46,10 @@ SIGNATURES = {
     "bfpsh_field_get": (_I, [_P, _I, _I, _P]),
     "bfpsh_field_copy_from": (_I, [_P, _I, _I]),
     "bfpsh_field_cospectrum": (_I, [_P, _I, _I, _DP]),
+    "bfpsh_field_normalize": (_I, [_P, _I]),
+    "bfpsh_field_rspace_stats": (_I, [_P, _I, _I, _DP, _DP, _LP]),
+    "bfpsh_compute_pressure": (_I, [_P, _I]),
+    "bfpsh_compute_lagrangian_acceleration": (_I, [_P, _I]),
     "bfpsh_particles_create": (_I, [_P, _L, _I, _I, _I, _I]),
     "bfpsh_particles_put": (_I, [_P, _DP, _DP, _LP]),
     "bfpsh_particles_get": (_I, [_P, _DP, _DP, _LP, _IP]),

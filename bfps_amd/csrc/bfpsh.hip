@@ -460,6 +460,38 @@ int bfpsh_field_cospectrum(bfpsh_ctx* ctx, int id_a, int id_b, double* out) {
     return PREC_CALL(cospectrum, ctx, ctx->fields[id_a], ctx->fields[id_b], out);
 }
 
+int bfpsh_field_normalize(bfpsh_ctx* ctx, int id) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id)) return fail(ctx, BFPSH_EINVAL, "bad field id");
+    return PREC_CALL(normalize, ctx, ctx->fields[id], ctx->field_nc[id]);
+}
+
+int bfpsh_field_rspace_stats(bfpsh_ctx* ctx, int id, int nbins, const double* max_estimate, double* moments,
+                             int64_t* hist) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id)) return fail(ctx, BFPSH_EINVAL, "bad field id");
+    if (!max_estimate || !moments || !hist) return fail(ctx, BFPSH_EINVAL, "null argument");
+    if (nbins < 1 || nbins > 4096) return fail(ctx, BFPSH_EINVAL, "nbins must be in [1, 4096]");
+    if (!ctx->field_real[id]) return fail(ctx, BFPSH_ESTATE, "field is not in real space (call ift first)");
+    for (int i = 0; i < (ctx->field_nc[id] == 3 ? 4 : ctx->field_nc[id]); i++)
+        if (!(max_estimate[i] > 0)) return fail(ctx, BFPSH_EINVAL, "max_estimate values must be positive");
+    return PREC_CALL(rspace_stats, ctx, ctx->fields[id], ctx->field_nc[id], nbins, max_estimate, moments, hist);
+}
+
+int bfpsh_compute_pressure(bfpsh_ctx* ctx, int id) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id) || ctx->field_nc[id] != 1) return fail(ctx, BFPSH_EINVAL, "needs a 1-component field");
+    ctx->field_real[id] = 0;
+    return PREC_CALL(pressure, ctx, ctx->fields[id]);
+}
+
+int bfpsh_compute_lagrangian_acceleration(bfpsh_ctx* ctx, int id) {
+    CTX_CHECK();
+    if (!field_ok(ctx, id) || ctx->field_nc[id] != 3) return fail(ctx, BFPSH_EINVAL, "needs a 3-component field");
+    ctx->field_real[id] = 0;
+    return PREC_CALL(lagrangian_acceleration, ctx, ctx->fields[id]);
+}
+
 // Slots in global-index order, on the host (order) and as a rank per slot
 // on the device.  The storage order of tracers never changes (sorting for
 // locality only permutes an index array), so this is set once per put.

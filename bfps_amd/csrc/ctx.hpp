@@ -213,6 +213,10 @@ struct OpsTable {
     int (*field_ift)(bfpsh_ctx*, void*, int);
     int (*field_dft)(bfpsh_ctx*, void*, int);
     int (*batch_step)(bfpsh_ctx*, const BatchDesc&);
+    int (*pressure)(bfpsh_ctx*, void*);
+    int (*lagrangian_acceleration)(bfpsh_ctx*, void*);
+    int (*normalize)(bfpsh_ctx*, void*, int);
+    int (*rspace_stats)(bfpsh_ctx*, const void*, int, int, const double*, double*, int64_t*);
 };
 
 int create_ctx(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, double dkz, int precision,

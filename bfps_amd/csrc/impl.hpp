@@ -2,6 +2,7 @@
 // (precision, N) pair.  Instantiated once per pair by impl_inst.hip.
 #pragma once
 #include "ctx.hpp"
+#include "stats.hpp"
 
 // ------------------------------------------------------------------ per-(T, N) launchers
 template <typename T, int N>
@@ -238,6 +239,93 @@ struct Impl {
         const size_t ps = size_t(N) * N * NXH;
         LAUNCH("biot_savart", (k_biot_savart<T><<<(ps + 255) / 256, 256, 0, ctx->stream>>>((const CT*)ctx->v0, (CT*)ctx->work, geom(ctx))));
         return symmetrize(ctx, ctx->work, 3);
+    }
+
+    // compute_pressure (vorticity_equation.cpp:441-520) of the velocity of
+    // cvorticity into the spectral plane p: the real velocity is made here
+    // (the reference takes `u` in real space), its products go through work
+    // planes 0..2
+    static int pressure(bfpsh_ctx* ctx, CT* p) {
+        int r = velocity_rspace(ctx);
+        if (r) return r;
+        const size_t n3 = size_t(N) * N * N, ps = size_t(N) * N * NXH;
+        const unsigned gr = unsigned(std::min<size_t>((n3 + 255) / 256, 65536));
+        for (int pass = 0; pass < 2; pass++) {
+            LAUNCH("pressure_uu", (k_uu<T><<<gr, 256, 0, ctx->stream>>>((const T*)ctx->ru, (T*)ctx->work, n3, pass)));
+            r = field_dft_nc<3>(ctx, ctx->work);
+            if (r) return r;
+            LAUNCH("pressure", (k_pressure<T><<<unsigned((ps + 255) / 256), 256, 0, ctx->stream>>>(
+                                   (const CT*)ctx->work, p, geom(ctx), pass)));
+        }
+        return BFPSH_OK;
+    }
+
+    // compute_Lagrangian_acceleration (vorticity_equation.cpp:522-578) into
+    // the 3 spectral planes acc; the pressure lives in work plane 3
+    static int lagrangian_acceleration(bfpsh_ctx* ctx, CT* acc) {
+        int r = ensure_work(ctx, 6);
+        if (r) return r;
+        const size_t ps = size_t(N) * N * NXH;
+        CT* p = (CT*)ctx->work + 3 * ps;
+        r = pressure(ctx, p);
+        if (r) return r;
+        r = spectral_velocity(ctx);
+        if (r) return r;
+        const StepParams& sp = ctx->sp;
+        LAUNCH("lagrangian_acceleration", (k_lagr_acc<T><<<unsigned((ps + 255) / 256), 256, 0, ctx->stream>>>(
+                                              (const CT*)ctx->work, p, acc, geom(ctx), sp.nu, sp.forcing == 1 ? 1 : 0,
+                                              sp.famp, sp.fk0, sp.fk1)));
+        return BFPSH_OK;
+    }
+
+    static int pressure_v(bfpsh_ctx* ctx, void* p) { return pressure(ctx, (CT*)p); }
+    static int lagrangian_acceleration_v(bfpsh_ctx* ctx, void* a) { return lagrangian_acceleration(ctx, (CT*)a); }
+
+    // field::normalize of an NC-component field (all its values)
+    static int normalize(bfpsh_ctx* ctx, void* data, int nc) {
+        const size_t cnt = size_t(nc) * N * N * N;
+        const unsigned gr = unsigned(std::min<size_t>((cnt + 255) / 256, 65536));
+        LAUNCH("normalize", (k_div<T><<<gr, 256, 0, ctx->stream>>>((T*)data, cnt, T(double(N) * N * N))));
+        return BFPSH_OK;
+    }
+
+    // field::compute_rspace_stats (field.cpp:661-850) of a real NC-component
+    // field: moments [10][nvals], hist [nbins][nvals]
+    static int rspace_stats(bfpsh_ctx* ctx, const void* data, int nc, int nbins, const double* maxest,
+                            double* moments, int64_t* hist) {
+        const size_t n3 = size_t(N) * N * N;
+        const int nvals = nc == 3 ? 4 : nc, nv = nc == 3 ? 4 : 1, groups = nc == 3 ? 1 : nc;
+        const int nblk = int(std::min<size_t>(1024, (n3 + 255) / 256));
+        const size_t part_n = size_t(groups) * nblk * 10 * nv;
+        double *d_me = nullptr, *d_part = nullptr, *d_mom = nullptr;
+        unsigned long long* d_hist = nullptr;
+        HIPCHK(hipMallocAsync((void**)&d_me, nvals * sizeof(double), ctx->stream));
+        HIPCHK(hipMallocAsync((void**)&d_part, part_n * sizeof(double), ctx->stream));
+        HIPCHK(hipMallocAsync((void**)&d_mom, 10 * nvals * sizeof(double), ctx->stream));
+        HIPCHK(hipMallocAsync((void**)&d_hist, size_t(nbins) * nvals * sizeof(unsigned long long), ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_me, maxest, nvals * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemsetAsync(d_hist, 0, size_t(nbins) * nvals * sizeof(unsigned long long), ctx->stream));
+        const size_t lds = size_t(nbins) * nv * sizeof(unsigned int);
+        if (nc == 3) {
+            LAUNCH("rspace_stats", (k_rstats<T, 4><<<dim3(nblk, 1), 256, lds, ctx->stream>>>(
+                                       (const T*)data, n3, nvals, nbins, d_me, d_part, d_hist)));
+            LAUNCH("rspace_stats_final", (k_rstats_final<4><<<1, 64, 0, ctx->stream>>>(d_part, nblk, nvals,
+                                                                                       double(n3), d_mom)));
+        } else {
+            LAUNCH("rspace_stats", (k_rstats<T, 1><<<dim3(nblk, nc), 256, lds, ctx->stream>>>(
+                                       (const T*)data, n3, nvals, nbins, d_me, d_part, d_hist)));
+            LAUNCH("rspace_stats_final", (k_rstats_final<1><<<nc, 64, 0, ctx->stream>>>(d_part, nblk, nvals,
+                                                                                        double(n3), d_mom)));
+        }
+        HIPCHK(hipMemcpyAsync(moments, d_mom, 10 * nvals * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(hist, d_hist, size_t(nbins) * nvals * sizeof(int64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipFreeAsync(d_me, ctx->stream));
+        HIPCHK(hipFreeAsync(d_part, ctx->stream));
+        HIPCHK(hipFreeAsync(d_mom, ctx->stream));
+        HIPCHK(hipFreeAsync(d_hist, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return BFPSH_OK;
     }
 
     static int cospectrum(bfpsh_ctx* ctx, const void* a, const void* b, double* out) {

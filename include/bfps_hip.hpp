@@ -149,13 +149,35 @@ class field {
         detail::check(box_->ctx, bfpsh_field_ift(box_->ctx, id_), "ift");
         real_space_representation = true;
     }
-    /* field::normalize (field.cpp:892-898): divide by npoints. */
+    /* field::normalize (field.cpp:891-898): divide by npoints. */
     void normalize() {
+        if (role_ == detail::role::standalone) {
+            box_->flush();
+            detail::check(box_->ctx, bfpsh_field_normalize(box_->ctx, id_), "normalize");
+            return;
+        }
         const bool r = real_space_representation;
         download();
         for (auto& v : host_) v /= rnumber(npoints);
         real_space_representation = r;
         upload();
+    }
+    /* field::compute_rspace_stats (field.cpp:661-850) without the HDF5
+     * writes: moments [10][nvals] and hist [nbins][nvals] (nvals = 4 for a
+     * THREE field, the magnitude last, else ncomp); max_estimate holds nvals
+     * values (the reference's max_estimate_vector). */
+    void compute_rspace_stats(const std::vector<double>& max_estimate, int nbins, std::vector<double>& moments,
+                              std::vector<int64_t>& hist) {
+        if (role_ != detail::role::standalone) throw error("rspace stats of a solver field: copy it first");
+        if (!real_space_representation) throw error("compute_rspace_stats needs the real-space representation");
+        const size_t nvals = ncomp(fc) == 3 ? 4 : ncomp(fc);
+        if (max_estimate.size() != nvals) throw error("max_estimate needs one value per statistic");
+        box_->flush();
+        moments.assign(10 * nvals, 0.0);
+        hist.assign(size_t(nbins) * nvals, 0);
+        detail::check(box_->ctx, bfpsh_field_rspace_stats(box_->ctx, id_, nbins, max_estimate.data(), moments.data(),
+                                                          hist.data()),
+                      "rspace_stats");
     }
     /* field::symmetrize (field.cpp:900-978). */
     void symmetrize() {
@@ -422,6 +444,26 @@ class vorticity_equation {
         if (!data) throw error("io_checkpoint(read) needs the vorticity data (no HDF5 in this backend)");
         *cvorticity = data;
         detail::check(box_.ctx, bfpsh_load_projection(box_.ctx), "io_checkpoint");
+    }
+
+    /* compute_pressure (vorticity_equation.cpp:441-520) of the velocity of
+     * cvorticity into a standalone ONE field (spectral). */
+    void compute_pressure(field<rnumber, be, ONE>* pressure) {
+        push_physics();
+        box_.flush();
+        if (pressure->handle() < 0) throw error("compute_pressure needs a standalone field");
+        detail::check(box_.ctx, bfpsh_compute_pressure(box_.ctx, pressure->handle()), "compute_pressure");
+        pressure->real_space_representation = false;
+    }
+    /* compute_Lagrangian_acceleration (vorticity_equation.cpp:522-578) into a
+     * standalone THREE field (spectral). */
+    void compute_Lagrangian_acceleration(field<rnumber, be, THREE>* acceleration) {
+        push_physics();
+        box_.flush();
+        if (acceleration->handle() < 0) throw error("compute_Lagrangian_acceleration needs a standalone field");
+        detail::check(box_.ctx, bfpsh_compute_lagrangian_acceleration(box_.ctx, acceleration->handle()),
+                      "compute_lagrangian_acceleration");
+        acceleration->real_space_representation = false;
     }
 
     /* compute_gradient(kk, cvelocity, grad) + grad->ift() into the

@@ -129,6 +129,27 @@ int bfpsh_field_copy_from(bfpsh_ctx* ctx, int id, int which);
 /* kspace::cospectrum (bfps/cpp/kspace.cpp:521-598) of two 3-component
  * standalone fields (spectral); out nshells*9 doubles. */
 int bfpsh_field_cospectrum(bfpsh_ctx* ctx, int id_a, int id_b, double* out);
+/* Replaces field::normalize (bfps/cpp/field.cpp:891-898): every value /=
+ * nx*ny*nz, in the field precision. */
+int bfpsh_field_normalize(bfpsh_ctx* ctx, int id);
+/* Replaces field::compute_rspace_stats (bfps/cpp/field.cpp:661-850) of a
+ * field in real space, without the HDF5 writes: moments [10][nvals] (row 0
+ * min, row 9 max, both starting at 0 -- the magnitude's min at its estimate
+ * --, rows 1..8 the means of value^1..value^8) and hist [nbins][nvals]
+ * counts; nvals = 4 for a 3-component field (value 3 = magnitude), else
+ * ncomp; max_estimate[nvals] sets the histogram ranges (-M..M, the
+ * magnitude's 0..M).  1 <= nbins <= 4096. */
+int bfpsh_field_rspace_stats(bfpsh_ctx* ctx, int id, int nbins, const double* max_estimate, double* moments,
+                             int64_t* hist);
+/* Replaces vorticity_equation::compute_pressure
+ * (bfps/cpp/vorticity_equation.cpp:441-520) into a 1-component field
+ * (spectral); the real-space velocity it takes is made here from
+ * cvorticity. */
+int bfpsh_compute_pressure(bfpsh_ctx* ctx, int id);
+/* Replaces vorticity_equation::compute_Lagrangian_acceleration
+ * (bfps/cpp/vorticity_equation.cpp:522-578) into a 3-component field
+ * (spectral). */
+int bfpsh_compute_lagrangian_acceleration(bfpsh_ctx* ctx, int id);
 
 /* Replaces particles_system_builder(...) (bfps/cpp/particles/particles_system_builder.hpp:238-257)
  * for one rank: np tracers, interpolation neighbours n in [1,10] and

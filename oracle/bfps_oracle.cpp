@@ -25,6 +25,9 @@
 //   * interpolation weights beta_Lagrange_n{n} / beta_n{n}_m{m}, rebuilt at
 //     run time from their definition (Lagrange basis, two-point Hermite
 //     matching of Lagrange derivatives) by numeric linear solves.
+//   * statistics either side of the path: compute_pressure and
+//     compute_Lagrangian_acceleration (vorticity_equation.cpp:441-578),
+//     field::compute_rspace_stats moments and histograms (field.cpp:661-850)
 //
 // Arithmetic types follow the reference: field data in rnumber (float or
 // double), wavenumbers / k^2 / factors in double, particle data in double.
@@ -51,6 +54,12 @@
 //     k1 + k2 -> k3 against its Fourier-coefficient closed form, with k3 where
 //     the SMOOTH filter is between 0.05 and 0.95, and across the TWO_THIRDS
 //     cut (test_oracle_closed_forms.py)
+//   * pressure and Lagrangian acceleration: the ABC flow, p = -|u|^2/2 +
+//     const and a = grad |u|^2/2 - nu u, in FP64 and FP32; real-space
+//     statistics: a constant field (min/max starting at 0), sin x (moments =
+//     central binomial coefficients / 2^n, histogram of the eight values of
+//     an 8-point period counted by hand) and a unit-magnitude field
+//     (test_oracle_closed_forms.py)
 //   * not reproducible bit for bit (last-bit effects, covered by
 //     tolerances): the reference's FFTW plan-dependent and -ffast-math
 //     rounding.
@@ -431,6 +440,8 @@ struct SolverBase {
     virtual void compute_vorticity_from_velocity() = 0;
     virtual void gradient_rspace() = 0;
     virtual void cospectrum(int a, int b, double* out) = 0;
+    virtual void pressure_out(double* out) = 0;
+    virtual void acceleration_out(double* out) = 0;
     virtual const KSpace& kspace() const = 0;
     // sample real-space field `which` (1 = velocity, 2 = gradient) at positions
     virtual void interpolate(int which, const double* pos, long np, int n, const double* tab,
@@ -692,6 +703,83 @@ struct Solver : SolverBase {
         });
         fft9.c2r(grad.data.data());
     }
+    // compute_pressure (vorticity_equation.cpp:441-520) of the real-space
+    // velocity in rvelocity (the reference takes `u` in real space):
+    // p = -k_i k_j F[u_i u_j] / (npoints k^2), products dealiased, diagonal
+    // terms first, then the off-diagonal ones
+    void compute_pressure(Field<R>& p) {
+        Field<R> prod(kk.nx, kk.ny, kk.nz, 3);
+        const int nx = kk.nx;
+        const double npoints = double(size_t(kk.nx) * kk.ny * kk.nz);
+        for (int pass = 0; pass < 2; pass++) {
+            prod.zero();
+            #pragma omp parallel for schedule(static)
+            for (long zy = 0; zy < long(kk.nz) * kk.ny; zy++)
+                for (int x = 0; x < nx; x++) {
+                    const size_t ri = size_t(zy) * (nx + 2) + x;
+                    for (int cc = 0; cc < 3; cc++)
+                        prod.rv(ri, cc) = rvelocity.rv(ri, cc) * rvelocity.rv(ri, pass == 0 ? cc : (cc + 1) % 3);
+                }
+            fft3.r2c(prod.data.data());
+            dealias(prod);
+            kk.cloop_k2([&](size_t ci, int x, int y, int z, double k2) {
+                const double kx = kk.kx[x], ky = kk.ky[y], kz = kk.kz[z];
+                if (k2 <= kk.kM2 && k2 > 0) {
+                    for (int i = 0; i < 2; i++) {
+                        if (pass == 0) {
+                            p.cv(ci, 0, i) = R(-(kx * kx * prod.cv(ci, 0, i) + ky * ky * prod.cv(ci, 1, i) +
+                                                 kz * kz * prod.cv(ci, 2, i)));
+                        } else {
+                            p.cv(ci, 0, i) = R(p.cv(ci, 0, i) - 2 * (kx * ky * prod.cv(ci, 0, i) +
+                                                                     ky * kz * prod.cv(ci, 1, i) +
+                                                                     kz * kx * prod.cv(ci, 2, i)));
+                            p.cv(ci, 0, i) = R(p.cv(ci, 0, i) / (npoints * k2));
+                        }
+                    }
+                } else if (pass == 0) {
+                    p.cv(ci, 0, 0) = 0;
+                    p.cv(ci, 0, 1) = 0;
+                }
+            });
+        }
+    }
+    // compute_Lagrangian_acceleration (vorticity_equation.cpp:522-578):
+    // a = -nu k^2 u (+ famplitude u in the band for linear forcing) - i k p
+    void lagrangian_acceleration(Field<R>& acc) {
+        velocity_rspace();
+        Field<R> p(kk.nx, kk.ny, kk.nz, 1);
+        compute_pressure(p);
+        compute_velocity(cvorticity);
+        acc.zero();
+        Field<R>& u = cvelocity;
+        kk.cloop_k2([&](size_t ci, int x, int y, int z, double k2) {
+            if (k2 <= kk.kM2) {
+                const double kv[3] = {kk.kx[x], kk.ky[y], kk.kz[z]};
+                for (int c = 0; c < 3; c++)
+                    for (int i = 0; i < 2; i++) acc.cv(ci, c, i) = R(-nu * k2 * u.cv(ci, c, i));
+                if (forcing == F_LINEAR) {
+                    const double kn = std::sqrt(k2);
+                    if (fk0 <= kn && fk1 >= kn)
+                        for (int c = 0; c < 3; c++)
+                            for (int i = 0; i < 2; i++) acc.cv(ci, c, i) = R(acc.cv(ci, c, i) + famplitude * u.cv(ci, c, i));
+                }
+                for (int c = 0; c < 3; c++) acc.cv(ci, c, 0) = R(acc.cv(ci, c, 0) + kv[c] * p.cv(ci, 0, 1));
+                for (int c = 0; c < 3; c++) acc.cv(ci, c, 1) = R(acc.cv(ci, c, 1) - kv[c] * p.cv(ci, 0, 0));
+            }
+        });
+    }
+    void pressure_out(double* out) override {
+        velocity_rspace();
+        Field<R> p(kk.nx, kk.ny, kk.nz, 1);
+        compute_pressure(p);
+        for (size_t i = 0; i < kk.csize() * 2; i++) out[i] = double(p.data[i]);
+    }
+    void acceleration_out(double* out) override {
+        Field<R> a(kk.nx, kk.ny, kk.nz, 3);
+        lagrangian_acceleration(a);
+        for (size_t i = 0; i < kk.csize() * 6; i++) out[i] = double(a.data[i]);
+    }
+
     // kspace::cospectrum (kspace.cpp:521-598) of two 3-component fields
     void cospectrum(int a, int b, double* out) override {
         Field<R>& fa = sel(a);
@@ -930,6 +1018,50 @@ struct Particles {
 
 using namespace orc;
 
+// ------------------------------------------------------------------------ //
+// field::compute_rspace_stats (field.cpp:661-850), one rank: data [npts][nc]
+// (the reference's real values), nvals = 4 for 3 components (the magnitude
+// is value 3), else nc.  moments [10][nvals]: 0 = min, 9 = max (both start
+// at 0, the magnitude's min at its estimate), 1..8 = mean of value^n;
+// hist [nbins][nvals]: bins of width 2 max_estimate / nbins from
+// -max_estimate (the magnitude's from 0, half the width)
+// ------------------------------------------------------------------------ //
+static void rspace_stats(const double* d, long npts, int nc, int nbins, const double* maxest, double* moments,
+                         int64_t* hist) {
+    const int nm = 10;
+    const int nvals = nc == 3 ? 4 : nc;
+    std::vector<double> mom(size_t(nm) * nvals, 0.0), binsize(nvals), val(nvals), pw(nvals);
+    if (nvals == 4) mom[3] = maxest[3];
+    std::fill(hist, hist + size_t(nbins) * nvals, int64_t(0));
+    for (int i = 0; i < nvals; i++) binsize[i] = 2 * maxest[i] / nbins;
+    for (long p = 0; p < npts; p++) {
+        std::fill(pw.begin(), pw.end(), 1.0);
+        if (nvals == 4) val[3] = 0.0;
+        for (int i = 0; i < nc; i++) {
+            val[i] = d[size_t(p) * nc + i];
+            if (nvals == 4) val[3] += val[i] * val[i];
+        }
+        if (nvals == 4) {
+            val[3] = std::sqrt(val[3]);
+            if (val[3] < mom[3]) mom[3] = val[3];
+            if (val[3] > mom[9 * nvals + 3]) mom[9 * nvals + 3] = val[3];
+            const int bin = int(std::floor(val[3] * 2 / binsize[3]));
+            if (bin >= 0 && bin < nbins) hist[size_t(bin) * nvals + 3]++;
+        }
+        for (int i = 0; i < nc; i++) {
+            if (val[i] < mom[i]) mom[i] = val[i];
+            if (val[i] > mom[(nm - 1) * nvals + i]) mom[(nm - 1) * nvals + i] = val[i];
+            const int bin = int(std::floor((val[i] + maxest[i]) / binsize[i]));
+            if (bin >= 0 && bin < nbins) hist[size_t(bin) * nvals + i]++;
+        }
+        for (int n = 1; n < nm - 1; n++)
+            for (int i = 0; i < nvals; i++) mom[size_t(n) * nvals + i] += (pw[i] = val[i] * pw[i]);
+    }
+    for (int n = 1; n < nm - 1; n++)
+        for (int i = 0; i < nvals; i++) mom[size_t(n) * nvals + i] /= double(npts);
+    std::copy(mom.begin(), mom.end(), moments);
+}
+
 extern "C" {
 
 void* orc_new(int nx, int ny, int nz, double dkx, double dky, double dkz, int fp32, int smooth) {
@@ -962,6 +1094,12 @@ void orc_compute_vorticity(void* h) { static_cast<SolverBase*>(h)->compute_vorti
 void orc_velocity_rspace(void* h) { static_cast<SolverBase*>(h)->velocity_rspace(); }
 void orc_gradient_rspace(void* h) { static_cast<SolverBase*>(h)->gradient_rspace(); }
 void orc_cospectrum(void* h, int a, int b, double* out) { static_cast<SolverBase*>(h)->cospectrum(a, b, out); }
+void orc_pressure(void* h, double* out) { static_cast<SolverBase*>(h)->pressure_out(out); }
+void orc_lagrangian_acceleration(void* h, double* out) { static_cast<SolverBase*>(h)->acceleration_out(out); }
+void orc_rspace_stats(const double* data, long npts, int nc, int nbins, const double* maxest, double* moments,
+                      int64_t* hist) {
+    rspace_stats(data, npts, nc, nbins, maxest, moments, hist);
+}
 
 // 3D FFT on unpadded real [nz][ny][nx][nc] <-> complex [ny][nz][nx/2+1][nc]
 void orc_fft3d(int nx, int ny, int nz, int nc, int fp32, int inverse, const double* in, double* out) {

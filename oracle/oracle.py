@@ -61,6 +61,9 @@ def lib():
             getattr(L, name).argtypes = [P]
         L.orc_step.argtypes = [P, D]
         L.orc_cospectrum.argtypes = [P, I, I, dp]
+        L.orc_pressure.argtypes = [P, dp]
+        L.orc_lagrangian_acceleration.argtypes = [P, dp]
+        L.orc_rspace_stats.argtypes = [dp, ctypes.c_long, I, I, dp, dp, ctypes.POINTER(ctypes.c_int64)]
         L.orc_fft3d.argtypes = [I, I, I, I, I, I, dp, dp]
         L.orc_symmetrize.argtypes = [I, I, I, I, dp]
         L.orc_interp_table.argtypes = [I, I, dp]
@@ -185,6 +188,18 @@ class Solver:
         lib().orc_cospectrum(self.h, FIELD[a], FIELD[b], _dp(out))
         return out
 
+    def pressure(self):
+        """compute_pressure of the velocity of cvorticity (spectral, [ny][nz][nx/2+1])."""
+        out = np.zeros(self.cshape(1)[:3], dtype=np.complex128)
+        lib().orc_pressure(self.h, _dp(out.view(np.float64)))
+        return out
+
+    def lagrangian_acceleration(self):
+        """compute_Lagrangian_acceleration (spectral, [ny][nz][nx/2+1][3])."""
+        out = np.zeros(self.cshape(3), dtype=np.complex128)
+        lib().orc_lagrangian_acceleration(self.h, _dp(out.view(np.float64)))
+        return out
+
     def kshell(self):
         ks = np.zeros(self.nshells)
         ns = np.zeros(self.nshells, dtype=np.int64)
@@ -227,6 +242,20 @@ class Particles:
         out = np.zeros((self.np, nc))
         lib().orc_part_sample(self.h, 1 if which == "velocity" else 2, _dp(out))
         return out
+
+
+def rspace_stats(data, nbins, max_estimate):
+    """field::compute_rspace_stats of real data [..., nc] (nc = 1, 3 or 9):
+    moments [10][nvals], histograms [nbins][nvals] (nvals = 4 for nc = 3)."""
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    nc = d.shape[-1]
+    nvals = 4 if nc == 3 else nc
+    me = np.ascontiguousarray(max_estimate, dtype=np.float64).reshape(nvals)
+    mom = np.zeros((10, nvals))
+    hist = np.zeros((nbins, nvals), dtype=np.int64)
+    lib().orc_rspace_stats(_dp(d), ctypes.c_long(d.size // nc), nc, nbins, _dp(me), _dp(mom),
+                           hist.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    return mom, hist
 
 
 def nsve_particles_step(solver, parts, dt):

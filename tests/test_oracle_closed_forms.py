@@ -363,3 +363,96 @@ def test_fixed_energy_injection_single_wave(oracle, nu):
     # close to the exact energy, and converging at third order in dt
     assert errs[0] < 1e-4, errs
     assert 8 * 0.7 < errs[0] / errs[1] < 8 * 1.4, errs
+
+
+# ---------------------------------------------------------------- statistics
+# compute_pressure / compute_Lagrangian_acceleration
+# (vorticity_equation.cpp:441-578) on the ABC flow: u x omega = 0, so
+# (u . grad) u = grad |u|^2 / 2 and the pressure Poisson equation gives
+# p = -|u|^2 / 2 + const, i.e. p = -(cos y sin z + cos z sin x + cos x sin y)
+# with the mean removed; the acceleration is -grad p + nu lap u =
+# grad |u|^2 / 2 - nu u (the linear forcing band 2..4 misses |k| = 1).
+def abc_pressure(x, y, z):
+    return -(np.cos(y) * np.sin(z) + np.cos(z) * np.sin(x) + np.cos(x) * np.sin(y))
+
+
+def abc_acceleration(x, y, z, nu):
+    g = np.stack([np.cos(z) * np.cos(x) - np.sin(x) * np.sin(y),
+                  np.cos(x) * np.cos(y) - np.sin(y) * np.sin(z),
+                  np.cos(y) * np.cos(z) - np.sin(z) * np.sin(x)], axis=-1)
+    return g - nu * abc_u(x, y, z)
+
+
+def real_of(spec, n):
+    """true Fourier coefficients [ky][kz][kx][c] -> real [z][y][x][c]."""
+    return np.fft.irfftn(spec.transpose(1, 0, 2, 3), s=(n, n, n), axes=(0, 1, 2)) * n ** 3
+
+
+@pytest.mark.parametrize("precision,dealias", [("double", "SMOOTH"), ("double", "TWO_THIRDS"), ("single", "SMOOTH")])
+def test_abc_pressure_and_lagrangian_acceleration(oracle, precision, dealias):
+    n, nu = 24, 0.03
+    s = oracle.Solver(n, precision=precision, dealias=dealias)
+    s.set_physics(nu, "linear", 0.5, 2.0, 4.0)
+    s.set_cfield("vorticity", abc_vorticity(n))
+    x, y, z = grid(n)
+    tol = 1e-13 if precision == "double" else 3e-6
+    p = real_of(s.pressure()[..., None], n)[..., 0]
+    assert rel(p, abc_pressure(x, y, z)) < tol
+    a = real_of(s.lagrangian_acceleration(), n)
+    assert rel(a, abc_acceleration(x, y, z, nu)) < tol
+    # the forcing term enters only for the linear type and only in its band:
+    # with the band moved onto |k| = 1 the acceleration gains famplitude u
+    s.set_physics(nu, "linear", 0.5, 0.5, 1.5)
+    a2 = real_of(s.lagrangian_acceleration(), n)
+    assert rel(a2, abc_acceleration(x, y, z, nu) + 0.5 * abc_u(x, y, z)) < tol
+    s.set_physics(nu, "Kolmogorov", 0.5, 0.5, 1.5)
+    a3 = real_of(s.lagrangian_acceleration(), n)
+    assert rel(a3, abc_acceleration(x, y, z, nu)) < tol
+
+
+# field::compute_rspace_stats (field.cpp:661-850): moments [10][nvals] with
+# min (row 0) and max (row 9) starting from 0 (the magnitude's min from its
+# estimate), rows 1..8 = mean value^n; histograms of width 2 M / nbins from
+# -M (the magnitude's from 0 with half that width).
+def test_rspace_stats_constant_field(oracle):
+    n, c = 8, -0.25
+    d = np.full((n, n, n, 1), c)
+    mom, hist = oracle.rspace_stats(d, 16, [1.0])
+    assert mom[0, 0] == c and mom[9, 0] == 0.0          # max starts at 0
+    assert np.allclose(mom[1:9, 0], [c ** k for k in range(1, 9)], rtol=1e-15, atol=0)
+    want = np.zeros((16, 1), dtype=np.int64)
+    want[int(np.floor((c + 1.0) / (2.0 / 16))), 0] = n ** 3
+    assert np.array_equal(hist, want)
+
+
+def test_rspace_stats_sine(oracle):
+    # u_x = sin x on an 8-point period: the values are 0, s, 1, s, 0, -s,
+    # -1, -s (s = 1/sqrt 2); mean sin^k over a full period is
+    # C(k, k/2) / 2^k for even k < 8 and 0 for odd k
+    n = 8
+    x, y, z = grid(n)
+    d = np.sin(x)[..., None]
+    mom, hist = oracle.rspace_stats(d, 4, [1.0])
+    from math import comb
+    want = [0.0 if k % 2 else comb(k, k // 2) / 2 ** k for k in range(1, 8)]
+    assert np.allclose(mom[1:8, 0], want, atol=1e-15)
+    assert mom[0, 0] == -1.0 and mom[9, 0] == 1.0
+    # bins of width 0.5 from -1: {-1, -s, -s} -> 0, {0, 0} -> 2, {s, s} -> 3,
+    # 1 -> bin 4 = nbins (dropped); per x-period, times n^2 (y, z)
+    assert list(hist[:, 0]) == [3 * n * n, 0, 2 * n * n, 2 * n * n]
+
+
+def test_rspace_stats_unit_magnitude(oracle):
+    # u = (cos z, sin z, 0): |u| = 1 everywhere
+    n = 8
+    x, y, z = grid(n)
+    d = np.stack([np.cos(z), np.sin(z), np.zeros_like(z)], axis=-1)
+    me = [1.0, 1.0, 1.0, np.sqrt(3.0)]
+    mom, hist = oracle.rspace_stats(d, 32, me)
+    assert mom.shape == (10, 4) and hist.shape == (32, 4)
+    assert np.allclose(mom[:, 3], 1.0, atol=1e-15)       # min |u| = min(sqrt 3, 1)
+    assert np.allclose(mom[2, :3], [0.5, 0.5, 0.0], atol=1e-15)
+    assert mom[0, 2] == 0.0 and mom[9, 2] == 0.0
+    b = int(np.floor(1.0 * 32 / np.sqrt(3.0)))            # magnitude bins of width sqrt 3 / 32
+    assert hist[b, 3] == n ** 3 and hist[:, 3].sum() == n ** 3
+    assert hist[16, 2] == n ** 3                          # u_z = 0 -> bin 16 of 32 on [-1, 1)
