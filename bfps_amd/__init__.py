@@ -441,6 +441,36 @@ class particles_system:
         return out
 
 
+def NSVE_do_stats(fs, max_velocity_estimate=1.0, max_vorticity_estimate=1.0, nbins=256):
+    """NSVE::do_stats (bfps/cpp/full_code/NSVE.cpp:101-135) without the
+    statistics file: field::compute_stats of the vorticity and of the
+    velocity, returned as {"vorticity": ..., "velocity": ...} (moments,
+    histograms, cospectrum).  Also returns the work field, which is left
+    holding the real-space velocity as in the reference (NSVEparticles'
+    do_stats samples it)."""
+    tmp = field(fs, "THREE")
+    out = {}
+    tmp.copy_from("vorticity")
+    out["vorticity"] = tmp.compute_stats(max_vorticity_estimate / np.sqrt(3.0), nbins)
+    tmp.copy_from("velocity")
+    out["velocity"] = tmp.compute_stats(max_velocity_estimate / np.sqrt(3.0), nbins)
+    return out, tmp
+
+
+def NSVEparticles_do_stats(fs, ps, max_velocity_estimate=1.0, max_vorticity_estimate=1.0, nbins=256):
+    """NSVEparticles::do_stats (bfps/cpp/full_code/NSVEparticles.cpp:70-98)
+    without the files: the fluid statistics, then the velocity and the
+    Lagrangian acceleration sampled at the tracers ([np][3] each, in
+    global-index order)."""
+    out, tmp = NSVE_do_stats(fs, max_velocity_estimate, max_vorticity_estimate, nbins)
+    out["tracers0/velocity"] = ps.sample_compute_field(tmp)
+    fs.compute_Lagrangian_acceleration(tmp)
+    tmp.ift()
+    out["tracers0/acceleration"] = ps.sample_compute_field(tmp)
+    tmp.close()
+    return out
+
+
 def NSVEparticles_step(fs, dt):
     """NSVEparticles::step: tracers advance with u(t), then the fluid step."""
     fs._push_physics()

@@ -209,3 +209,32 @@ def test_stats_errors(B):
     with pytest.raises(B.capi.BfpshError):
         fs.compute_Lagrangian_acceleration(p)  # needs three
     fs.close()
+
+
+def test_nsveparticles_do_stats_abc(B):
+    # NSVEparticles::do_stats on the ABC flow: the sampled velocity and
+    # Lagrangian acceleration at random tracers against the analytic fields
+    # (interpolation n4 m1 on 32^3), the statistics against their closed
+    # forms: |u|^2 mean 3, u_x^2 mean 1, vorticity = velocity
+    n, nu = 32, 0.03
+    fs = make(B, n, "double", "SMOOTH", abc_spectral(n), nu=nu)
+    rng = np.random.default_rng(5)
+    pos = rng.random((3000, 3)) * 2 * np.pi
+    ps = B.particles_system(fs, 3000, 4, 1, 4, step_idx=1)
+    ps.put(pos)
+    # estimates 3 sqrt(3): component ranges +-3 hold the ABC values, whose
+    # maximum 2 would sit on the top edge (dropped) with a range of +-2
+    st = B.NSVEparticles_do_stats(fs, ps, 3 * np.sqrt(3.0), 3 * np.sqrt(3.0), 64)
+    x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]
+    u = abc_u(x, y, z)
+    a = np.stack([np.cos(z) * np.cos(x) - np.sin(x) * np.sin(y), np.cos(x) * np.cos(y) - np.sin(y) * np.sin(z),
+                  np.cos(y) * np.cos(z) - np.sin(z) * np.sin(x)], axis=-1) - nu * u
+    eu = np.abs(st["tracers0/velocity"] - u).max()
+    ea = np.abs(st["tracers0/acceleration"] - a).max()
+    say("  sampled velocity err %.3g acceleration err %.3g" % (eu, ea))
+    assert eu < 1e-4 and ea < 1e-4
+    for name in ("velocity", "vorticity"):
+        mom = st[name]["moments"]
+        assert abs(mom[2, 3] - 3.0) < 1e-12 and np.allclose(mom[2, :3], 1.0, atol=1e-12)
+        assert st[name]["histograms"][:, :3].sum() == 3 * n ** 3
+    fs.close()
