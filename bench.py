@@ -137,10 +137,41 @@ def run_precision(B, ic, wl, precision, steps, warmup, gradient=True):
             ps.sample("gradient")
         fs.synchronize()
         grad_s = (time.perf_counter() - t1) / reps
+    stats = None
+    if gradient and ps is not None:
+        stats = stats_leg(B, fs, ps, n, 4 if precision == "single" else 8)
     res = {"ms_per_step": ms / steps, "wall_ms_per_step": 1e3 * wall / steps, "profile": prof,
-           "grad_sample_s": grad_s, "device_bytes": fs.device_bytes()}
+           "grad_sample_s": grad_s, "device_bytes": fs.device_bytes(), "stats": stats}
     fs.close()
     return res
+
+
+def stats_leg(B, fs, ps, n, es):
+    """One NSVEparticles::do_stats (bfps/cpp/full_code/NSVEparticles.cpp:70-98)
+    on the device: vorticity and velocity statistics (cospectrum + real-space
+    moments and histograms of 256 bins), then the velocity and the Lagrangian
+    acceleration sampled at every tracer.  The real-space statistics kernel
+    reads its 3 real planes once: its roofline is 3 N^3 e bytes per launch."""
+    B.NSVEparticles_do_stats(fs, ps, 1.0, 1.0, 256)  # warm-up (allocations)
+    fs.synchronize()
+    fs.profile(True)
+    t0 = time.perf_counter()
+    B.NSVEparticles_do_stats(fs, ps, 1.0, 1.0, 256)
+    fs.synchronize()
+    wall = time.perf_counter() - t0
+    prof = fs.profile_report()
+    fs.profile(False)
+    out = {"do_stats_ms": 1e3 * wall, "note": "wall time incl. the host copies of the sampled tracer values"}
+    if "rspace_stats" in prof:
+        ms, cnt = prof["rspace_stats"]
+        avg = ms / cnt
+        byt = 3 * n ** 3 * es
+        out["rspace_stats"] = {"avg_ms": avg, "bytes_per_launch": byt, "achieved_gbps": byt / (avg / 1e3) / 1e9,
+                               "frac": byt / (avg / 1e3) / HBM_PEAK}
+    for k in ("pressure", "pressure_uu", "lagrangian_acceleration"):
+        if k in prof:
+            out[k + "_ms"] = prof[k][0] / prof[k][1]
+    return out
 
 
 def roofline(prof, n, es, np_, nb, traffic_ok, steps):
@@ -369,6 +400,7 @@ def main():
                    "tracers": np_, "neighbours": wl["nb"], "smoothness": wl["m"]},
         "particle_interps_per_s": np_ / (ms / 1e3),
         "gradient_sample_interps_per_s": np_ / r32["grad_sample_s"] if r32["grad_sample_s"] else None,
+        "stats": r32["stats"],
         "hbm_frac_step": alg / (ms / 1e3) / HBM_PEAK,
         "algorithmic_bytes_per_step": alg,
         "roofline": roofline(r32["profile"], n, 4, np_, wl["nb"], default_case, args.steps),
