@@ -110,7 +110,7 @@ __global__ void k_lagr_acc(const C2<T>* __restrict__ u, const C2<T>* __restrict_
     for (int c = 0; c < 3; c++) acc[c * ps + idx] = a[c];
 }
 
-// field::normalize (field.cpp:889-895): data /= npoints in the field
+// field::normalize (field.cpp:891-898): data /= npoints in the field
 // precision
 template <typename T>
 __global__ void k_div(T* __restrict__ d, size_t count, T npoints) {
@@ -123,13 +123,17 @@ __global__ void k_div(T* __restrict__ d, size_t count, T npoints) {
 // component blockIdx.y of a 1- or 9-component field.  Per value: min and
 // max (starting at 0; the magnitude's min at its estimate), sums of v^1..v^8
 // in FP64, histogram bins of width 2 M / nbins from -M (the magnitude's from
-// 0 with half that width), counted in LDS and added to hist[nbins][nvals].
+// 0 with half that width), counted in LDS ([value][bin], so that the lanes
+// of a wave spread over the banks) and added to hist[nbins][nvals].
 // Each workgroup writes its partial moments to part[group][block][10][NV].
 template <typename T, int NV>
 __global__ void __launch_bounds__(256)
 k_rstats(const T* __restrict__ d, size_t n3, int nvals, int nbins, const double* __restrict__ maxest,
          double* __restrict__ part, unsigned long long* __restrict__ hist) {
-    extern __shared__ unsigned int lh[];  // [nbins][NV]
+    // every product and sum rounded on its own, as the reference's loop (no
+    // fused multiply-adds)
+#pragma clang fp contract(off)
+    extern __shared__ unsigned int lh[];  // [NV][nbins]
     __shared__ double red[4][10 * NV];
     constexpr int NCV = NV == 4 ? 3 : 1;  // components read per point
     const int c0 = NV == 4 ? 0 : blockIdx.y;
@@ -147,16 +151,33 @@ k_rstats(const T* __restrict__ d, size_t n3, int nvals, int nbins, const double*
         for (int k = 0; k < 8; k++) s[k][v] = 0.0;
     }
     if (NV == 4) mn[3] = me[3];
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + tid; i < n3; i += size_t(gridDim.x) * blockDim.x) {
+    // the loads of the next PF points of this thread are in flight while a
+    // point is processed (the per-point work is short against the memory
+    // latency at the occupancy the FP64 accumulators allow)
+    constexpr int PF = 4;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    const size_t i0 = size_t(blockIdx.x) * blockDim.x + tid;
+    T buf[PF][NCV];
+#pragma unroll
+    for (int p = 0; p < PF; p++)
+#pragma unroll
+        for (int c = 0; c < NCV; c++) buf[p][c] = i0 + p * stride < n3 ? d[size_t(c0 + c) * n3 + i0 + p * stride] : T(0);
+    for (size_t i = i0; i < n3; i += stride) {
         double val[NV];
 #pragma unroll
-        for (int c = 0; c < NCV; c++) val[c] = double(d[size_t(c0 + c) * n3 + i]);
+        for (int c = 0; c < NCV; c++) val[c] = double(buf[0][c]);
+#pragma unroll
+        for (int p = 0; p + 1 < PF; p++)
+#pragma unroll
+            for (int c = 0; c < NCV; c++) buf[p][c] = buf[p + 1][c];
+        const size_t inext = i + PF * stride;
+#pragma unroll
+        for (int c = 0; c < NCV; c++) buf[PF - 1][c] = inext < n3 ? d[size_t(c0 + c) * n3 + inext] : T(0);
         if (NV == 4) {
-            // |u| = sqrt(((0 + u0^2) + u1^2) + u2^2), each product rounded
-            // (no fused multiply-add), as the reference's loop
-            double m2 = __dmul_rn(val[0], val[0]);
-            m2 = __dadd_rn(m2, __dmul_rn(val[1], val[1]));
-            m2 = __dadd_rn(m2, __dmul_rn(val[2], val[2]));
+            // |u| = sqrt(((0 + u0^2) + u1^2) + u2^2)
+            double m2 = val[0] * val[0];
+            m2 = m2 + val[1] * val[1];
+            m2 = m2 + val[2] * val[2];
             val[NV - 1] = sqrt(m2);
         }
 #pragma unroll
@@ -164,13 +185,13 @@ k_rstats(const T* __restrict__ d, size_t n3, int nvals, int nbins, const double*
             mn[v] = val[v] < mn[v] ? val[v] : mn[v];
             mx[v] = val[v] > mx[v] ? val[v] : mx[v];
             const bool magnitude = NV == 4 && v == 3;
-            const double fb = magnitude ? floor(__dmul_rn(val[v], 2.0) / bs[v]) : floor(__dadd_rn(val[v], me[v]) / bs[v]);
-            if (fb >= 0.0 && fb < double(nbins)) atomicAdd(&lh[int(fb) * NV + v], 1u);
+            const double fb = floor((magnitude ? val[v] * 2.0 : val[v] + me[v]) / bs[v]);
+            if (fb >= 0.0 && fb < double(nbins)) atomicAdd(&lh[v * nbins + int(fb)], 1u);
             double pw = 1.0;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                pw = __dmul_rn(val[v], pw);
-                s[k][v] = __dadd_rn(s[k][v], pw);
+                pw = val[v] * pw;
+                s[k][v] = s[k][v] + pw;
             }
         }
     }
@@ -207,7 +228,7 @@ k_rstats(const T* __restrict__ d, size_t n3, int nvals, int nbins, const double*
     }
     for (int i = tid; i < nbins * NV; i += blockDim.x) {
         const unsigned int cnt = lh[i];
-        if (cnt) atomicAdd(&hist[size_t(i / NV) * nvals + c0 + i % NV], (unsigned long long)cnt);
+        if (cnt) atomicAdd(&hist[size_t(i % nbins) * nvals + c0 + i / nbins], (unsigned long long)cnt);
     }
 }
 
