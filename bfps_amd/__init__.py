@@ -21,10 +21,12 @@ real ``[nz][ny][nx][ncomp]``).  Everything runs on the GPU through
 libbfps_hip.so; there is no CPU fallback.
 """
 import ctypes
+import os
 
 import numpy as np
 
 from . import capi
+from . import h5io
 
 __all__ = ["vorticity_equation", "kspace", "field", "particles_system", "NSVEparticles_step", "ensemble", "capi"]
 
@@ -92,6 +94,10 @@ class vorticity_equation:
         self.fk1 = 4.0
         self.forcing_type = "linear"
         self.particles = None
+        # checkpoint files <name>_checkpoint_<checkpoint>.h5, holding up to
+        # checkpoints_per_file fields (NSVE.cpp:46-56 sets both)
+        self.checkpoint = 0
+        self.checkpoints_per_file = 1
 
     # ---------------------------------------------------------------- basics
     @property
@@ -144,16 +150,43 @@ class vorticity_equation:
         a = np.ascontiguousarray(np.asarray(data).reshape(self.cshape()), dtype=self.cdtype)
         capi.check(self._h, capi.load().bfpsh_put_cfield(self._h, capi.CVORTICITY, a.ctypes.data), "put_cfield")
 
+    def get_current_fname(self):
+        """<name>_checkpoint_<checkpoint>.h5 (vorticity_equation.hpp:98-104)."""
+        return "%s_checkpoint_%d.h5" % (self.name, self.checkpoint)
+
+    def update_checkpoint(self):
+        """vorticity_equation::update_checkpoint (vorticity_equation.cpp:49-108):
+        move to the next file once the current one holds checkpoints_per_file
+        fields and not this iteration's; create the file (with the group
+        vorticity/complex) when it does not exist."""
+        fname = self.get_current_fname()
+        if os.path.exists(fname):
+            with h5io.File(fname) as f:
+                stored = f.keys("vorticity/complex") if "vorticity/complex" in f else []
+            if len(stored) >= self.checkpoints_per_file and str(self.iteration) not in stored:
+                self.checkpoint += 1
+        else:
+            with h5io.File(fname, "w") as f:
+                f.create_group("vorticity/complex")
+
     def io_checkpoint(self, read=True, data=None):
-        """read=True: load `data` as cvorticity and apply the reference's read
-        projections (symmetrize, low_pass(kM), force_divfree;
-        bfps/cpp/vorticity_equation.hpp:107-128).  read=False: return the
-        current cvorticity as it would be written."""
+        """vorticity_equation::io_checkpoint (vorticity_equation.hpp:107-128).
+        read=True: cvorticity <- vorticity/complex/<iteration> of the current
+        checkpoint file (or `data`, an array in the same layout), then the
+        reference's read projections (symmetrize, low_pass(kM),
+        force_divfree).  read=False: update_checkpoint, then write cvorticity
+        as vorticity/complex/<iteration> ({r, i} compound of rnumber)."""
         if read:
+            if data is None:
+                with h5io.File(self.get_current_fname()) as f:
+                    data = f["vorticity/complex/%d" % self.iteration]
             self.cvorticity = data
             capi.check(self._h, capi.load().bfpsh_load_projection(self._h), "load_projection")
             return None
-        return self.cvorticity
+        self.update_checkpoint()
+        with h5io.File(self.get_current_fname(), "a") as f:
+            f["vorticity/complex/%d" % self.iteration] = self.cvorticity
+        return None
 
     def compute_velocity(self):
         """cvelocity = Biot-Savart(cvorticity) + symmetrize (spectral)."""
@@ -431,6 +464,28 @@ class particles_system:
         capi.check(self.fs.handle, rc, "particles_sample")
         return out
 
+    def write_checkpoint(self, fname, iteration):
+        """The tracer part of a checkpoint (particles_output_hdf5,
+        bfps/cpp/particles/particles_output_hdf5.hpp:110-300):
+        tracers0/state/<iteration> [np][3] and tracers0/rhs/<iteration>
+        [nsteps][np][3], FP64, in global-index order."""
+        pos, rhs, _, _ = self.get()
+        with h5io.File(fname, "a") as f:
+            f["tracers0/state/%d" % iteration] = pos
+            f["tracers0/rhs/%d" % iteration] = rhs
+
+    def read_checkpoint(self, fname, iteration):
+        """Loads state and history as particles_system_builder does
+        (particles_system_builder.hpp:193-233): tracers0/state/<iteration>
+        and tracers0/rhs/<iteration>, global indexes 0..np-1 in file order."""
+        with h5io.File(fname) as f:
+            pos = f["tracers0/state/%d" % iteration]
+            rhs = f["tracers0/rhs/%d" % iteration]
+        if pos.shape != (self.np, 3) or rhs.shape != (self.nsteps, self.np, 3):
+            raise capi.BfpshError("checkpoint holds %s / %s, the system has %d tracers and %d steps of history"
+                                  % (pos.shape, rhs.shape, self.np, self.nsteps))
+        self.put(pos, rhs)
+
     def sample_compute_field(self, f, out=None):
         """sample_compute_field of a standalone field in real space
         (bfps/cpp/particles/particles_system.hpp:143-170): [np][ncomp]."""
@@ -469,6 +524,23 @@ def NSVEparticles_do_stats(fs, ps, max_velocity_estimate=1.0, max_vorticity_esti
     out["tracers0/acceleration"] = ps.sample_compute_field(tmp)
     tmp.close()
     return out
+
+
+def NSVEparticles_write_checkpoint(fs, ps):
+    """NSVEparticles::write_checkpoint (NSVEparticles.cpp:44-56): the
+    vorticity, then the tracers into the same checkpoint file."""
+    fs.io_checkpoint(False)
+    if ps is not None:
+        ps.write_checkpoint(fs.get_current_fname(), fs.iteration)
+
+
+def NSVEparticles_read_checkpoint(fs, ps):
+    """NSVE::initialize + NSVEparticles::initialize (NSVE.cpp:35-60,
+    NSVEparticles.cpp:7-30): vorticity and tracers of fs.iteration from the
+    current checkpoint file."""
+    fs.io_checkpoint(True)
+    if ps is not None:
+        ps.read_checkpoint(fs.get_current_fname(), fs.iteration)
 
 
 def NSVEparticles_step(fs, dt):

@@ -1,0 +1,480 @@
+"""Checkpoint files in the reference's format without libhdf5.
+
+The reference stores its state in HDF5 files written through the HDF5 C
+library: ``<simname>_checkpoint_<k>.h5`` with ``vorticity/complex/<it>``
+(compound {r, i} of rnumber, ``[ny][nz][nx/2+1][3]``),
+``tracers0/state/<it>`` (``[np][3]`` f64) and ``tracers0/rhs/<it>``
+(``[nsteps][np][3]`` f64) (bfps/cpp/field.cpp:164-356,
+bfps/cpp/vorticity_equation.cpp:49-108,
+bfps/cpp/particles/particles_output_hdf5.hpp:110-300).  libhdf5 and h5py
+are not part of this build, so this module reads and writes the subset of
+the HDF5 file format those files use:
+
+* superblock version 0, 8-byte offsets and lengths, group leaf K 4 and
+  internal K 16 (the library defaults);
+* groups as symbol tables: a version-1 B-tree of symbol-table nodes plus a
+  local heap of names;
+* version-1 object headers; dataspace (v1, v2), datatype (fixed point,
+  floating point, and the two-member compound {r, i} of floats that holds
+  complex numbers), fill value, data layout v3 (contiguous or compact) and
+  modification-time messages; header continuations when reading.
+
+The writer lays files out as the C library does for these objects (the
+same message set, flags and encodings, dataspaces with maximum dimensions,
+early space allocation): a fresh file is written in one pass; opening an
+existing file in mode "a" reads every dataset and rewrites the whole file
+with the additions (a cost proportional to the file).  Anything outside the
+subset raises ``H5FormatError`` so that a misread cannot pass silently.
+"""
+import os
+import struct
+import time
+
+import numpy as np
+
+SIGNATURE = b"\x89HDF\r\n\x1a\n"
+UNDEF = 0xFFFFFFFFFFFFFFFF
+LEAF_K = 4        # symbol-table node holds up to 2 LEAF_K entries
+INTERNAL_K = 16   # B-tree node holds up to 2 INTERNAL_K children
+
+
+class H5FormatError(Exception):
+    pass
+
+
+# ---------------------------------------------------------------- datatypes
+def _float_props(size):
+    """(bit field bytes, properties) of an IEEE little-endian float."""
+    if size == 4:
+        return bytes([0x20, 31, 0]), struct.pack("<HHBBBBI", 0, 32, 23, 8, 0, 23, 127)
+    if size == 8:
+        return bytes([0x20, 63, 0]), struct.pack("<HHBBBBI", 0, 64, 52, 11, 0, 52, 1023)
+    raise H5FormatError("float size %d" % size)
+
+
+def _encode_datatype(dt):
+    """Datatype message body for a numpy dtype."""
+    dt = np.dtype(dt)
+    if dt.kind == "f":
+        bits, props = _float_props(dt.itemsize)
+        return bytes([0x11]) + bits + struct.pack("<I", dt.itemsize) + props
+    if dt.kind in "iu":
+        signed = 0x08 if dt.kind == "i" else 0x00
+        return (bytes([0x10, signed, 0, 0]) + struct.pack("<I", dt.itemsize) +
+                struct.pack("<HH", 0, 8 * dt.itemsize))
+    if dt.kind == "c":
+        half = dt.itemsize // 2
+        member = _encode_datatype(np.dtype("<f%d" % half))
+        body = b""
+        for i, name in enumerate((b"r", b"i")):
+            nm = name + b"\x00" * (8 - len(name))
+            body += nm + struct.pack("<I", i * half) + bytes([0, 0, 0, 0]) + struct.pack("<I", 0) + b"\x00" * 4
+            body += b"\x00" * 16 + member
+        return bytes([0x16, 2, 0, 0]) + struct.pack("<I", dt.itemsize) + body
+    raise H5FormatError("dtype %s" % dt)
+
+
+def _decode_datatype(b, off):
+    """numpy dtype of the datatype encoded at b[off:], and its length."""
+    cls, ver = b[off] & 0x0F, b[off] >> 4
+    bits = b[off + 1:off + 4]
+    size = struct.unpack_from("<I", b, off + 4)[0]
+    if cls == 0:  # fixed point
+        if bits[0] & 0x01:
+            raise H5FormatError("big-endian integers")
+        return np.dtype("<%s%d" % ("i" if bits[0] & 0x08 else "u", size)), 8 + 4
+    if cls == 1:  # floating point
+        if bits[0] & 0x01 or bits[0] & 0x40:
+            raise H5FormatError("non little-endian float")
+        if size not in (4, 8):
+            raise H5FormatError("float size %d" % size)
+        return np.dtype("<f%d" % size), 8 + 12
+    if cls == 6:  # compound: {r, i} floats -> complex
+        nmemb = bits[0] | (bits[1] << 8)
+        p = off + 8
+        members = []
+        for _ in range(nmemb):
+            e = b.index(b"\x00", p)
+            name = b[p:e].decode()
+            if ver == 3:
+                p = e + 1
+                nb = 1
+                while size >= (1 << (8 * nb)):
+                    nb += 1
+                moff = int.from_bytes(b[p:p + nb], "little")
+                p += nb
+            else:
+                p += ((e - p) // 8 + 1) * 8
+                moff = struct.unpack_from("<I", b, p)[0]
+                p += 4
+                if ver == 1:
+                    if b[p] != 0:
+                        raise H5FormatError("array members")
+                    p += 28
+            mdt, mlen = _decode_datatype(b, p)
+            p += mlen
+            members.append((name, moff, mdt))
+        if (len(members) == 2 and [m[0] for m in members] == ["r", "i"] and members[0][2] == members[1][2]
+                and members[0][2].kind == "f" and members[0][1] == 0 and members[1][1] == members[0][2].itemsize
+                and size == 2 * members[0][2].itemsize):
+            return np.dtype("<c%d" % size), p - off
+        raise H5FormatError("compound type other than {r, i}: %s" % members)
+    raise H5FormatError("datatype class %d" % cls)
+
+
+# ---------------------------------------------------------------- reading
+class _Dataset:
+    def __init__(self, dtype, shape, address=None, data=None, compact=None):
+        self.dtype, self.shape = np.dtype(dtype), tuple(int(s) for s in shape)
+        self.address, self.data, self.compact = address, data, compact
+
+
+def _u(b, off, n):
+    return int.from_bytes(b[off:off + n], "little")
+
+
+class _Parser:
+    def __init__(self, path):
+        self.f = open(path, "rb")
+        head = self.f.read(96)
+        if head[:8] != SIGNATURE:
+            raise H5FormatError("%s is not an HDF5 file" % path)
+        if head[8] != 0:
+            raise H5FormatError("superblock version %d" % head[8])
+        if head[13] != 8 or head[14] != 8:
+            raise H5FormatError("offset/length sizes %d/%d" % (head[13], head[14]))
+        self.root = _u(head, 56 + 8, 8)
+
+    def read(self, addr, n):
+        self.f.seek(addr)
+        b = self.f.read(n)
+        if len(b) != n:
+            raise H5FormatError("short read at %d" % addr)
+        return b
+
+    def messages(self, addr):
+        h = self.read(addr, 16)
+        if h[0] != 1:
+            raise H5FormatError("object header version %d" % h[0])
+        nmsg, hsize = _u(h, 2, 2), _u(h, 8, 4)
+        blocks, out = [(addr + 16, hsize)], []
+        while blocks:
+            start, length = blocks.pop(0)
+            b = self.read(start, length)
+            p = 0
+            while p + 8 <= length and len(out) < nmsg:
+                mtype, msize, flags = _u(b, p, 2), _u(b, p + 2, 2), b[p + 4]
+                body = b[p + 8:p + 8 + msize]
+                if mtype == 0x10:
+                    blocks.append((_u(body, 0, 8), _u(body, 8, 8)))
+                if flags & 0x02 and mtype not in (0x00, 0x10):
+                    raise H5FormatError("shared message type 0x%x" % mtype)
+                out.append((mtype, body))
+                p += 8 + msize
+        return out
+
+    def object(self, addr):
+        msgs = self.messages(addr)
+        for t, body in msgs:
+            if t == 0x11:
+                return self.group(_u(body, 0, 8), _u(body, 8, 8))
+        dtype = shape = layout = None
+        for t, body in msgs:
+            if t == 0x01:
+                ver, rank, flags = body[0], body[1], body[2]
+                p = 8 if ver == 1 else 4
+                if ver not in (1, 2):
+                    raise H5FormatError("dataspace version %d" % ver)
+                if ver == 2 and body[3] == 0:
+                    rank = 0
+                shape = tuple(_u(body, p + 8 * i, 8) for i in range(rank))
+            elif t == 0x03:
+                dtype = _decode_datatype(body, 0)[0]
+            elif t == 0x08:
+                if body[0] != 3:
+                    raise H5FormatError("layout version %d" % body[0])
+                if body[1] == 1:
+                    layout = ("contiguous", _u(body, 2, 8), _u(body, 10, 8))
+                elif body[1] == 0:
+                    n = _u(body, 2, 2)
+                    layout = ("compact", bytes(body[4:4 + n]))
+                else:
+                    raise H5FormatError("layout class %d (chunked data)" % body[1])
+            elif t in (0x0B, 0x0C) or t > 0x18:
+                raise H5FormatError("object header message 0x%x" % t)
+        if dtype is None or shape is None or layout is None:
+            raise H5FormatError("object at %d is neither a group nor a dataset" % addr)
+        if layout[0] == "compact":
+            return _Dataset(dtype, shape, compact=layout[1])
+        return _Dataset(dtype, shape, address=None if layout[1] == UNDEF else layout[1])
+
+    def group(self, btree, heap):
+        h = self.read(heap, 32)
+        if h[:4] != b"HEAP":
+            raise H5FormatError("local heap signature")
+        names = self.read(_u(h, 24, 8), _u(h, 8, 8))
+        entries = {}
+        self._btree(btree, names, entries)
+        return {name: self.object(addr) for name, addr in entries.items()}
+
+    def _btree(self, addr, names, out):
+        h = self.read(addr, 24)
+        if h[:4] != b"TREE" or h[4] != 0:
+            raise H5FormatError("group B-tree node")
+        level, used = h[5], _u(h, 6, 2)
+        body = self.read(addr + 24, 8 * (2 * used + 1))
+        for i in range(used):
+            child = _u(body, 8 + 16 * i, 8)
+            if level > 0:
+                self._btree(child, names, out)
+            else:
+                self._snod(child, names, out)
+
+    def _snod(self, addr, names, out):
+        h = self.read(addr, 8)
+        if h[:4] != b"SNOD":
+            raise H5FormatError("symbol table node signature")
+        n = _u(h, 6, 2)
+        body = self.read(addr + 8, 40 * n)
+        for e in range(n):
+            noff, obj = _u(body, 40 * e, 8), _u(body, 40 * e + 8, 8)
+            end = names.index(b"\x00", noff)
+            out[names[noff:end].decode()] = obj
+
+    def data(self, ds):
+        n = int(np.prod(ds.shape, dtype=np.int64)) * ds.dtype.itemsize
+        if ds.compact is not None:
+            raw = ds.compact[:n]
+        elif ds.address is None:
+            return np.zeros(ds.shape, ds.dtype)
+        else:
+            raw = self.read(ds.address, n)
+        return np.frombuffer(raw, ds.dtype).reshape(ds.shape).copy()
+
+
+# ---------------------------------------------------------------- writing
+def _pad8(n):
+    return (n + 7) // 8 * 8
+
+
+def _msg(mtype, body, flags=0):
+    body = body + b"\x00" * (_pad8(len(body)) - len(body))
+    return struct.pack("<HHB3x", mtype, len(body), flags) + body
+
+
+def _object_header(messages):
+    data = b"".join(messages)
+    return struct.pack("<BBHII4x", 1, 0, len(messages), 1, len(data)) + data
+
+
+def _dataset_messages(shape, dtype, data_addr, nbytes, mtime):
+    rank = len(shape)
+    space = struct.pack("<BBB5x", 1, rank, 1) + b"".join(struct.pack("<Q", s) for s in shape) * 2
+    return [
+        _msg(0x01, space),
+        _msg(0x03, _encode_datatype(dtype), flags=1),
+        _msg(0x05, bytes([2, 1, 2, 1]) + struct.pack("<I", 0), flags=1),
+        _msg(0x08, bytes([3, 1]) + struct.pack("<QQ", data_addr, nbytes), flags=1),
+        _msg(0x12, struct.pack("<B3xI", 1, int(mtime))),
+    ]
+
+
+class _Writer:
+    """Lays out a tree of groups ({name: subtree}) and arrays in one file."""
+
+    def __init__(self, fh):
+        self.fh = fh
+        self.pos = 96
+        self.mtime = time.time()
+
+    def alloc(self, n):
+        a = self.pos
+        self.pos += _pad8(n)
+        return a
+
+    def put(self, addr, b):
+        self.fh.seek(addr)
+        self.fh.write(b)
+
+    def write_object(self, node):
+        """Returns (object header address, cache type, scratch bytes)."""
+        if isinstance(node, dict):
+            btree, heap = self.write_group(node)
+            return self._group_header(btree, heap), 1, struct.pack("<QQ", btree, heap)
+        arr = np.array(node, order="C")  # (ascontiguousarray would make a 0-d array 1-d)
+        if arr.dtype.byteorder == ">":
+            arr = arr.astype(arr.dtype.newbyteorder("<"))
+        nbytes = arr.nbytes
+        probe = _object_header(_dataset_messages(arr.shape, arr.dtype, 0, nbytes, self.mtime))
+        hdr = self.alloc(len(probe))
+        daddr = self.alloc(nbytes) if nbytes else UNDEF
+        self.put(hdr, _object_header(_dataset_messages(arr.shape, arr.dtype, daddr, nbytes, self.mtime)))
+        if nbytes:
+            self.put(daddr, arr.tobytes())
+        return hdr, 0, b"\x00" * 16
+
+    def _group_header(self, btree, heap):
+        h = _object_header([_msg(0x11, struct.pack("<QQ", btree, heap))])
+        a = self.alloc(len(h))
+        self.put(a, h)
+        return a
+
+    def write_group(self, tree):
+        names = sorted(tree, key=lambda s: s.encode())
+        # local heap: "" at offset 0, then each name, 8-byte aligned
+        seg, offs = b"\x00" * 8, {}
+        for nm in names:
+            enc = nm.encode()
+            if b"\x00" in enc or not enc or "/" in nm:
+                raise H5FormatError("bad link name %r" % nm)
+            offs[nm] = len(seg)
+            seg += enc + b"\x00" * (_pad8(len(enc) + 1) - len(enc))
+        heap = self.alloc(32)
+        dseg = self.alloc(len(seg))
+        self.put(heap, b"HEAP" + bytes([0, 0, 0, 0]) + struct.pack("<QQQ", len(seg), UNDEF, dseg))
+        self.put(dseg, seg)
+        children = {nm: self.write_object(tree[nm]) for nm in names}
+        if not names:  # an empty group: a B-tree node without entries
+            a = self.alloc(24 + 8 * (4 * INTERNAL_K + 1))
+            self.put(a, b"TREE" + bytes([0, 0]) + struct.pack("<HQQ", 0, UNDEF, UNDEF) +
+                     b"\x00" * (8 * (4 * INTERNAL_K + 1)))
+            return a, heap
+        # symbol-table nodes of up to 2 LEAF_K entries, in name order
+        leaves = []
+        for i in range(0, len(names), 2 * LEAF_K):
+            chunk = names[i:i + 2 * LEAF_K]
+            body = b"".join(struct.pack("<QQI4x", offs[nm], children[nm][0], children[nm][1]) + children[nm][2]
+                            for nm in chunk)
+            body += b"\x00" * (40 * 2 * LEAF_K - len(body))
+            a = self.alloc(8 + len(body))
+            self.put(a, b"SNOD" + bytes([1, 0]) + struct.pack("<H", len(chunk)) + body)
+            leaves.append((a, offs[chunk[-1]]))
+        # B-tree levels of up to 2 INTERNAL_K children
+        level, nodes = 0, leaves
+        while True:
+            parents = []
+            for i in range(0, len(nodes), 2 * INTERNAL_K):
+                chunk = nodes[i:i + 2 * INTERNAL_K]
+                body = struct.pack("<Q", 0)
+                for child, last in chunk:
+                    body += struct.pack("<QQ", child, last)
+                body += b"\x00" * (8 * (4 * INTERNAL_K + 1) - len(body))
+                a = self.alloc(24 + len(body))
+                self.put(a, b"TREE" + bytes([0, level]) + struct.pack("<HQQ", len(chunk), UNDEF, UNDEF) + body)
+                parents.append((a, chunk[-1][1]))
+            if len(parents) == 1:
+                return parents[0][0], heap
+            nodes, level = parents, level + 1
+
+
+def _write_file(path, tree):
+    tmp = path + ".tmp%d" % os.getpid()
+    with open(tmp, "wb") as fh:
+        w = _Writer(fh)
+        btree, heap = w.write_group(tree)
+        root = w._group_header(btree, heap)
+        eof = w.pos
+        sb = SIGNATURE + bytes([0, 0, 0, 0, 0, 8, 8, 0]) + struct.pack("<HHI", LEAF_K, INTERNAL_K, 0)
+        sb += struct.pack("<QQQQ", 0, UNDEF, eof, UNDEF)
+        sb += struct.pack("<QQI4x", 0, root, 1) + struct.pack("<QQ", btree, heap)
+        w.put(0, sb)
+        fh.truncate(eof)
+    os.replace(tmp, path)
+
+
+# ---------------------------------------------------------------- interface
+class File:
+    """A checkpoint file: ``File(path, "r" | "a" | "w")``.  Datasets are
+    addressed by slash-separated paths (``f["vorticity/complex/0"]``);
+    complex arrays are stored as the reference's {r, i} compound.  In modes
+    "a" and "w" assignments are written when the file is closed."""
+
+    def __init__(self, path, mode="r"):
+        if mode not in ("r", "a", "w"):
+            raise ValueError("mode must be r, a or w")
+        self.path, self.mode = path, mode
+        self._tree, self._parser, self._dirty = {}, None, mode == "w"
+        if mode == "r" or (mode == "a" and os.path.exists(path)):
+            self._parser = _Parser(path)
+            self._tree = self._parser.object(self._parser.root)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _node(self, path, create=False):
+        parts = [p for p in path.split("/") if p]
+        node = self._tree
+        for i, p in enumerate(parts):
+            if not isinstance(node, dict):
+                raise KeyError(path)
+            if p not in node:
+                if not create:
+                    raise KeyError(path)
+                node[p] = {} if i + 1 < len(parts) else None
+            if i + 1 < len(parts):
+                node = node[p]
+        return node, parts[-1] if parts else None
+
+    def __contains__(self, path):
+        try:
+            parent, last = self._node(path)
+            return last is None or last in parent
+        except KeyError:
+            return False
+
+    def keys(self, path="/"):
+        parent, last = self._node(path)
+        node = parent if last is None else parent[last]
+        if not isinstance(node, dict):
+            raise KeyError("%s is a dataset" % path)
+        return sorted(node)
+
+    def __getitem__(self, path):
+        parent, last = self._node(path)
+        node = parent[last]
+        if isinstance(node, dict):
+            raise KeyError("%s is a group" % path)
+        if isinstance(node, _Dataset):
+            return self._parser.data(node)
+        return np.array(node)
+
+    def create_group(self, path):
+        if self.mode == "r":
+            raise IOError("file opened read-only")
+        node = self._tree
+        for p in [p for p in path.split("/") if p]:
+            node = node.setdefault(p, {})
+            if not isinstance(node, dict):
+                raise KeyError("%s is a dataset" % path)
+        self._dirty = True
+
+    def __setitem__(self, path, value):
+        if self.mode == "r":
+            raise IOError("file opened read-only")
+        parent, last = self._node(path, create=True)
+        arr = np.asarray(value)
+        if arr.dtype.kind not in "fiuc":
+            raise H5FormatError("dtype %s" % arr.dtype)
+        parent[last] = arr
+        self._dirty = True
+
+    def close(self):
+        if self.mode != "r" and self._dirty:
+            def materialise(node):
+                if isinstance(node, dict):
+                    return {k: materialise(v) for k, v in node.items()}
+                if isinstance(node, _Dataset):
+                    return self._parser.data(node)
+                return node
+            tree = materialise(self._tree)
+            if self._parser is not None:
+                self._parser.f.close()
+                self._parser = None
+            _write_file(self.path, tree)
+            self._dirty = False
+        if self._parser is not None:
+            self._parser.f.close()
+            self._parser = None

@@ -1,0 +1,108 @@
+"""Checkpoint files on the device path: the reference's own regression
+(bfps/test/test_bfps_NSVEparticles.py) run through files -- a new run's
+checkpoint 0 is made from the golden file's iteration 0, the run advances 32
+steps and writes its checkpoint, a second solver restarts from that file and
+writes iteration 64 -- and both written iterations match the golden file at
+the reference's 1e-5.  Also: checkpoint-file rollover
+(vorticity_equation.cpp:49-108) and an FP64 write/read round trip.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = os.path.join(HERE, "golden", "B32p1e4_checkpoint_0.h5")
+NU = (2.0 * 2 / 32) ** (4. / 3)
+DT = 0.5 / 32
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def B():
+    import bfps_amd
+    bfps_amd.capi.load()
+    return bfps_amd
+
+
+def solver(B, name, iteration):
+    fs = B.vorticity_equation(name, 32, precision="single")
+    fs.nu, fs.famplitude, fs.fk0, fs.fk1, fs.forcing_type = NU, 0.5, 2.0, 4.0, "linear"
+    fs.checkpoints_per_file = 3
+    fs.iteration = iteration
+    return fs
+
+
+def test_fixture_run_through_checkpoint_files(B, tmp_path, monkeypatch):
+    from bfps_amd import h5io
+    say("case: golden fixture through checkpoint files (32^3 single, 2 x 32 steps)")
+    monkeypatch.chdir(tmp_path)
+    # the run's initial checkpoint: iteration 0 of the golden file
+    with h5io.File(REF) as g, h5io.File("run_checkpoint_0.h5", "w") as f:
+        for k in ("vorticity/complex/0", "tracers0/state/0", "tracers0/rhs/0"):
+            f[k] = g[k]
+    fs = solver(B, "run", 0)
+    ps = B.particles_system(fs, 10000, 1, 1, 4, step_idx=1)
+    B.NSVEparticles_read_checkpoint(fs, ps)
+    for _ in range(32):
+        B.NSVEparticles_step(fs, DT)
+    B.NSVEparticles_write_checkpoint(fs, ps)
+    fs.close()
+    fs2 = solver(B, "run", 32)
+    ps2 = B.particles_system(fs2, 10000, 1, 1, 4, step_idx=33)
+    B.NSVEparticles_read_checkpoint(fs2, ps2)
+    for _ in range(32):
+        B.NSVEparticles_step(fs2, DT)
+    B.NSVEparticles_write_checkpoint(fs2, ps2)
+    assert fs2.get_current_fname() == "run_checkpoint_0.h5"  # 3 fields per file
+    fs2.close()
+    with h5io.File(REF) as g, h5io.File("run_checkpoint_0.h5") as f:
+        assert f.keys("vorticity/complex") == ["0", "32", "64"]
+        for it in (32, 64):
+            ev = np.abs(f["vorticity/complex/%d" % it] - g["vorticity/complex/%d" % it]).max()
+            es = np.abs(f["tracers0/state/%d" % it] - g["tracers0/state/%d" % it]).max()
+            er = np.abs(f["tracers0/rhs/%d" % it] - g["tracers0/rhs/%d" % it]).max()
+            say("  iteration %d: vorticity %.3g state %.3g rhs %.3g" % (it, ev, es, er))
+            assert f["vorticity/complex/%d" % it].dtype == np.complex64
+            assert ev < 1e-5 and es < 1e-5 and er < 1e-5
+
+
+def test_checkpoint_rollover_and_fp64_roundtrip(B, tmp_path, monkeypatch):
+    from bfps_amd import h5io, ic
+    monkeypatch.chdir(tmp_path)
+    fs = B.vorticity_equation("r", 16, precision="double")
+    fs.checkpoints_per_file = 2
+    fs.io_checkpoint(read=True, data=ic.solenoidal_vorticity(16, seed=4, k0=2.0))
+    written = {}
+    for it in (0, 4, 8, 12, 16):
+        fs.iteration = it
+        fs.io_checkpoint(False)
+        written[it] = (fs.get_current_fname(), fs.cvorticity)
+        fs.step(0.01)
+    # two fields per file: 0, 4 | 8, 12 | 16
+    assert [written[i][0] for i in (0, 4, 8, 12, 16)] == ["r_checkpoint_0.h5"] * 2 + ["r_checkpoint_1.h5"] * 2 + \
+        ["r_checkpoint_2.h5"]
+    with h5io.File("r_checkpoint_1.h5") as f:
+        assert f.keys("vorticity/complex") == ["12", "8"]
+        assert f["vorticity/complex/8"].dtype == np.complex128
+    # rewriting an iteration already in a full file stays in that file
+    fs.checkpoint = 1
+    fs.iteration = 8
+    fs.io_checkpoint(False)
+    assert fs.get_current_fname() == "r_checkpoint_1.h5"
+    # reading back: the read projections (symmetrize, low_pass, divfree)
+    # change a field the step already projected only by rounding
+    fs2 = B.vorticity_equation("r", 16, precision="double")
+    fs2.checkpoint, fs2.iteration = 1, 12
+    fs2.io_checkpoint(True)
+    w12 = written[12][1]
+    assert np.abs(fs2.cvorticity - w12).max() < 1e-13 * np.abs(w12).max()
+    fs.close()
+    fs2.close()
