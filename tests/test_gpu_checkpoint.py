@@ -106,3 +106,39 @@ def test_checkpoint_rollover_and_fp64_roundtrip(B, tmp_path, monkeypatch):
     assert np.abs(fs2.cvorticity - w12).max() < 1e-13 * np.abs(w12).max()
     fs.close()
     fs2.close()
+
+
+def test_driver_main_loop_and_stop_file(B, tmp_path):
+    # the golden case through the driver: launch rules for n = 32, the
+    # run's checkpoint 0 from the golden file's iteration 0, main_loop over
+    # 32 iterations (statistics every 16, sampling every 16), checkpoint 32
+    # against the golden file; then a restart stopped by stop_<simname>
+    from bfps_amd import dns, h5io
+    say("case: driver main loop from the golden checkpoint, stop file")
+    wd = str(tmp_path)
+    with h5io.File(REF) as g, h5io.File(os.path.join(wd, "run_checkpoint_0.h5"), "w") as f:
+        for k in ("vorticity/complex/0", "tracers0/state/0", "tracers0/rhs/0"):
+            f[k] = g[k]
+    p = dns.launch_parameters(32, nparticles=10000, niter_todo=32, niter_out=32, niter_stat=16, niter_part=16,
+                              checkpoints_per_file=3, histogram_bins=64)
+    run = dns.NSVEparticles("run", p, iteration=0, work_dir=wd)
+    run.initialize()
+    run.main_loop()
+    assert run.iteration == 32
+    assert [it for it, _ in run.statistics] == [0, 16, 32]
+    assert run.statistics[-1][1]["tracers0/acceleration"].shape == (10000, 3)
+    run.finalize()
+    with h5io.File(REF) as g, h5io.File(os.path.join(wd, "run_checkpoint_0.h5")) as f:
+        ev = np.abs(f["vorticity/complex/32"] - g["vorticity/complex/32"]).max()
+        es = np.abs(f["tracers0/state/32"] - g["tracers0/state/32"]).max()
+        say("  iteration 32: vorticity %.3g state %.3g" % (ev, es))
+        assert ev < 1e-5 and es < 1e-5
+    open(os.path.join(wd, "stop_run"), "w").close()
+    run = dns.NSVEparticles("run", p, iteration=32, work_dir=wd)
+    run.initialize()
+    run.main_loop()
+    assert run.iteration == 33  # one step, then the stop file
+    run.finalize()
+    with h5io.File(os.path.join(wd, "run_checkpoint_0.h5")) as f:
+        assert f.keys("vorticity/complex") == ["0", "32", "33"]
+        assert "tracers0/rhs/33" in f
