@@ -742,6 +742,8 @@ struct bfpsh_ensemble {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool captured_ru = false;          // the graph writes the real velocity
+    static constexpr int NPB = 4;      // neighbour counts batched in one step
+    PartBatch pb[NPB];                 // tracer-loop tables, one per neighbour count
 };
 
 static int efail(bfpsh_ensemble* e, int code, const std::string& msg) {
@@ -756,6 +758,7 @@ static int efail(bfpsh_ensemble* e, int code, const std::string& msg) {
 
 static void ensemble_free(bfpsh_ensemble* e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (PartBatch& pb : e->pb) bfpsh_part_batch_free(pb);
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     for (bfpsh_ctx* c : e->boxes)
@@ -927,12 +930,21 @@ int bfpsh_ensemble_step(bfpsh_ensemble* e, const double* dt, int with_particles)
         if (r) return r;
     }
     EHIPCHK(hipGraphLaunch(e->exec, e->stream));
-    for (size_t i = 0; i < e->boxes.size(); i++) {
-        bfpsh_ctx* c = e->boxes[i];
-        if (any_part && with_particles && c->nsteps > 0 && c->np > 0) {
-            r = bfpsh_part_loop(c, dt[i]);
-            if (r) return efail(e, r, c->err);
+    if (any_part) {
+        // the tracers of all boxes, one launch per neighbour count
+        std::vector<bfpsh_ctx*> pbox;
+        std::vector<double> pdt;
+        for (size_t i = 0; i < e->boxes.size(); i++) {
+            bfpsh_ctx* c = e->boxes[i];
+            if (c->nsteps > 0 && c->np > 0) {
+                pbox.push_back(c);
+                pdt.push_back(dt[i]);
+            }
         }
+        r = bfpsh_part_loop_batch(pbox, pdt, e->pb, bfpsh_ensemble::NPB);
+        if (r) return efail(e, r, pbox[0]->err);
+    }
+    for (bfpsh_ctx* c : e->boxes) {
         c->ru_valid = false;
         c->iteration++;
     }

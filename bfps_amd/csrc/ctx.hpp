@@ -227,6 +227,19 @@ int create_ctx(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, 
 int bfpsh_part_loop(bfpsh_ctx* ctx, double dt);
 int bfpsh_part_sample(bfpsh_ctx* ctx, const void* field, int nc, double* d_out);
 int bfpsh_part_release(bfpsh_ctx* ctx);
+// tracer loops of many boxes (on one stream) in one launch per neighbour
+// count; pb holds nbatch device tables and their pinned host staging
+struct PartBatch {
+    void* d = nullptr;          // device table
+    size_t cap = 0;
+    void* h[2] = {nullptr, nullptr};  // pinned host staging, used in turn
+    size_t hcap = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int k = 0;
+};
+int bfpsh_part_loop_batch(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch* pb,
+                          int nbatch);
+void bfpsh_part_batch_free(PartBatch& pb);
 
 #define BFPSH_FOR_SIZES(X) X(16) X(32) X(48) X(64) X(128) X(256) X(512) X(1024)
 #define BFPSH_DECL_OPS(N) const OpsTable* bfpsh_ops_f32_##N(); const OpsTable* bfpsh_ops_f64_##N();

@@ -160,19 +160,15 @@ __device__ __forceinline__ void gather_coop(const T* __restrict__ f, int n, cons
     }
 }
 
+// One tracer of the Adams-Bashforth loop (MODE 0 of the brick kernel, for
+// the global-gather path): velocity at the tracer into the history head,
+// then the position update of order `order`.  t = rank of the tracer in the
+// sorted order, jx = lane in its group.
 template <typename T, int NB>
-__global__ void __launch_bounds__(256)
-k_interp_ab(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ rhs,
-            const uint32_t* __restrict__ perm, int64_t np, int nsteps, int head, int order, int add_head,
-            double dt, const double* __restrict__ tab, int deg, PartGeomD pg) {
-    constexpr int LP = Lanes<NB>::LP;
-    extern __shared__ double stab[];
-    const int tabsize = (2 * NB + 2) * (deg + 1);  // derivative-0 block only
-    for (int i = threadIdx.x; i < tabsize; i += blockDim.x) stab[i] = tab[i];
-    __syncthreads();
-    const int64_t t = (xcd_block() * blockDim.x + threadIdx.x) / LP;
-    const int jx = threadIdx.x % LP;
-    if (t >= np) return;  // whole lane groups exit together
+__device__ __forceinline__ void interp_ab_one(const T* __restrict__ f, double* __restrict__ pos,
+                                              double* __restrict__ rhs, const uint32_t* __restrict__ perm,
+                                              int64_t np, int nsteps, int head, int order, int add_head, double dt,
+                                              const double* stab, int deg, const PartGeomD& pg, int64_t t, int jx) {
     const int64_t p = perm[t];
     double x[3] = {pos[p * 3 + 0], pos[p * 3 + 1], pos[p * 3 + 2]};
     double r0[3];
@@ -206,6 +202,63 @@ k_interp_ab(const T* __restrict__ f, double* __restrict__ pos, double* __restric
     pos[p * 3 + 0] = x[0];
     pos[p * 3 + 1] = x[1];
     pos[p * 3 + 2] = x[2];
+}
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(256)
+k_interp_ab(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ rhs,
+            const uint32_t* __restrict__ perm, int64_t np, int nsteps, int head, int order, int add_head,
+            double dt, const double* __restrict__ tab, int deg, PartGeomD pg) {
+    constexpr int LP = Lanes<NB>::LP;
+    extern __shared__ double stab[];
+    const int tabsize = (2 * NB + 2) * (deg + 1);  // derivative-0 block only
+    for (int i = threadIdx.x; i < tabsize; i += blockDim.x) stab[i] = tab[i];
+    __syncthreads();
+    const int64_t t = (xcd_block() * blockDim.x + threadIdx.x) / LP;
+    const int jx = threadIdx.x % LP;
+    if (t >= np) return;  // whole lane groups exit together
+    interp_ab_one<T, NB>(f, pos, rhs, perm, np, nsteps, head, order, add_head, dt, stab, deg, pg, t, jx);
+}
+
+// The tracer loops of many boxes (an ensemble) in one launch: box b owns
+// workgroups [block0_b, block0_b + ceil(np_b LP / 256)) and its own field,
+// tracers, history state and geometry.
+struct TracerBox {
+    const void* f;
+    double* pos;
+    double* rhs;
+    const uint32_t* perm;
+    const double* tab;
+    int64_t np;
+    PartGeomD pg;
+    double dt;
+    int nsteps, head, order, add_head, deg;
+    uint32_t block0;
+};
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(256)
+k_interp_ab_multi(const TracerBox* __restrict__ boxes, int nboxes) {
+    constexpr int LP = Lanes<NB>::LP;
+    extern __shared__ double stab[];
+    // the box of this workgroup: the last one starting at or before it
+    int lo = 0, hi = nboxes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (boxes[mid].block0 <= blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const TracerBox& bx = boxes[lo];
+    const int deg = bx.deg;
+    const int tabsize = (2 * NB + 2) * (deg + 1);
+    for (int i = threadIdx.x; i < tabsize; i += blockDim.x) stab[i] = bx.tab[i];
+    __syncthreads();
+    const int64_t t = (int64_t(blockIdx.x - bx.block0) * blockDim.x + threadIdx.x) / LP;
+    const int jx = threadIdx.x % LP;
+    if (t >= bx.np) return;  // whole lane groups exit together
+    const PartGeomD pg = bx.pg;
+    interp_ab_one<T, NB>((const T*)bx.f, bx.pos, bx.rhs, bx.perm, bx.np, bx.nsteps, bx.head, bx.order, bx.add_head,
+                         bx.dt, stab, deg, pg, t, jx);
 }
 
 template <typename T, int NB, int NC>
@@ -699,7 +752,132 @@ int sample_t(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
     return fail(ctx, BFPSH_EINVAL, "bad neighbour count");
 }
 
+// The tracer loops of the boxes of one neighbour count in one launch
+// (boxes on one stream, global-gather kernel); per-box sorts as in loop_nb.
+template <typename T, int NB>
+int loop_multi_nb(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch& pb) {
+    constexpr int LP = Lanes<NB>::LP;
+    bfpsh_ctx* ctx = boxes[0];  // the shared stream; errors are reported on this box
+    const size_t nb = boxes.size();
+    const size_t bytes = nb * sizeof(TracerBox);
+    // pinned host staging, two buffers in turn, each reused only once the
+    // copy issued from it two batches ago has completed
+    if (pb.hcap < bytes) {
+        for (int k = 0; k < 2; k++) {
+            if (pb.ev[k]) HIPCHK(hipEventSynchronize(pb.ev[k]));
+            if (pb.h[k]) HIPCHK(hipHostFree(pb.h[k]));
+            HIPCHK(hipHostMalloc(&pb.h[k], bytes));
+        }
+        pb.hcap = bytes;
+    }
+    if (pb.cap < bytes) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (pb.d) HIPCHK(hipFree(pb.d));
+        HIPCHK(hipMalloc(&pb.d, bytes));
+        pb.cap = bytes;
+    }
+    for (int k = 0; k < 2; k++)
+        if (!pb.ev[k]) HIPCHK(hipEventCreateWithFlags(&pb.ev[k], hipEventDisableTiming));
+    const int k = pb.k;
+    pb.k ^= 1;
+    HIPCHK(hipEventSynchronize(pb.ev[k]));
+    TracerBox* tb = static_cast<TracerBox*>(pb.h[k]);
+    uint32_t blocks = 0;
+    int maxdeg = 0;
+    for (size_t i = 0; i < nb; i++) {
+        bfpsh_ctx* c = boxes[i];
+        if (c->sort_age >= c->sort_every) {
+            int r = sort_tracers(c);
+            if (r) return fail(ctx, r, c->err);
+        }
+        TracerBox& b = tb[i];
+        b.f = c->ru;
+        b.pos = c->d_pos;
+        b.rhs = c->d_rhs;
+        b.perm = c->d_perm + c->np;
+        b.tab = c->d_tab;
+        b.np = c->np;
+        b.pg = pgeom_d(c);
+        b.dt = dts[i];
+        b.nsteps = c->nsteps;
+        b.head = c->head;
+        b.order = std::min(c->step_idx, c->nsteps);
+        b.add_head = c->add_head ? 1 : 0;
+        b.deg = c->deg;
+        b.block0 = blocks;
+        blocks += uint32_t((c->np * LP + 255) / 256);
+        maxdeg = std::max(maxdeg, c->deg);
+    }
+    HIPCHK(hipMemcpyAsync(pb.d, tb, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipEventRecord(pb.ev[k], ctx->stream));
+    const size_t lds = size_t(2 * NB + 2) * (maxdeg + 1) * sizeof(double);
+    LAUNCH("interp_ab_multi", (k_interp_ab_multi<T, NB><<<blocks, kThreads, lds, ctx->stream>>>(
+                                  (const TracerBox*)pb.d, int(nb))));
+    return BFPSH_OK;
+}
+
+template <typename T>
+int loop_multi_t(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch& pb, int nbr) {
+    switch (nbr) {
+    case 1: return loop_multi_nb<T, 1>(boxes, dts, pb);
+    case 2: return loop_multi_nb<T, 2>(boxes, dts, pb);
+    case 3: return loop_multi_nb<T, 3>(boxes, dts, pb);
+    case 4: return loop_multi_nb<T, 4>(boxes, dts, pb);
+    case 5: return loop_multi_nb<T, 5>(boxes, dts, pb);
+    case 6: return loop_multi_nb<T, 6>(boxes, dts, pb);
+    case 7: return loop_multi_nb<T, 7>(boxes, dts, pb);
+    case 8: return loop_multi_nb<T, 8>(boxes, dts, pb);
+    case 9: return loop_multi_nb<T, 9>(boxes, dts, pb);
+    case 10: return loop_multi_nb<T, 10>(boxes, dts, pb);
+    }
+    return fail(boxes[0], BFPSH_EINVAL, "bad neighbour count");
+}
+
 }  // namespace
+
+int bfpsh_part_loop_batch(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch* pb,
+                          int nbatch) {
+    // one launch per neighbour count (nbatch tables, one per group)
+    std::vector<int> counts;
+    for (bfpsh_ctx* c : boxes)
+        if (std::find(counts.begin(), counts.end(), c->pn) == counts.end()) counts.push_back(c->pn);
+    if (int(counts.size()) > nbatch) return fail(boxes[0], BFPSH_EINVAL, "too many neighbour counts in one ensemble");
+    for (size_t gi = 0; gi < counts.size(); gi++) {
+        std::vector<bfpsh_ctx*> grp;
+        std::vector<double> gdt;
+        for (size_t i = 0; i < boxes.size(); i++)
+            if (boxes[i]->pn == counts[gi]) {
+                grp.push_back(boxes[i]);
+                gdt.push_back(dts[i]);
+            }
+        int r = grp[0]->prec == BFPSH_FP32 ? loop_multi_t<float>(grp, gdt, pb[gi], counts[gi])
+                                           : loop_multi_t<double>(grp, gdt, pb[gi], counts[gi]);
+        if (r) return r;
+    }
+    for (bfpsh_ctx* c : boxes) {
+        c->sort_age += 1;
+        c->step_idx += 1;
+        c->head = (c->head + c->nsteps - 1) % c->nsteps;
+        c->head_zero = true;
+        c->add_head = false;
+    }
+    return BFPSH_OK;
+}
+
+void bfpsh_part_batch_free(PartBatch& pb) {
+    for (int k = 0; k < 2; k++) {
+        if (pb.ev[k]) {
+            (void)hipEventSynchronize(pb.ev[k]);
+            (void)hipEventDestroy(pb.ev[k]);
+        }
+        if (pb.h[k]) (void)hipHostFree(pb.h[k]);
+        pb.ev[k] = nullptr;
+        pb.h[k] = nullptr;
+    }
+    if (pb.d) (void)hipFree(pb.d);
+    pb.d = nullptr;
+    pb.cap = pb.hcap = 0;
+}
 
 int bfpsh_part_release(bfpsh_ctx* ctx) {
     for (void* p : {(void*)ctx->d_keys, (void*)ctx->d_perm, ctx->d_sort_tmp, (void*)ctx->d_bstart})
