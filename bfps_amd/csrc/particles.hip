@@ -100,9 +100,13 @@ __device__ __forceinline__ void setup_axis(double s, const double* __restrict__ 
 }
 
 // Lanes per tracer: the W = 2n+2 stencil columns along x are spread over LP
-// lanes (LP = 16 for n <= 7, else 32), so one wavefront load instruction
-// reads 64/LP contiguous row segments instead of 64 scattered words.
-template <int NB> struct Lanes { static constexpr int LP = (2 * NB + 2) <= 16 ? 16 : 32; };
+// lanes (the power of two from 4 to 32 that holds W), so one wavefront load
+// instruction reads 64/LP contiguous row segments instead of 64 scattered
+// words, with few idle lanes for small stencils.
+template <int NB> struct Lanes {
+    static constexpr int W = 2 * NB + 2;
+    static constexpr int LP = W <= 4 ? 4 : (W <= 8 ? 8 : (W <= 16 ? 16 : 32));
+};
 
 // sum over the stencil of one tracer, computed by the LP lanes of its group;
 // lane jx < W holds column jx.  Result valid in every lane of the group.
