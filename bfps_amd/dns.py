@@ -3,9 +3,19 @@ defaults and the launch-time rules of bfps/DNS.py, the tracer initial state,
 and the driver loop of the generated C++ code (NSVE / NSVEparticles /
 direct_numerical_simulation) over this package's device solver.
 
-Statistics are returned (``NSVEparticles.statistics``) instead of being
-written to ``<simname>.h5``; checkpoints are the reference's files
-(``bfps_amd.h5io``).
+Files are the reference's (through ``bfps_amd.h5io``): checkpoints
+``<simname>_checkpoint_<k>.h5``; the run file ``<simname>.h5`` with the
+parameters, ``iteration``/``checkpoint``, the wavenumber shells and the
+statistics time series (``write_par``, bfps/_base.py:158-173,
+bfps/DNS.py:372-452); the tracer samples ``<simname>_particles.h5``.  When
+``<simname>.h5`` is absent the driver keeps the statistics in memory only
+(``NSVEparticles.statistics`` holds them in either case).
+
+Differences from files written through libhdf5: the statistics datasets
+are contiguous with fixed maximum dimensions rather than chunked and
+unlimited (they are grown by rewriting the file, as every append in
+``h5io`` is), and strings are fixed-length ASCII rather than h5py's
+variable-length strings.
 """
 import os
 
@@ -62,6 +72,79 @@ def launch_parameters(n, dns_type="NSVEparticles", precision="single", kMeta=2.0
     return p
 
 
+def kspace_info(p):
+    """DNS.get_kspace (bfps/DNS.py:167-192): kM and dk of the dealiasing
+    rule, kx (0..nx/2), ky and kz (numpy's fftfreq order times dk), and
+    zeroed nshell / kshell of int(kM/dk) + 2 shells (the solver fills
+    them in when it starts, kspace::store, bfps/cpp/kspace.cpp:178-230)."""
+    div = 2 if p["dealias_type"] == 1 else 3
+    kM = max(p["dk" + a] * (p["n" + a] // div - 1) for a in "xyz")
+    dk = min(p["dkx"], p["dky"], p["dkz"])
+    nshells = int(kM / dk) + 2
+    out = {"kM": float(kM), "dk": float(dk), "nshell": np.zeros(nshells, np.int64),
+           "kshell": np.zeros(nshells), "kx": np.arange(0, p["nx"] // 2 + 1).astype(np.float64) * p["dkx"]}
+    for a in "yz":
+        n = p["n" + a]
+        k = np.arange(-n // 2 + 1, n // 2 + 1).astype(np.float64) * p["dk" + a]
+        out["k" + a] = np.roll(k, n // 2 + 1)
+    return out
+
+
+def write_par(simname, parameters, dns_type="NSVEparticles", iter0=0, work_dir="."):
+    """Creates the run file <simname>.h5 as _base.write_par + DNS.write_par
+    do (bfps/_base.py:158-173, bfps/DNS.py:372-419): parameters/<name>,
+    iteration, bfps_info/{solver_class, exec_name}, kspace/*, one row of
+    each statistics dataset (spectra/<f>_<f> [t][nshells][3][3] f64,
+    moments/<f> [t][10][4] f64, histograms/<f> [t][nbins][4] i64 for f in
+    velocity, vorticity) and checkpoint = 0; for the particle DNS types
+    also <simname>_particles.h5 with the tracers0/velocity and
+    tracers0/acceleration groups (DNS.launch_jobs, DNS.py:911-916).  The
+    schedule assertions of DNS.write_par hold first.  An existing run file
+    is an error, as h5py's mode "w-"."""
+    p = parameters
+    assert p["niter_todo"] % p["niter_stat"] == 0
+    assert p["niter_todo"] % p["niter_out"] == 0
+    assert p["niter_out"] % p["niter_stat"] == 0
+    particles = dns_type in ("NSVEparticles", "NSVEparticles_no_output")
+    if particles:
+        assert p["niter_todo"] % p["niter_part"] == 0
+        assert p["niter_out"] % p["niter_part"] == 0
+    os.makedirs(work_dir, exist_ok=True)
+    path = os.path.join(work_dir, simname + ".h5")
+    if os.path.exists(path):
+        raise FileExistsError(path)
+    with h5io.File(path, "w") as f:
+        for k, v in p.items():
+            f["parameters/" + k] = v.encode("ascii") if isinstance(v, str) else v
+        f["iteration"] = np.int64(iter0)
+        f["bfps_info/solver_class"] = b"DNS"
+        f["bfps_info/exec_name"] = dns_type.encode("ascii")
+        ks = kspace_info(p)
+        for k, v in ks.items():
+            f["kspace/" + k] = v
+        nshells, nbins = ks["nshell"].shape[0], p["histogram_bins"]
+        for k in ("velocity", "vorticity"):
+            f["statistics/spectra/%s_%s" % (k, k)] = np.zeros((1, nshells, 3, 3))
+            f["statistics/moments/" + k] = np.zeros((1, 10, 4))
+            f["statistics/histograms/" + k] = np.zeros((1, nbins, 4), np.int64)
+        f["checkpoint"] = np.int64(0)
+    if particles:
+        ppath = os.path.join(work_dir, simname + "_particles.h5")
+        if not os.path.exists(ppath):
+            with h5io.File(ppath, "w") as f:
+                f.create_group("tracers0/velocity")
+                f.create_group("tracers0/acceleration")
+    return path
+
+
+def read_iteration(simname, work_dir="."):
+    """direct_numerical_simulation::read_iteration
+    (bfps/cpp/full_code/direct_numerical_simulation.cpp:17-55): iteration
+    and checkpoint from <simname>.h5."""
+    with h5io.File(os.path.join(work_dir, simname + ".h5")) as f:
+        return int(f["iteration"]), int(f["checkpoint"])
+
+
 def generate_tracer_state(nparticles, rseed=None, batch_size=10 ** 6):
     """DNS.generate_tracer_state (bfps/DNS.py:711-733): uniform positions in
     [0, 2 pi)^3 from numpy's legacy generator (seeded with rseed when
@@ -82,13 +165,21 @@ class NSVEparticles:
     NSVEparticles.cpp:7-98) and main_loop
     (direct_numerical_simulation.cpp:88-118, with the stop file of
     code_base.cpp:15-35).  `parameters` is a launch_parameters() dict.
-    The checkpoint file holding `iteration` is found by its index
-    (`checkpoint`, as stored in <simname>.h5 by the reference), or searched
-    for when that is None."""
+    With `iteration` None the run starts at the iteration and checkpoint
+    index stored in <simname>.h5 (as the reference's read_iteration);
+    otherwise the checkpoint file holding `iteration` is the one numbered
+    `checkpoint`, or searched for when that is None.  When <simname>.h5
+    exists, statistics go into it and tracer samples into
+    <simname>_particles.h5."""
 
-    def __init__(self, simname, parameters, iteration=0, checkpoint=None, precision="single", work_dir="."):
+    def __init__(self, simname, parameters, iteration=None, checkpoint=None, precision="single", work_dir="."):
         self.simname, self.p, self.precision = simname, dict(parameters), precision
         self.work_dir = work_dir
+        self.stat_path = self._path(simname + ".h5")
+        self.particle_path = self._path(simname + "_particles.h5")
+        if iteration is None:
+            # the reference starts where <simname>.h5 says
+            iteration, checkpoint = read_iteration(simname, work_dir)
         self.iteration0, self.checkpoint0 = int(iteration), checkpoint
         self.statistics = []
         self.fs = self.ps = None
@@ -124,6 +215,55 @@ class NSVEparticles:
             self.ps = particles_system(fs, p["nparticles"], p["tracers0_neighbours"], p["tracers0_smoothness"],
                                        p["tracers0_integration_steps"], step_idx=fs.iteration + 1)
         NSVEparticles_read_checkpoint(fs, self.ps)
+        if os.path.exists(self.stat_path):
+            self._grow_and_store_kspace()
+
+    def _grow_and_store_kspace(self):
+        """NSVE::initialize on the run file: every statistics dataset grows
+        by niter_todo / niter_stat rows (hdf5_tools::grow_file_datasets,
+        bfps/cpp/hdf5_tools.cpp:23-90), then kspace::store writes the
+        solver's shells (bfps/cpp/kspace.cpp:178-230)."""
+        grow = self.p["niter_todo"] // self.p["niter_stat"]
+        kk = self.fs.kk
+        with h5io.File(self.stat_path, "a") as f:
+            for kind in f.keys("statistics"):
+                for name in f.keys("statistics/" + kind):
+                    key = "statistics/%s/%s" % (kind, name)
+                    a = f[key]
+                    f[key] = np.concatenate([a, np.zeros((grow,) + a.shape[1:], a.dtype)])
+            if f["kspace/kshell"].shape[0] != kk.nshells:
+                raise ValueError("run file holds %d shells, the solver %d" % (f["kspace/kshell"].shape[0], kk.nshells))
+            f["kspace/kshell"] = kk.kshell
+            f["kspace/nshell"] = kk.nshell
+            f["kspace/kM"] = np.float64(kk.kM)
+            f["kspace/dk"] = np.float64(kk.dk)
+
+    def _write_stats(self, it, st):
+        """The statistics rows of field::compute_stats
+        (bfps/cpp/field.cpp:836-866, kspace.cpp:561-597) at row
+        iteration / niter_stat, and the tracer samples of
+        sample_from_particles_system (particles_sampling.hpp:15-50):
+        tracers0/<name>/<step_idx> of shape [1][np][3], kept when present."""
+        if os.path.exists(self.stat_path) and it % self.p["niter_stat"] == 0:
+            row = it // self.p["niter_stat"]
+            with h5io.File(self.stat_path, "a") as f:
+                for name in ("vorticity", "velocity"):
+                    for key, val in (("spectra/%s_%s" % (name, name), st[name]["cospectrum"]),
+                                     ("moments/" + name, st[name]["moments"]),
+                                     ("histograms/" + name, st[name]["histograms"])):
+                        key = "statistics/" + key
+                        a = f[key]
+                        if row >= a.shape[0]:
+                            raise IndexError("%s has %d rows; row %d is past its end" % (key, a.shape[0], row))
+                        a[row] = val
+                        f[key] = a
+        if "tracers0/velocity" in st and os.path.exists(self.particle_path):
+            sidx = self.ps.get_step_idx()
+            with h5io.File(self.particle_path, "a") as f:
+                for name in ("velocity", "acceleration"):
+                    key = "tracers0/%s/%d" % (name, sidx)
+                    if key not in f:
+                        f[key] = st["tracers0/" + name][None]
 
     @property
     def iteration(self):
@@ -139,14 +279,24 @@ class NSVEparticles:
         it, p = self.fs.iteration, self.p
         args = (p["max_velocity_estimate"], p["max_vorticity_estimate"], p["histogram_bins"])
         if self.ps is not None and it % p["niter_part"] == 0:
-            self.statistics.append((it, NSVEparticles_do_stats(self.fs, self.ps, *args)))
+            st = NSVEparticles_do_stats(self.fs, self.ps, *args)
         elif it % p["niter_stat"] == 0:
             st, tmp = NSVE_do_stats(self.fs, *args)
             tmp.close()
-            self.statistics.append((it, st))
+        else:
+            return
+        self.statistics.append((it, st))
+        self._write_stats(it, st)
 
     def write_checkpoint(self):
+        """NSVEparticles::write_checkpoint, then write_iteration
+        (direct_numerical_simulation.cpp:57-87): iteration and checkpoint
+        into the run file."""
         NSVEparticles_write_checkpoint(self.fs, self.ps)
+        if os.path.exists(self.stat_path):
+            with h5io.File(self.stat_path, "a") as f:
+                f["iteration"] = np.int64(self.fs.iteration)
+                f["checkpoint"] = np.int64(self.fs.checkpoint)
 
     def check_stopping_condition(self):
         return os.path.exists(self._path("stop_" + self.simname))
@@ -171,4 +321,5 @@ class NSVEparticles:
             self.fs = self.ps = None
 
 
-__all__ = ["default_parameters", "launch_parameters", "generate_tracer_state", "NSVEparticles"]
+__all__ = ["default_parameters", "launch_parameters", "kspace_info", "write_par", "read_iteration",
+           "generate_tracer_state", "NSVEparticles"]

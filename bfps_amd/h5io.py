@@ -15,9 +15,12 @@ the HDF5 file format those files use:
 * groups as symbol tables: a version-1 B-tree of symbol-table nodes plus a
   local heap of names;
 * version-1 object headers; dataspace (v1, v2), datatype (fixed point,
-  floating point, and the two-member compound {r, i} of floats that holds
-  complex numbers), fill value, data layout v3 (contiguous or compact) and
-  modification-time messages; header continuations when reading.
+  floating point, fixed-length strings, and the two-member compound {r, i}
+  of floats that holds complex numbers), fill value, data layout v3
+  (contiguous or compact) and modification-time messages; header
+  continuations when reading.  Strings are written fixed-length (a Python
+  str as ASCII bytes); variable-length strings and chunked (extendable)
+  datasets are not read.
 
 The writer lays files out as the C library does for these objects (the
 same message set, flags and encodings, dataspaces with maximum dimensions,
@@ -62,6 +65,10 @@ def _encode_datatype(dt):
         signed = 0x08 if dt.kind == "i" else 0x00
         return (bytes([0x10, signed, 0, 0]) + struct.pack("<I", dt.itemsize) +
                 struct.pack("<HH", 0, 8 * dt.itemsize))
+    if dt.kind == "S":
+        # fixed-length ASCII string, null padded (numpy bytes_, as h5py
+        # stores a Python bytes value)
+        return bytes([0x13, 0x01, 0, 0]) + struct.pack("<I", dt.itemsize)
     if dt.kind == "c":
         half = dt.itemsize // 2
         member = _encode_datatype(np.dtype("<f%d" % half))
@@ -89,6 +96,8 @@ def _decode_datatype(b, off):
         if size not in (4, 8):
             raise H5FormatError("float size %d" % size)
         return np.dtype("<f%d" % size), 8 + 12
+    if cls == 3:  # fixed-length string (any padding, ASCII or UTF-8)
+        return np.dtype("S%d" % size), 8
     if cls == 6:  # compound: {r, i} floats -> complex
         nmemb = bits[0] | (bits[1] << 8)
         p = off + 8
@@ -455,8 +464,14 @@ class File:
         if self.mode == "r":
             raise IOError("file opened read-only")
         parent, last = self._node(path, create=True)
+        if isinstance(value, str):
+            value = value.encode("ascii")
         arr = np.asarray(value)
-        if arr.dtype.kind not in "fiuc":
+        if arr.dtype.kind == "b":
+            arr = arr.astype(np.int64)
+        if arr.dtype.kind == "S" and arr.dtype.itemsize == 0:
+            arr = arr.astype("S1")
+        if arr.dtype.kind not in "fiucS":
             raise H5FormatError("dtype %s" % arr.dtype)
         parent[last] = arr
         self._dirty = True

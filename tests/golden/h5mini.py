@@ -153,6 +153,8 @@ class File:
         if cls == 0:  # fixed point
             signed = bool(bits & 0x08)
             return np.dtype(("<i%d" if signed else "<u%d") % size)
+        if cls == 3:  # fixed-length string
+            return np.dtype("S%d" % size)
         if cls == 6:  # compound (version 1 members)
             nmemb = bits & 0xFFFF
             p = data + 8

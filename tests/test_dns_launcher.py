@@ -38,3 +38,50 @@ def test_launch_rules():
     assert dns.launch_parameters(64, niter_todo=10, niter_out=8)["niter_out"] == 10
     assert dns.launch_parameters(64, nu=0.02)["nu"] == 0.02          # a given value wins
     assert "nparticles" not in dns.launch_parameters(64, dns_type="NSVE")
+
+
+def test_write_par_run_file(tmp_path):
+    # the run file of DNS.write_par (bfps/_base.py:158-173, bfps/DNS.py:372-419)
+    import os
+    import sys
+    import pytest
+    from bfps_amd import h5io
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import h5mini
+    p = dns.launch_parameters(32, nparticles=10000, niter_todo=32, niter_out=32, niter_stat=16, niter_part=16,
+                              histogram_bins=64)
+    path = dns.write_par("run", p, work_dir=str(tmp_path))
+    assert dns.read_iteration("run", str(tmp_path)) == (0, 0)
+    with h5io.File(path) as f:
+        assert f["parameters/nu"] == NU and f["parameters/nu"].dtype == np.float64
+        assert f["parameters/nx"] == 32 and f["parameters/nx"].dtype == np.int64
+        assert f["parameters/forcing_type"] == b"linear"
+        assert f["bfps_info/exec_name"] == b"NSVEparticles"
+        # wavenumbers: k_i = i up to the Nyquist index (positive), i - n above
+        n = 32
+        signed = np.array([i if i <= n // 2 else i - n for i in range(n)], dtype=np.float64)
+        assert np.array_equal(f["kspace/kx"], np.arange(n // 2 + 1.0))
+        assert np.array_equal(f["kspace/ky"], signed) and np.array_equal(f["kspace/kz"], signed)
+        assert f["kspace/kM"] == 15.0 and f["kspace/dk"] == 1.0
+        assert f["kspace/nshell"].shape == (17,) and f["kspace/nshell"].dtype == np.int64
+        for k in ("velocity", "vorticity"):
+            assert f["statistics/spectra/%s_%s" % (k, k)].shape == (1, 17, 3, 3)
+            assert f["statistics/moments/" + k].shape == (1, 10, 4)
+            assert f["statistics/histograms/" + k].shape == (1, 64, 4)
+            assert f["statistics/histograms/" + k].dtype == np.int64
+        assert f.keys("statistics") == ["histograms", "moments", "spectra"]
+    # the independent test reader agrees
+    assert h5mini.read(path, "parameters/forcing_type") == b"linear"
+    assert h5mini.read(path, "statistics/moments/velocity").shape == (1, 10, 4)
+    with h5io.File(os.path.join(str(tmp_path), "run_particles.h5")) as f:
+        assert f.keys("tracers0") == ["acceleration", "velocity"]
+    with pytest.raises(FileExistsError):
+        dns.write_par("run", p, work_dir=str(tmp_path))
+    # TWO_THIRDS shells: kM = dk (n // 3 - 1)
+    q = dns.launch_parameters(48, dns_type="NSVE", dealias_type=0, niter_todo=8, niter_out=8)
+    ks = dns.kspace_info(q)
+    assert ks["kM"] == 15.0 and ks["nshell"].shape == (17,)
+    # the schedule rules of DNS.write_par
+    bad = dict(p, niter_out=8, niter_stat=3)
+    with pytest.raises(AssertionError):
+        dns.write_par("bad", bad, work_dir=str(tmp_path))

@@ -121,24 +121,44 @@ def test_driver_main_loop_and_stop_file(B, tmp_path):
             f[k] = g[k]
     p = dns.launch_parameters(32, nparticles=10000, niter_todo=32, niter_out=32, niter_stat=16, niter_part=16,
                               checkpoints_per_file=3, histogram_bins=64)
-    run = dns.NSVEparticles("run", p, iteration=0, work_dir=wd)
+    dns.write_par("run", p, work_dir=wd)
+    run = dns.NSVEparticles("run", p, work_dir=wd)  # iteration 0 from run.h5
     run.initialize()
+    kk = run.fs.kk
     run.main_loop()
     assert run.iteration == 32
     assert [it for it, _ in run.statistics] == [0, 16, 32]
     assert run.statistics[-1][1]["tracers0/acceleration"].shape == (10000, 3)
     run.finalize()
+    # the run file: rows iteration / niter_stat of each statistics dataset,
+    # the solver's shells, iteration and checkpoint; tracer samples named by
+    # the step index (iteration + 1) with shape [1][np][3]
+    with h5io.File(os.path.join(wd, "run.h5")) as f, h5io.File(os.path.join(wd, "run_particles.h5")) as fp:
+        assert int(f["iteration"]) == 32 and int(f["checkpoint"]) == 0
+        assert np.array_equal(f["kspace/kshell"], kk.kshell) and np.array_equal(f["kspace/nshell"], kk.nshell)
+        for row, (it, st) in enumerate(run.statistics):
+            for name in ("velocity", "vorticity"):
+                assert np.array_equal(f["statistics/moments/" + name][row], st[name]["moments"])
+                assert np.array_equal(f["statistics/histograms/" + name][row], st[name]["histograms"])
+                assert np.array_equal(f["statistics/spectra/%s_%s" % (name, name)][row], st[name]["cospectrum"])
+            for name in ("velocity", "acceleration"):
+                assert np.array_equal(fp["tracers0/%s/%d" % (name, it + 1)], st["tracers0/" + name][None])
+        assert f["statistics/moments/velocity"].shape == (3, 10, 4)
+        assert fp.keys("tracers0/velocity") == ["1", "17", "33"]
     with h5io.File(REF) as g, h5io.File(os.path.join(wd, "run_checkpoint_0.h5")) as f:
         ev = np.abs(f["vorticity/complex/32"] - g["vorticity/complex/32"]).max()
         es = np.abs(f["tracers0/state/32"] - g["tracers0/state/32"]).max()
         say("  iteration 32: vorticity %.3g state %.3g" % (ev, es))
         assert ev < 1e-5 and es < 1e-5
     open(os.path.join(wd, "stop_run"), "w").close()
-    run = dns.NSVEparticles("run", p, iteration=32, work_dir=wd)
+    run = dns.NSVEparticles("run", p, work_dir=wd)  # iteration 32, checkpoint 0 from run.h5
     run.initialize()
     run.main_loop()
     assert run.iteration == 33  # one step, then the stop file
     run.finalize()
+    with h5io.File(os.path.join(wd, "run.h5")) as f:
+        assert int(f["iteration"]) == 33
+        assert f["statistics/moments/velocity"].shape == (5, 10, 4)  # grown by niter_todo / niter_stat
     with h5io.File(os.path.join(wd, "run_checkpoint_0.h5")) as f:
         assert f.keys("vorticity/complex") == ["0", "32", "33"]
         assert "tracers0/rhs/33" in f
