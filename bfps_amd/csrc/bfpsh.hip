@@ -36,49 +36,53 @@ void build_twiddles(std::vector<unsigned char>& out, int n) {
 }  // namespace
 
 // host [ky][kz][kx=N/2+1][nc]{re,im} <-> device planar [c][ky][kz][kx<N/2]
+// (ky rows row_stride(n) complex apart, planes plane_size(n) apart)
 template <typename T>
 static void host_to_planar(const T* h, T* p, int n, int nc = 3) {
     const int nxh = n / 2 + 1, nxs = n / 2;
-    const size_t ps = size_t(n) * n * nxs;
+    const size_t ps = plane_size(n);
     for (size_t yz = 0; yz < size_t(n) * n; yz++)
         for (int x = 0; x < nxs; x++)
             for (int c = 0; c < nc; c++) {
-                p[(c * ps + yz * nxs + x) * 2 + 0] = h[((yz * nxh + x) * nc + c) * 2 + 0];
-                p[(c * ps + yz * nxs + x) * 2 + 1] = h[((yz * nxh + x) * nc + c) * 2 + 1];
+                const size_t d = c * ps + sphys(n, yz * nxs + x);
+                p[d * 2 + 0] = h[((yz * nxh + x) * nc + c) * 2 + 0];
+                p[d * 2 + 1] = h[((yz * nxh + x) * nc + c) * 2 + 1];
             }
 }
 template <typename T>
 static void planar_to_host(const T* p, T* h, int n, int nc = 3) {
     const int nxh = n / 2 + 1, nxs = n / 2;
-    const size_t ps = size_t(n) * n * nxs;
+    const size_t ps = plane_size(n);
     for (size_t yz = 0; yz < size_t(n) * n; yz++)
         for (int x = 0; x < nxh; x++)
             for (int c = 0; c < nc; c++) {
                 T re = 0, im = 0;
                 if (x < nxs) {
-                    re = p[(c * ps + yz * nxs + x) * 2 + 0];
-                    im = p[(c * ps + yz * nxs + x) * 2 + 1];
+                    const size_t d = c * ps + sphys(n, yz * nxs + x);
+                    re = p[d * 2 + 0];
+                    im = p[d * 2 + 1];
                 }
                 h[((yz * nxh + x) * nc + c) * 2 + 0] = re;
                 h[((yz * nxh + x) * nc + c) * 2 + 1] = im;
             }
 }
-// host real [z][y][x][nc] <-> device planar [c][y][z][x]
+// host real [z][y][x][nc] <-> device planar [c][y][z][x] (real rows y
+// 2 row_stride(n) reals apart, planes 2 plane_size(n) apart)
 template <typename T>
 static void host_real_to_planar(const T* h, T* p, int n, int nc) {
-    const size_t n3 = size_t(n) * n * n;
+    const size_t ps = 2 * plane_size(n), ys = 2 * row_stride(n);
     for (size_t y = 0; y < size_t(n); y++)
         for (size_t z = 0; z < size_t(n); z++)
             for (size_t x = 0; x < size_t(n); x++)
-                for (int c = 0; c < nc; c++) p[c * n3 + (y * n + z) * n + x] = h[((z * n + y) * n + x) * nc + c];
+                for (int c = 0; c < nc; c++) p[c * ps + y * ys + z * n + x] = h[((z * n + y) * n + x) * nc + c];
 }
 template <typename T>
 static void planar_to_host_real(const T* p, T* h, int n, int nc) {
-    const size_t n3 = size_t(n) * n * n;
+    const size_t ps = 2 * plane_size(n), ys = 2 * row_stride(n);
     for (size_t y = 0; y < size_t(n); y++)
         for (size_t z = 0; z < size_t(n); z++)
             for (size_t x = 0; x < size_t(n); x++)
-                for (int c = 0; c < nc; c++) h[((z * n + y) * n + x) * nc + c] = p[c * n3 + (y * n + z) * n + x];
+                for (int c = 0; c < nc; c++) h[((z * n + y) * n + x) * nc + c] = p[c * ps + y * ys + z * n + x];
 }
 
 #define CTX_CHECK()                                  \
@@ -164,7 +168,7 @@ int create_ctx(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, 
     for (int s = 0; s < ctx->nshells; s++)
         if (ctx->nshell[s]) ctx->kshell[s] /= ctx->nshell[s];
     // device state
-    ctx->ps = size_t(n) * n * (n / 2);
+    ctx->ps = plane_size(n);
     const size_t vbytes = 3 * ctx->ps * 2 * ctx->es;
     if (slab) {
         // 18 planes: v0 (3), v1 (3), v2 (3), work (6), real velocity (3); the
@@ -290,8 +294,8 @@ int bfpsh_get_rfield(bfpsh_ctx* ctx, int which, void* host) {
     const void* src = which == BFPSH_RVELOCITY ? ctx->ru : (which == BFPSH_RGRADIENT ? ctx->rgrad : nullptr);
     const int nc = which == BFPSH_RVELOCITY ? 3 : 9;
     if (!src) return fail(ctx, BFPSH_ESTATE, "real field not computed");
-    const size_t n = ctx->n, n3 = n * n * n;
-    std::vector<unsigned char> tmp(nc * n3 * ctx->es);
+    const size_t n = ctx->n, rps = 2 * ctx->ps, rys = 2 * row_stride(ctx->n);
+    std::vector<unsigned char> tmp(nc * rps * ctx->es);
     HIPCHK(hipMemcpyAsync(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     // device [c][y][z][x] -> host [z][y][x][c]
@@ -299,7 +303,7 @@ int bfpsh_get_rfield(bfpsh_ctx* ctx, int which, void* host) {
         for (size_t z = 0; z < n; z++)
             for (size_t x = 0; x < n; x++)
                 for (int c = 0; c < nc; c++) {
-                    const size_t di = c * n3 + (y * n + z) * n + x;
+                    const size_t di = c * rps + y * rys + z * n + x;
                     const size_t hi = ((z * n + y) * n + x) * nc + c;
                     if (ctx->prec == BFPSH_FP32) ((float*)host)[hi] = ((const float*)tmp.data())[di];
                     else ((double*)host)[hi] = ((const double*)tmp.data())[di];
@@ -869,7 +873,7 @@ int bfpsh_ensemble_create(bfpsh_ensemble** out, int nboxes, const int* sizes, in
         e->cls.back().members.push_back(i);
     }
     for (auto& c : e->cls) {
-        const size_t ps = size_t(c.n) * c.n * (c.n / 2);
+        const size_t ps = plane_size(c.n);
         const size_t box_bytes = 18 * ps * 2 * es;
         const size_t nb = c.members.size();
         c.bstride = 18 * ps;

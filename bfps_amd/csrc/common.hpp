@@ -22,5 +22,32 @@ template <typename T, int SIGN> BFPSH_DEV C2<T> cmul_i(C2<T> a) {
     return SIGN < 0 ? mk<T>(a.y, -a.x) : mk<T>(-a.y, a.x);
 }
 
+// Device layout strides.  A spectral plane (one component) holds N ky rows,
+// each the kz-kx slab of N*N/2 complex values; consecutive ky rows are
+// row_stride(N) = N*N/2 + BFPSH_PAD_Y complex apart and the components of
+// a vector field plane_size(N) = N*row_stride(N) apart.  The pad breaks
+// the power-of-two stride of the y passes (4 MiB at 1024^3 FP32), whose
+// rows otherwise meet on the same HBM banks: one strided y pass over a
+// plane measured 2.1 -> 2.9 TB/s at 1024^3 FP32 and 4.5 -> 4.8 TB/s at
+// 512^3 with a pad of 16.  A real plane uses the same bytes: real row y
+// (the [z][x] slab) starts at y * 2*row_stride(N) reals.  Pad elements are
+// never read as data.
+#ifndef BFPSH_PAD_Y
+#define BFPSH_PAD_Y 16
+#endif
+__host__ __device__ constexpr size_t row_stride(int n) { return size_t(n) * size_t(n / 2) + BFPSH_PAD_Y; }
+__host__ __device__ constexpr size_t plane_size(int n) { return size_t(n) * row_stride(n); }
+// physical offset (complex) of the logical spectral index (ky*N + kz)*N/2 + kx
+// (logical indices of N <= 1024 fit 32 bits)
+__host__ __device__ inline size_t sphys(int n, size_t idx) {
+    const uint32_t s = uint32_t(n) * uint32_t(n / 2), i = uint32_t(idx);
+    return size_t(i / s) * row_stride(n) + i % s;
+}
+// physical offset (reals) of the logical real index (y*N + z)*N + x
+__host__ __device__ inline size_t rphys(int n, size_t idx) {
+    const uint32_t s = uint32_t(n) * uint32_t(n), i = uint32_t(idx);
+    return size_t(i / s) * 2 * row_stride(n) + i % s;
+}
+
 // Wave-size facts for CDNA4.
 constexpr int WAVE = 64;

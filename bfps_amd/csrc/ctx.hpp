@@ -33,7 +33,7 @@ struct bfpsh_ctx {
     bool force_coop = false;       // tracers: global-gather kernel, no per-step sort
     // device buffers (element size es = sizeof(rnumber))
     size_t es = 4;
-    size_t ps = 0;                 // complex elements per spectral plane
+    size_t ps = 0;                 // complex elements per plane, row pads included (plane_size)
     void* v0 = nullptr;            // cvorticity (3 planes)
     void* v1 = nullptr;            // v[1]
     void* v2 = nullptr;            // v[2]
@@ -184,7 +184,7 @@ inline int ensure_work(bfpsh_ctx* ctx, int planes) {
 
 inline int ensure_real(bfpsh_ctx* ctx, void** buf, int planes) {
     if (*buf) return BFPSH_OK;
-    const size_t sz = size_t(planes) * ctx->n * ctx->n * ctx->n * ctx->es;
+    const size_t sz = size_t(planes) * ctx->ps * 2 * ctx->es;  // real planes of the padded layout
     HIPCHK(hipMalloc(buf, sz));
     HIPCHK(hipMemsetAsync(*buf, 0, sz, ctx->stream));
     ctx->bytes += sz;

@@ -79,7 +79,7 @@ struct Impl {
             if (r) return r;
         }
         LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 6 * nb), NT, 0, ctx->stream>>>(
-                             w, size_t(N) * NXH, size_t(NXH), tw, 6, bs)));
+                             w, row_stride(N), size_t(NXH), tw, 6, bs)));
         if constexpr (RowsW<T, N>::ok && !BFPSH_ROWS_TILE)
             LAUNCH("rows_cross", (k_rows_cross_w<T, N><<<dim3(N * N / 2 / RowsW<T, N>::RPB, nb), RowsW<T, N>::NT, 0,
                                                          ctx->stream>>>(w, ru, tw, T(double(N) * N * N), bs)));
@@ -87,7 +87,7 @@ struct Impl {
             LAUNCH("rows_cross", (k_rows_cross<T, N><<<dim3(N * N / 2 / CF::RP, nb), CF::NT_ROW, 0, ctx->stream>>>(
                                       w, ru, tw, T(double(N) * N * N), bs)));
         LAUNCH("fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, 3 * nb), NT, 0, ctx->stream>>>(
-                             w, size_t(NXH), size_t(N) * NXH, tw, 3, bs)));
+                             w, size_t(NXH), row_stride(N), tw, 3, bs)));
         const dim3 tiles3(NXH / CF::B_Y3, N, nb);
         const char* nm = s == 1 ? "fwd_y_update1" : (s == 2 ? "fwd_y_update2" : "fwd_y_update3");
         const CT* vp = s == 1 ? v1 : (s == 2 ? v1 : v2);
@@ -185,7 +185,7 @@ struct Impl {
         const CT* tw = (const CT*)ctx->tw;
         r = bs_pass<1>(ctx, (const CT*)ctx->v0, w, tw, geom(ctx), "inv_z_bs_u");
         if (r) return r;
-        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 3), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw, 3, 0)));
+        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 3), NT, 0, ctx->stream>>>(w, row_stride(N), size_t(NXH), tw, 3, 0)));
         LAUNCH("rows_c2r3", (k_rows_c2r<T, N, 3><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->ru, tw)));
         ctx->ru_valid = true;
         return BFPSH_OK;
@@ -200,7 +200,7 @@ struct Impl {
         const CT* tw = (const CT*)ctx->tw;
         r = bs_pass<2>(ctx, (const CT*)ctx->v0, w, tw, geom(ctx), "inv_z_bs_grad");
         if (r) return r;
-        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 9), NT, 0, ctx->stream>>>(w, size_t(N) * NXH, size_t(NXH), tw, 9, 0)));
+        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 9), NT, 0, ctx->stream>>>(w, row_stride(N), size_t(NXH), tw, 9, 0)));
         LAUNCH("rows_c2r9", (k_rows_c2r<T, N, 9><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->rgrad, tw)));
         return BFPSH_OK;
     }
@@ -211,8 +211,8 @@ struct Impl {
     static int field_ift_nc(bfpsh_ctx* ctx, void* data) {
         CT* d = (CT*)data;
         const CT* tw = (const CT*)ctx->tw;
-        LAUNCH("field_inv_z", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), size_t(N) * NXH, tw, NC, 0)));
-        LAUNCH("field_inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(N) * NXH, size_t(NXH), tw, NC, 0)));
+        LAUNCH("field_inv_z", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), row_stride(N), tw, NC, 0)));
+        LAUNCH("field_inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, row_stride(N), size_t(NXH), tw, NC, 0)));
         LAUNCH("field_rows_c2r", (k_rows_c2r<T, N, NC><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(d, (T*)d, tw)));
         return BFPSH_OK;
     }
@@ -221,8 +221,8 @@ struct Impl {
         CT* d = (CT*)data;
         const CT* tw = (const CT*)ctx->tw;
         LAUNCH("field_rows_r2c", (k_rows_r2c<T, N, NC><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>((const T*)d, d, tw)));
-        LAUNCH("field_fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), size_t(N) * NXH, tw, NC, 0)));
-        LAUNCH("field_fwd_y", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(N) * NXH, size_t(NXH), tw, NC, 0)));
+        LAUNCH("field_fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), row_stride(N), tw, NC, 0)));
+        LAUNCH("field_fwd_y", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, row_stride(N), size_t(NXH), tw, NC, 0)));
         return BFPSH_OK;
     }
     static int field_ift(bfpsh_ctx* ctx, void* data, int nc) {
@@ -248,10 +248,11 @@ struct Impl {
     static int pressure(bfpsh_ctx* ctx, CT* p) {
         int r = velocity_rspace(ctx);
         if (r) return r;
-        const size_t n3 = size_t(N) * N * N, ps = size_t(N) * N * NXH;
-        const unsigned gr = unsigned(std::min<size_t>((n3 + 255) / 256, 65536));
+        // products over whole real planes (pads included), k over the logical half spectrum
+        const size_t rpn = 2 * plane_size(N), ps = size_t(N) * N * NXH;
+        const unsigned gr = unsigned(std::min<size_t>((rpn + 255) / 256, 65536));
         for (int pass = 0; pass < 2; pass++) {
-            LAUNCH("pressure_uu", (k_uu<T><<<gr, 256, 0, ctx->stream>>>((const T*)ctx->ru, (T*)ctx->work, n3, pass)));
+            LAUNCH("pressure_uu", (k_uu<T><<<gr, 256, 0, ctx->stream>>>((const T*)ctx->ru, (T*)ctx->work, rpn, pass)));
             r = field_dft_nc<3>(ctx, ctx->work);
             if (r) return r;
             LAUNCH("pressure", (k_pressure<T><<<unsigned((ps + 255) / 256), 256, 0, ctx->stream>>>(
@@ -265,8 +266,8 @@ struct Impl {
     static int lagrangian_acceleration(bfpsh_ctx* ctx, CT* acc) {
         int r = ensure_work(ctx, 6);
         if (r) return r;
-        const size_t ps = size_t(N) * N * NXH;
-        CT* p = (CT*)ctx->work + 3 * ps;
+        const size_t ps = size_t(N) * N * NXH;  // logical modes per plane
+        CT* p = (CT*)ctx->work + 3 * plane_size(N);
         r = pressure(ctx, p);
         if (r) return r;
         r = spectral_velocity(ctx);
@@ -281,9 +282,10 @@ struct Impl {
     static int pressure_v(bfpsh_ctx* ctx, void* p) { return pressure(ctx, (CT*)p); }
     static int lagrangian_acceleration_v(bfpsh_ctx* ctx, void* a) { return lagrangian_acceleration(ctx, (CT*)a); }
 
-    // field::normalize of an NC-component field (all its values)
+    // field::normalize of an NC-component field (all its values; the row
+    // pads are divided too and never read)
     static int normalize(bfpsh_ctx* ctx, void* data, int nc) {
-        const size_t cnt = size_t(nc) * N * N * N;
+        const size_t cnt = size_t(nc) * 2 * plane_size(N);
         const unsigned gr = unsigned(std::min<size_t>((cnt + 255) / 256, 65536));
         LAUNCH("normalize", (k_div<T><<<gr, 256, 0, ctx->stream>>>((T*)data, cnt, T(double(N) * N * N))));
         return BFPSH_OK;
@@ -308,12 +310,12 @@ struct Impl {
         const size_t lds = size_t(nbins) * nv * sizeof(unsigned int);
         if (nc == 3) {
             LAUNCH("rspace_stats", (k_rstats<T, 4><<<dim3(nblk, 1), 256, lds, ctx->stream>>>(
-                                       (const T*)data, n3, nvals, nbins, d_me, d_part, d_hist)));
+                                       (const T*)data, N, nvals, nbins, d_me, d_part, d_hist)));
             LAUNCH("rspace_stats_final", (k_rstats_final<4><<<1, 64, 0, ctx->stream>>>(d_part, nblk, nvals,
                                                                                        double(n3), d_mom)));
         } else {
             LAUNCH("rspace_stats", (k_rstats<T, 1><<<dim3(nblk, nc), 256, lds, ctx->stream>>>(
-                                       (const T*)data, n3, nvals, nbins, d_me, d_part, d_hist)));
+                                       (const T*)data, N, nvals, nbins, d_me, d_part, d_hist)));
             LAUNCH("rspace_stats_final", (k_rstats_final<1><<<nc, 64, 0, ctx->stream>>>(d_part, nblk, nvals,
                                                                                         double(n3), d_mom)));
         }

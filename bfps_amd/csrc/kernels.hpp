@@ -3,8 +3,10 @@
 // Device data layout (one GPU holds the whole periodic box, cubic N^3):
 //   spectral plane  [ky][kz][kx],  kx = 0 .. N/2-1 stored (NXH = N/2)
 //   real plane      [y][z][x],     x  = 0 .. N-1
-// A vector field is 3 (or 6, 9) consecutive planes of PS = N*N*NXH complex
-// (spectral) or N^3 real values.  The kx = N/2 plane of the reference's
+// A vector field is 3 (or 6, 9) planes plane_size(N) complex values apart;
+// within a plane, ky (or y) rows are row_stride(N) = N*NXH + BFPSH_PAD_Y
+// complex apart (common.hpp), and a real row (y, z) occupies the bytes of
+// the spectral row (ky = y, kz = z).  The kx = N/2 plane of the reference's
 // half spectrum is not stored: it lies outside the dealiasing sphere
 // (kx^2 >= (N/2)^2 > kM^2), the reference's io_checkpoint read zeroes it
 // (low_pass, bfps/cpp/kspace.cpp:235-253) and the time step never writes it
@@ -93,6 +95,11 @@ struct Cfg {
     static_assert(NT <= 1024, "tile threads");
 };
 
+// offsets of row (y, z), row = y N + z, of a plane: complex values of a
+// spectral row and reals of a real row (the same bytes)
+template <int N> BFPSH_DEV size_t crow(size_t row) { return (row / N) * row_stride(N) + (row % N) * (N / 2); }
+template <int N> BFPSH_DEV size_t rrow(size_t row) { return (row / N) * 2 * row_stride(N) + (row % N) * N; }
+
 // ------------------------------------------------------------ parameters
 struct Geom {
     int n;
@@ -143,7 +150,7 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
     w += blockIdx.z * bstride;
     out += blockIdx.z * bstride;
     constexpr int B = CF::B_BS, NT = CF::NT_BS, E = CF::E_BS, NXH = CF::NXH;
-    constexpr size_t PS = size_t(N) * N * NXH;
+    constexpr size_t PS = plane_size(N), YS = row_stride(N);
     constexpr int NF = BsOut<MODE>::NF;
     using F = RegFft<T, N, B, B, NT, +1>;
     __shared__ C2<T> lds[N * B];
@@ -157,7 +164,7 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int n = F::in_row(grp, e);
-        const size_t off = (size_t(ky) * N + n) * NXH + kxi;
+        const size_t off = size_t(ky) * YS + size_t(n) * NXH + kxi;
 #pragma unroll
         for (int c = 0; c < 3; c++) wr[c][e] = w[c * PS + off];
     }
@@ -188,7 +195,7 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
             C2<T> a0 = wr[0][e], a1 = wr[1][e], a2 = wr[2][e];
             // symmetrized kx = 0 plane: the lower half takes conj(mirror)
             if (MIRROR && kxi == 0 && ((ky > N / 2) || (ky == 0 && n > N / 2))) {
-                const size_t moff = (size_t((N - ky) % N) * N + (N - n) % N) * NXH;
+                const size_t moff = size_t((N - ky) % N) * YS + size_t((N - n) % N) * NXH;
                 a0 = cconj<T>(w[moff]);
                 a1 = cconj<T>(w[PS + moff]);
                 a2 = cconj<T>(w[2 * PS + moff]);
@@ -210,7 +217,7 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
         F::run(v, lds, tw, gq, b);
         C2<T>* o = out + size_t(f) * PS;
 #pragma unroll
-        for (int e = 0; e < E; e++) o[(size_t(ky) * N + F::out_row(gq, e)) * NXH + kxi] = v[e];
+        for (int e = 0; e < E; e++) o[size_t(ky) * YS + size_t(F::out_row(gq, e)) * NXH + kxi] = v[e];
     }
     if constexpr (MODE == 0) {
         sfor<0, 3>([&](auto cc) {
@@ -218,7 +225,7 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
             F::run(wr[c], lds, tw, grp, b);
             C2<T>* o = out + size_t(3 + c) * PS;
 #pragma unroll
-            for (int e = 0; e < E; e++) o[(size_t(ky) * N + F::out_row(grp, e)) * NXH + kxi] = wr[c][e];
+            for (int e = 0; e < E; e++) o[size_t(ky) * YS + size_t(F::out_row(grp, e)) * NXH + kxi] = wr[c][e];
         });
     }
 }
@@ -226,14 +233,14 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
 // ------------------------------------------------------------ generic strided pass (in place)
 // element (n, b) of tile (blockIdx.x, blockIdx.y) of plane p of box q
 // (blockIdx.z = q * planes + p) at
-//   base + q * bstride + p * PS + blockIdx.y * ostride + n * estride + kx0 + b
+//   base + q * bstride + p * plane_size(N) + blockIdx.y * ostride + n * estride + kx0 + b
 template <typename T, int N, int SIGN>
 __global__ void __launch_bounds__((Cfg<T, N>::NT_PASS))
 k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __restrict__ tw, int planes,
        size_t bstride) {
     using CF = Cfg<T, N>;
     constexpr int B = CF::B, NT = CF::NT_PASS, E = CF::E_PASS, NXH = CF::NXH;
-    constexpr size_t PS = size_t(N) * N * NXH;
+    constexpr size_t PS = plane_size(N), YS = row_stride(N);
     using F = RegFft<T, N, B, B, NT, SIGN>;
     __shared__ C2<T> lds[N * B];
     const int tid = threadIdx.x;
@@ -307,7 +314,7 @@ k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restri
     using CF = Cfg<T, N>;
     // LDS row stride: odd, so lanes striding by rows hit distinct banks
     constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = 6 * RP + 1;
-    constexpr size_t PS = size_t(N) * N * NXH;
+    constexpr size_t PS = plane_size(N), YS = row_stride(N);
     __shared__ C2<T> lds[N * LS];
     // box blockIdx.y of a batch (complex planes and real velocity move together)
     data += blockIdx.y * bstride;
@@ -322,8 +329,8 @@ k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restri
 #pragma unroll
             for (int p = 0; p < RP; p++) {
                 const size_t row = size_t(q0 + p) * 2;
-                ra[c * RP + p] = data + c * PS + row * NXH;
-                rb[c * RP + p] = data + c * PS + (row + 1) * NXH;
+                ra[c * RP + p] = data + c * PS + crow<N>(row);
+                rb[c * RP + p] = data + c * PS + crow<N>(row) + NXH;
             }
         rows_unpack_inverse<T, N, LS, 6 * RP, NT>(lds, ra, rb, tid);
     }
@@ -341,8 +348,8 @@ k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restri
             const size_t row = size_t(q0 + p) * 2;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                ru[c * size_t(N) * N * N + row * N + nn] = u[c].x;
-                ru[c * size_t(N) * N * N + (row + 1) * N + nn] = u[c].y;
+                ru[c * 2 * PS + rrow<N>(row) + nn] = u[c].x;
+                ru[c * 2 * PS + rrow<N>(row) + N + nn] = u[c].y;
             }
         }
 #pragma unroll
@@ -358,8 +365,8 @@ k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restri
     for (int c = 0; c < 3; c++)
         for (int p = 0; p < RP; p++) {
             const size_t row = size_t(q0 + p) * 2;
-            rows_pack_forward<T, N, LS>(lds, c * RP + p, data + c * PS + row * NXH,
-                                        data + c * PS + (row + 1) * NXH, tid, NT);
+            rows_pack_forward<T, N, LS>(lds, c * RP + p, data + c * PS + crow<N>(row),
+                                        data + c * PS + crow<N>(row) + NXH, tid, NT);
         }
 }
 
@@ -444,7 +451,7 @@ k_rows_cross_w(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __rest
                size_t bstride) {
     using RW = RowsW<T, N>;
     constexpr int E = RW::E, G = RW::G, H = RW::H, RPB = RW::RPB, NXH = N / 2;
-    constexpr size_t PS = size_t(N) * N * NXH;
+    constexpr size_t PS = plane_size(N), YS = row_stride(N);
     static_assert(RW::ok && 64 % G == 0, "row pair inside one wave");
     __shared__ C2<T> lds[RPB * 3 * RW::SL];
     data += blockIdx.y * bstride;
@@ -454,7 +461,7 @@ k_rows_cross_w(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __rest
     const int partner = (tid & 63) - g + (G - g) % G;  // lane holding index N - k
     C2<T>* reg = lds + rp * 3 * RW::SL;
     const size_t q = size_t(blockIdx.x) * RPB + rp;  // rows 2q, 2q + 1 (row = y N + z)
-    const size_t ra = 2 * q * NXH, rb = ra + NXH;
+    const size_t ra = crow<N>(2 * q), rb = ra + NXH;
     C2<T> v[6][E];
     // unpack (as rows_unpack_inverse): Z[k] = A[k] + i B[k], Z[N/2] = 0,
     // Z[N-k] = conj(A[k]) + i conj(B[k]); rows k = g + G m, m < E / 2.  The
@@ -510,8 +517,8 @@ k_rows_cross_w(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __rest
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int x = g + G * RW::tau(e);
-                ru[c * size_t(N) * N * N + 2 * q * N + x] = v[c][e].x;
-                ru[c * size_t(N) * N * N + (2 * q + 1) * N + x] = v[c][e].y;
+                ru[c * 2 * PS + rrow<N>(2 * q) + x] = v[c][e].x;
+                ru[c * 2 * PS + rrow<N>(2 * q) + N + x] = v[c][e].y;
             }
     }
     // (u x omega) / N^3 on rows g + G tau(e), reordered to the forward FFT's
@@ -555,7 +562,7 @@ k_rows_c2r(const C2<T>* data, T* out, const C2<T>* __restrict__ tw) {
     using CF = Cfg<T, N>;
     constexpr int NG = NC < 3 ? NC : 3;
     constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = NG * RP + 1;
-    constexpr size_t PS = size_t(N) * N * NXH;
+    constexpr size_t PS = plane_size(N), YS = row_stride(N);
     static_assert(NC % NG == 0, "component groups");
     __shared__ C2<T> lds[N * LS];
     const int tid = threadIdx.x;
@@ -569,8 +576,8 @@ k_rows_c2r(const C2<T>* data, T* out, const C2<T>* __restrict__ tw) {
 #pragma unroll
                 for (int p = 0; p < RP; p++) {
                     const size_t row = size_t(q0 + p) * 2;
-                    ra[c * RP + p] = data + (c0 + c) * PS + row * NXH;
-                    rb[c * RP + p] = data + (c0 + c) * PS + (row + 1) * NXH;
+                    ra[c * RP + p] = data + (c0 + c) * PS + crow<N>(row);
+                    rb[c * RP + p] = data + (c0 + c) * PS + crow<N>(row) + NXH;
                 }
             rows_unpack_inverse<T, N, LS, NG * RP, NT>(lds, ra, rb, tid);
         }
@@ -581,8 +588,8 @@ k_rows_c2r(const C2<T>* data, T* out, const C2<T>* __restrict__ tw) {
             const int c = col / RP, p = col % RP;
             const size_t row = size_t(q0 + p) * 2;
             const C2<T> v = lds[nn * LS + col];
-            out[(c0 + c) * size_t(N) * N * N + row * N + nn] = v.x;
-            out[(c0 + c) * size_t(N) * N * N + (row + 1) * N + nn] = v.y;
+            out[(c0 + c) * 2 * PS + rrow<N>(row) + nn] = v.x;
+            out[(c0 + c) * 2 * PS + rrow<N>(row) + N + nn] = v.y;
         }
         __syncthreads();
     }
@@ -596,7 +603,7 @@ k_rows_r2c(const T* data, C2<T>* out, const C2<T>* __restrict__ tw) {
     using CF = Cfg<T, N>;
     constexpr int NG = NC < 3 ? NC : 3;
     constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = NG * RP + 1;
-    constexpr size_t PS = size_t(N) * N * NXH;
+    constexpr size_t PS = plane_size(N), YS = row_stride(N);
     static_assert(NC % NG == 0, "component groups");
     __shared__ C2<T> lds[N * LS];
     const int tid = threadIdx.x;
@@ -606,16 +613,16 @@ k_rows_r2c(const T* data, C2<T>* out, const C2<T>* __restrict__ tw) {
             const int col = e / N, nn = e % N;
             const int c = col / RP, p = col % RP;
             const size_t row = size_t(q0 + p) * 2;
-            lds[nn * LS + col] = mk<T>(data[(c0 + c) * size_t(N) * N * N + row * N + nn],
-                                       data[(c0 + c) * size_t(N) * N * N + (row + 1) * N + nn]);
+            lds[nn * LS + col] = mk<T>(data[(c0 + c) * 2 * PS + rrow<N>(row) + nn],
+                                       data[(c0 + c) * 2 * PS + rrow<N>(row) + N + nn]);
         }
         __syncthreads();
         fft_tile<T, N, NG * RP, LS, NT, CF::RMAX, -1>(lds, tw, tid);
         for (int c = 0; c < NG; c++)
             for (int p = 0; p < RP; p++) {
                 const size_t row = size_t(q0 + p) * 2;
-                rows_pack_forward<T, N, LS>(lds, c * RP + p, out + (c0 + c) * PS + row * NXH,
-                                            out + (c0 + c) * PS + (row + 1) * NXH, tid, NT);
+                rows_pack_forward<T, N, LS>(lds, c * RP + p, out + (c0 + c) * PS + crow<N>(row),
+                                            out + (c0 + c) * PS + crow<N>(row) + NXH, tid, NT);
             }
         __syncthreads();
     }
@@ -646,7 +653,7 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
     }
     const StepParams sp = BATCH ? spb[blockIdx.z] : sp0;
     constexpr int B = CF::B_Y3, NT = CF::NT_Y3, E = CF::E_Y3, NXH = CF::NXH;
-    constexpr size_t PS = size_t(N) * N * NXH;
+    constexpr size_t PS = plane_size(N), YS = row_stride(N);
     using F = RegFft<T, N, B, B, NT, -1>;
     __shared__ C2<T> lds[N * B];
     const int tid = threadIdx.x;
@@ -658,7 +665,7 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
     for (int c = 0; c < 3; c++) {
         const C2<T>* src = data + c * PS + size_t(kz) * NXH + kxi;
 #pragma unroll
-        for (int e = 0; e < E; e++) res[c][e] = src[size_t(F::in_row(grp, e)) * N * NXH];
+        for (int e = 0; e < E; e++) res[c][e] = src[size_t(F::in_row(grp, e)) * YS];
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) F::run(res[c], lds, tw, grp, b);
@@ -668,7 +675,7 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
         const int ky = F::out_row(grp, i);
         const double kyv = kwave(ky, N, g.dky);
         const double k2 = kx * kx + kyv * kyv + kzv * kzv;
-        const size_t off = (size_t(ky) * N + kz) * NXH + kxi;
+        const size_t off = size_t(ky) * YS + size_t(kz) * NXH + kxi;
         // dealias
         C2<T> a[3];
         if (g.smooth) {
@@ -790,7 +797,7 @@ __global__ void __launch_bounds__(256) k_band_amplitude(const C2<T>* __restrict_
         fk1 = p.fk1;
     }
     constexpr int NXH = N / 2;
-    constexpr size_t PS = size_t(N) * N * NXH;
+    constexpr size_t PS = plane_size(N), YS = row_stride(N);
     __shared__ double part[256];
     const int KX = min(NXH - 1, int(fk1 / g.dkx));
     const int KY = min(N / 2 - 1, int(fk1 / g.dky)), KZ = min(N / 2 - 1, int(fk1 / g.dkz));
@@ -803,7 +810,7 @@ __global__ void __launch_bounds__(256) k_band_amplitude(const C2<T>* __restrict_
         const double k2 = kx * kx + ky * ky + kz * kz;
         const double kn = sqrt(k2);
         if (!(fk0 <= kn && fk1 >= kn && k2 > 0)) continue;
-        const size_t off = (size_t(yy < 0 ? yy + N : yy) * N + (zz < 0 ? zz + N : zz)) * NXH + x;
+        const size_t off = size_t(yy < 0 ? yy + N : yy) * YS + size_t(zz < 0 ? zz + N : zz) * NXH + x;
         double m = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -832,7 +839,7 @@ template <typename T>
 __global__ void k_symmetrize(C2<T>* v, int n, int nc, size_t bstride) {
     v += blockIdx.y * bstride;  // box blockIdx.y of a batch
     const int nxh = n / 2;
-    const size_t ps = size_t(n) * n * nxh;
+    const size_t ps = plane_size(n), ys = row_stride(n);
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * n) return;
     const int ky = idx / n, kz = idx % n;
@@ -841,8 +848,8 @@ __global__ void k_symmetrize(C2<T>* v, int n, int nc, size_t bstride) {
         if (ky == 0 && kz == 0) {
             p[0].y = T(0);
         } else if ((ky > n / 2) || (ky == 0 && kz > n / 2)) {
-            const C2<T> m = p[(size_t((n - ky) % n) * n + (n - kz) % n) * nxh];
-            p[(size_t(ky) * n + kz) * nxh] = cconj<T>(m);
+            const C2<T> m = p[size_t((n - ky) % n) * ys + size_t((n - kz) % n) * nxh];
+            p[size_t(ky) * ys + size_t(kz) * nxh] = cconj<T>(m);
         }
     }
 }
@@ -852,10 +859,11 @@ __global__ void k_symmetrize(C2<T>* v, int n, int nc, size_t bstride) {
 template <typename T>
 __global__ void k_lowpass_divfree(C2<T>* v, Geom g) {
     const int n = g.n, nxh = n / 2;
-    const size_t ps = size_t(n) * n * nxh;
-    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= ps) return;
-    const int kxi = idx % nxh, kz = (idx / nxh) % n, ky = idx / (size_t(nxh) * n);
+    const size_t ps = plane_size(n);
+    const size_t lidx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lidx >= size_t(n) * n * nxh) return;
+    const size_t idx = sphys(n, lidx);
+    const int kxi = lidx % nxh, kz = (lidx / nxh) % n, ky = lidx / (size_t(nxh) * n);
     const double kx = kxi * g.dkx, kyv = kwave(ky, n, g.dky), kzv = kwave(kz, n, g.dkz);
     const double k2 = kx * kx + kyv * kyv + kzv * kzv;
     C2<T> a[3];
@@ -878,10 +886,11 @@ __global__ void k_lowpass_divfree(C2<T>* v, Geom g) {
 template <typename T>
 __global__ void k_biot_savart(const C2<T>* __restrict__ w, C2<T>* __restrict__ u, Geom g) {
     const int n = g.n, nxh = n / 2;
-    const size_t ps = size_t(n) * n * nxh;
-    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= ps) return;
-    const int kxi = idx % nxh, kz = (idx / nxh) % n, ky = idx / (size_t(nxh) * n);
+    const size_t ps = plane_size(n);
+    const size_t lidx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lidx >= size_t(n) * n * nxh) return;
+    const size_t idx = sphys(n, lidx);
+    const int kxi = lidx % nxh, kz = (lidx / nxh) % n, ky = lidx / (size_t(nxh) * n);
     const double kx = kxi * g.dkx, kyv = kwave(ky, n, g.dky), kzv = kwave(kz, n, g.dkz);
     const double k2 = kx * kx + kyv * kyv + kzv * kzv;
     const C2<T> a0 = w[idx], a1 = w[ps + idx], a2 = w[2 * ps + idx];
@@ -902,11 +911,12 @@ __global__ void k_cospectrum(const C2<T>* __restrict__ a, const C2<T>* __restric
                              Geom g, double dk, int nshells) {
     extern __shared__ double acc[];
     const int n = g.n, nxh = n / 2;
-    const size_t ps = size_t(n) * n * nxh;
+    const size_t ps = plane_size(n), nlog = size_t(n) * n * nxh;
     for (int i = threadIdx.x; i < nshells * 9; i += blockDim.x) acc[i] = 0.0;
     __syncthreads();
-    for (size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < ps; idx += size_t(gridDim.x) * blockDim.x) {
-        const int kxi = idx % nxh, kz = (idx / nxh) % n, ky = idx / (size_t(nxh) * n);
+    for (size_t lidx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; lidx < nlog; lidx += size_t(gridDim.x) * blockDim.x) {
+        const int kxi = lidx % nxh, kz = (lidx / nxh) % n, ky = lidx / (size_t(nxh) * n);
+        const size_t idx = sphys(n, lidx);
         const double kx = kxi * g.dkx, kyv = kwave(ky, n, g.dky), kzv = kwave(kz, n, g.dkz);
         const double k2 = kx * kx + kyv * kyv + kzv * kzv;
         if (k2 > g.kM2) continue;

@@ -137,7 +137,8 @@ __device__ __forceinline__ void gather_coop(const T* __restrict__ f, int n, cons
         v = v < 0 ? v + n : (v >= n ? v - n : v);
         ixl = v;
     }
-    const size_t ps = size_t(n) * n * n;
+    // real planes 2 plane_size(n) reals apart, y rows 2 row_stride(n) apart
+    const size_t ps = 2 * plane_size(n), ys = 2 * row_stride(n);
 #pragma unroll
     for (int c = 0; c < NC; c++) acc[c] = 0.0;
     if (jx < W) {
@@ -147,7 +148,7 @@ __device__ __forceinline__ void gather_coop(const T* __restrict__ f, int n, cons
             for (int c = 0; c < NC; c++) col[c] = 0.0;
 #pragma unroll 2
             for (int jy = 0; jy < W; jy++) {
-                const size_t pt = (size_t(iy[jy]) * n + iz[jz]) * n + ixl;
+                const size_t pt = size_t(iy[jy]) * ys + size_t(iz[jz]) * n + ixl;
 #pragma unroll
                 for (int c = 0; c < NC; c++) col[c] = fma(by[jy], double(f[c * ps + pt]), col[c]);
             }
@@ -407,7 +408,10 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
     const int bz = int(brick / (int64_t(nbx) * nby)), by = int((brick / nbx) % nby), bx = int(brick % nbx);
     const int x0 = bx * BC::KX - NB, y0 = by * BC::KY - NB, z0 = bz * BC::KZ - NB;
     const int base[3] = {x0 + NB, y0 + NB, z0 + NB};
-    const size_t ps = size_t(n) * n * n;
+    // real planes 2 plane_size(n) reals apart, y rows ys reals apart (the
+    // offsets within a plane fit 32 bits for n <= 1024)
+    const size_t ps = 2 * plane_size(n);
+    const uint32_t ys = uint32_t(2 * row_stride(n));
     for (uint32_t bs = s0; bs < e0; bs += PB) {
         const int cnt = int(min(uint32_t(PB), e0 - bs));
         __syncthreads();  // previous batch done with frac / lcell / res
@@ -450,7 +454,7 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
                         gx = gx < 0 ? gx + n : (gx >= n ? gx - n : gx);
                         gy = gy < 0 ? gy + n : (gy >= n ? gy - n : gy);
                         gz = gz < 0 ? gz + n : (gz >= n ? gz - n : gz);
-                        vals[k] = fc[(uint32_t(gy) * uint32_t(n) + uint32_t(gz)) * uint32_t(n) + uint32_t(gx)];
+                        vals[k] = fc[uint32_t(gy) * ys + uint32_t(gz) * uint32_t(n) + uint32_t(gx)];
                     }
                 }
 #pragma unroll

@@ -10,7 +10,8 @@
 
 namespace bfpsh {
 
-// Products of the real velocity (planar [c][y][z][x], n3 points per plane)
+// Products of the real velocity (planar [c][y][z][x], n3 reals per plane
+// including the row pads, whose products are never read)
 // for the pressure: pass 0 the diagonal u_c u_c, pass 1 u_c u_{c+1}
 // (vorticity_equation.cpp:451-459,484-492), in the field precision.
 template <typename T>
@@ -37,10 +38,11 @@ __global__ void k_uu(const T* __restrict__ u, T* __restrict__ out, size_t n3, in
 template <typename T>
 __global__ void k_pressure(const C2<T>* __restrict__ f, C2<T>* __restrict__ p, Geom g, int pass) {
     const int n = g.n, nxh = n / 2;
-    const size_t ps = size_t(n) * n * nxh;
-    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= ps) return;
-    const int kxi = idx % nxh, kz = (idx / nxh) % n, ky = idx / (size_t(nxh) * n);
+    const size_t ps = plane_size(n);
+    const size_t lidx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lidx >= size_t(n) * n * nxh) return;
+    const size_t idx = sphys(n, lidx);
+    const int kxi = lidx % nxh, kz = (lidx / nxh) % n, ky = lidx / (size_t(nxh) * n);
     const double kx = kxi * g.dkx, kyv = kwave(ky, n, g.dky), kzv = kwave(kz, n, g.dkz);
     const double k2 = kx * kx + kyv * kyv + kzv * kzv;
     if (!(k2 <= g.kM2 && k2 > 0)) {
@@ -81,10 +83,11 @@ template <typename T>
 __global__ void k_lagr_acc(const C2<T>* __restrict__ u, const C2<T>* __restrict__ p, C2<T>* __restrict__ acc,
                            Geom g, double nu, int linear, double famp, double fk0, double fk1) {
     const int n = g.n, nxh = n / 2;
-    const size_t ps = size_t(n) * n * nxh;
-    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= ps) return;
-    const int kxi = idx % nxh, kz = (idx / nxh) % n, ky = idx / (size_t(nxh) * n);
+    const size_t ps = plane_size(n);
+    const size_t lidx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lidx >= size_t(n) * n * nxh) return;
+    const size_t idx = sphys(n, lidx);
+    const int kxi = lidx % nxh, kz = (lidx / nxh) % n, ky = lidx / (size_t(nxh) * n);
     const double kv[3] = {kxi * g.dkx, kwave(ky, n, g.dky), kwave(kz, n, g.dkz)};
     const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
     C2<T> a[3] = {mk<T>(T(0), T(0)), mk<T>(T(0), T(0)), mk<T>(T(0), T(0))};
@@ -128,7 +131,7 @@ __global__ void k_div(T* __restrict__ d, size_t count, T npoints) {
 // Each workgroup writes its partial moments to part[group][block][10][NV].
 template <typename T, int NV>
 __global__ void __launch_bounds__(256)
-k_rstats(const T* __restrict__ d, size_t n3, int nvals, int nbins, const double* __restrict__ maxest,
+k_rstats(const T* __restrict__ d, int n, int nvals, int nbins, const double* __restrict__ maxest,
          double* __restrict__ part, unsigned long long* __restrict__ hist) {
     // every product and sum rounded on its own, as the reference's loop (no
     // fused multiply-adds)
@@ -155,13 +158,17 @@ k_rstats(const T* __restrict__ d, size_t n3, int nvals, int nbins, const double*
     // point is processed (the per-point work is short against the memory
     // latency at the occupancy the FP64 accumulators allow)
     constexpr int PF = 4;
+    // logical point i = (y n + z) n + x of a plane at reals rphys(n, i);
+    // planes cps reals apart
+    const size_t n3 = size_t(n) * n * n, cps = 2 * plane_size(n);
     const size_t stride = size_t(gridDim.x) * blockDim.x;
     const size_t i0 = size_t(blockIdx.x) * blockDim.x + tid;
     T buf[PF][NCV];
 #pragma unroll
     for (int p = 0; p < PF; p++)
 #pragma unroll
-        for (int c = 0; c < NCV; c++) buf[p][c] = i0 + p * stride < n3 ? d[size_t(c0 + c) * n3 + i0 + p * stride] : T(0);
+        for (int c = 0; c < NCV; c++)
+            buf[p][c] = i0 + p * stride < n3 ? d[size_t(c0 + c) * cps + rphys(n, i0 + p * stride)] : T(0);
     for (size_t i = i0; i < n3; i += stride) {
         double val[NV];
 #pragma unroll
@@ -172,7 +179,7 @@ k_rstats(const T* __restrict__ d, size_t n3, int nvals, int nbins, const double*
             for (int c = 0; c < NCV; c++) buf[p][c] = buf[p + 1][c];
         const size_t inext = i + PF * stride;
 #pragma unroll
-        for (int c = 0; c < NCV; c++) buf[PF - 1][c] = inext < n3 ? d[size_t(c0 + c) * n3 + inext] : T(0);
+        for (int c = 0; c < NCV; c++) buf[PF - 1][c] = inext < n3 ? d[size_t(c0 + c) * cps + rphys(n, inext)] : T(0);
         if (NV == 4) {
             // |u| = sqrt(((0 + u0^2) + u1^2) + u2^2)
             double m2 = val[0] * val[0];
