@@ -22,6 +22,11 @@
 #ifndef BFPSH_TILE_BYTES
 #define BFPSH_TILE_BYTES 128
 #endif
+// LDS bytes a tile of the plain strided pass may take (B columns x N
+// complex values; gfx950 has 160 KB of LDS per CU)
+#ifndef BFPSH_TILE_LDS
+#define BFPSH_TILE_LDS 131072
+#endif
 // register-resident x pass (k_rows_cross_w): minimum waves per SIMD and
 // LDS padding (one slot every 2^BFPSH_RW_PAD rows); measured at 512^3 FP32,
 // 3 waves with padding every 16 rows beat 2 waves with padding every 8
@@ -54,16 +59,21 @@ struct Cfg {
     static constexpr int E = ereg();
     static constexpr int G = N / E;  // thread groups per column
     // columns per strided tile: BFPSH_TILE_BYTES-byte row segments (default
-    // 128), tile <= 64 KB of LDS
-    static constexpr int bcap() {
+    // 128) in at most lds bytes of LDS
+    static constexpr int bcap(int lds) {
         int b = int(BFPSH_TILE_BYTES / sizeof(C2<T>));
-        const int lim = int(65536 / (N * sizeof(C2<T>)));
+        const int lim = int(lds / (N * sizeof(C2<T>)));
         if (b > lim) b = lim;
         if (b > NXH) b = NXH;
         while (NXH % b) b /= 2;
         return b;
     }
-    static constexpr int B = bcap();
+    // plain strided pass (k_pass): up to BFPSH_TILE_LDS bytes, which only
+    // widens the 1024^3 tiles (FP32 8 -> 16 columns: one y pass over a plane
+    // 2.97 -> 2.46 ms); the fused first and last passes keep 64 KB tiles
+    // (their three components in registers need the smaller workgroups)
+    static constexpr int B = bcap(BFPSH_TILE_LDS);
+    static constexpr int BF = bcap(65536);
     static constexpr int NT = G * B;
     // Biot-Savart pass: three source components stay in registers; half the
     // columns (when NT would exceed 512) keep it within a 256-VGPR budget
@@ -72,10 +82,10 @@ struct Cfg {
 #endif
     static constexpr int E_BS = (sizeof(T) == 4 && N == 512 && BFPSH_E_BS > 0 && N % BFPSH_E_BS == 0) ? BFPSH_E_BS : E;
     static constexpr int G_BS = N / E_BS;
-    static constexpr int B_BS = (G_BS * B > 512 && B > 1) ? B / 2 : B;
+    static constexpr int B_BS = (G_BS * BF > 512 && BF > 1) ? BF / 2 : BF;
     static constexpr int NT_BS = G_BS * B_BS;
     // last forward pass with the update epilogue (3 components in registers)
-    static constexpr int B_Y3 = B;
+    static constexpr int B_Y3 = BF;
     // elements per thread of the plain strided pass (measured at 512^3 FP32:
     // 16 elements x 512 threads beat 8 x 1024) and of the last forward pass
     // (E; the register budget of its epilogue favours more threads)
@@ -85,10 +95,10 @@ struct Cfg {
 #ifndef BFPSH_E_Y3
 #define BFPSH_E_Y3 0
 #endif
-    static constexpr int pick_e(int e) { return (e > 0 && N % e == 0 && (N / e) * B <= 1024) ? e : E; }
-    static constexpr int E_PASS = pick_e(sizeof(T) == 4 && N == 512 ? BFPSH_E_PASS : 0);
+    static constexpr int pick_e(int e, int b) { return (e > 0 && N % e == 0 && (N / e) * b <= 1024) ? e : E; }
+    static constexpr int E_PASS = pick_e(sizeof(T) == 4 && N == 512 ? BFPSH_E_PASS : 0, B);
     static constexpr int NT_PASS = (N / E_PASS) * B;
-    static constexpr int E_Y3 = pick_e(sizeof(T) == 4 && N == 512 ? BFPSH_E_Y3 : 0);
+    static constexpr int E_Y3 = pick_e(sizeof(T) == 4 && N == 512 ? BFPSH_E_Y3 : 0, B_Y3);
     static constexpr int NT_Y3 = (N / E_Y3) * B_Y3;
     static constexpr int RP = (N * int(sizeof(C2<T>)) >= 8192) ? 1 : 2;  // row pairs per row block
     static constexpr int NT_ROW = N >= 256 ? 256 : (N >= 64 ? 128 : 64);
