@@ -668,8 +668,11 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
     __shared__ C2<T> lds[N * B];
     const int tid = threadIdx.x;
     const int b = tid % B, grp = tid / B;
-    const int kxi = blockIdx.x * B + b;
-    const int kz = blockIdx.y;
+    // tiles narrower than a 128-byte line (1024^3): neighbouring tiles, the
+    // two halves of each line, run on the same XCD and share its L2
+    int tile = blockIdx.x, kz = blockIdx.y;
+    if constexpr (B * sizeof(C2<T>) < 128) xcd_tile(tile, kz);
+    const int kxi = tile * B + b;
     C2<T> res[3][E];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
