@@ -101,7 +101,9 @@ struct Cfg {
     static constexpr int E_Y3 = pick_e(sizeof(T) == 4 && N == 512 ? BFPSH_E_Y3 : 0, B_Y3);
     static constexpr int NT_Y3 = (N / E_Y3) * B_Y3;
     static constexpr int RP = (N * int(sizeof(C2<T>)) >= 8192) ? 1 : 2;  // row pairs per row block
-    static constexpr int NT_ROW = N >= 256 ? 256 : (N >= 64 ? 128 : 64);
+    // threads of the LDS-tile row passes (1024: two blocks of 512 per CU,
+    // four waves per SIMD, instead of two blocks of 256)
+    static constexpr int NT_ROW = N >= 1024 ? 512 : (N >= 256 ? 256 : (N >= 64 ? 128 : 64));
     static_assert(NT <= 1024, "tile threads");
 };
 
