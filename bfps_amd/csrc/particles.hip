@@ -299,7 +299,10 @@ struct BrickCfg {
     static constexpr int W = 2 * NB + 2;
     static constexpr int KX = sizeof(T) == 4 ? 16 : 8;
     static constexpr int KY = sizeof(T) == 4 ? 16 : 8;
-    static constexpr int KZ = 8;
+    // 16 planes deep for FP32 stencils up to 10 wide (n <= 4): the tile
+    // (at most 25 x 25 x 25 floats, 62.5 KB) still lets two bricks share a
+    // CU and the halo per brick is amortised over twice the tracers
+    static constexpr int KZ = (sizeof(T) == 4 && W <= 10) ? 16 : 8;
     static constexpr int WX = KX + W - 1, WY = KY + W - 1, WZ = KZ + W - 1;
     static constexpr int TILE = WX * WY * WZ;
     static constexpr size_t TILE_BYTES = size_t(TILE) * sizeof(T);
