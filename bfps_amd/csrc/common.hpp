@@ -43,11 +43,6 @@ __host__ __device__ inline size_t sphys(int n, size_t idx) {
     const uint32_t s = uint32_t(n) * uint32_t(n / 2), i = uint32_t(idx);
     return size_t(i / s) * row_stride(n) + i % s;
 }
-// physical offset (reals) of the logical real index (y*N + z)*N + x
-__host__ __device__ inline size_t rphys(int n, size_t idx) {
-    const uint32_t s = uint32_t(n) * uint32_t(n), i = uint32_t(idx);
-    return size_t(i / s) * 2 * row_stride(n) + i % s;
-}
 
 // Wave-size facts for CDNA4.
 constexpr int WAVE = 64;

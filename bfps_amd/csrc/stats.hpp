@@ -158,17 +158,32 @@ k_rstats(const T* __restrict__ d, int n, int nvals, int nbins, const double* __r
     // point is processed (the per-point work is short against the memory
     // latency at the occupancy the FP64 accumulators allow)
     constexpr int PF = 4;
-    // logical point i = (y n + z) n + x of a plane at reals rphys(n, i);
-    // planes cps reals apart
-    const size_t n3 = size_t(n) * n * n, cps = 2 * plane_size(n);
+    // logical point j = y n^2 + (z n + x) of a plane lies at reals
+    // y ys2 + (z n + x); planes cps reals apart.  The load cursor keeps
+    // (y, z n + x) of the next point to load and advances by the grid
+    // stride without a division per point.
+    const size_t n3 = size_t(n) * n * n, cps = 2 * plane_size(n), ys2 = 2 * row_stride(n);
+    const uint32_t n2 = uint32_t(n) * uint32_t(n);
     const size_t stride = size_t(gridDim.x) * blockDim.x;
     const size_t i0 = size_t(blockIdx.x) * blockDim.x + tid;
+    const uint32_t sq = uint32_t(stride / n2), sr = uint32_t(stride % n2);
+    uint32_t lq = uint32_t(i0 / n2), lr = uint32_t(i0 % n2);
+    size_t lj = i0;  // logical index at the cursor
+    auto load_next = [&](T (&dst)[NCV]) {
+        const size_t off = size_t(lq) * ys2 + lr;
+#pragma unroll
+        for (int c = 0; c < NCV; c++) dst[c] = lj < n3 ? d[size_t(c0 + c) * cps + off] : T(0);
+        lj += stride;
+        lq += sq;
+        lr += sr;
+        if (lr >= n2) {
+            lr -= n2;
+            lq++;
+        }
+    };
     T buf[PF][NCV];
 #pragma unroll
-    for (int p = 0; p < PF; p++)
-#pragma unroll
-        for (int c = 0; c < NCV; c++)
-            buf[p][c] = i0 + p * stride < n3 ? d[size_t(c0 + c) * cps + rphys(n, i0 + p * stride)] : T(0);
+    for (int p = 0; p < PF; p++) load_next(buf[p]);
     for (size_t i = i0; i < n3; i += stride) {
         double val[NV];
 #pragma unroll
@@ -177,9 +192,7 @@ k_rstats(const T* __restrict__ d, int n, int nvals, int nbins, const double* __r
         for (int p = 0; p + 1 < PF; p++)
 #pragma unroll
             for (int c = 0; c < NCV; c++) buf[p][c] = buf[p + 1][c];
-        const size_t inext = i + PF * stride;
-#pragma unroll
-        for (int c = 0; c < NCV; c++) buf[PF - 1][c] = inext < n3 ? d[size_t(c0 + c) * cps + rphys(n, inext)] : T(0);
+        load_next(buf[PF - 1]);  // the point PF strides ahead
         if (NV == 4) {
             // |u| = sqrt(((0 + u0^2) + u1^2) + u2^2)
             double m2 = val[0] * val[0];
