@@ -145,6 +145,75 @@ def read_iteration(simname, work_dir="."):
         return int(f["iteration"]), int(f["checkpoint"])
 
 
+def copy_complex_field(src_path, src_name, dst, dst_name, n, ctype):
+    """DNS.copy_complex_field (bfps/DNS.py:810-853): the field
+    [nz][ny][nx/2+1][3] of src_path:src_name into dst[dst_name] at n^3, cast
+    to ctype; a source of another resolution is cut to, or zero-padded
+    into, the leading [min][min][min] block of every axis (the reference's
+    rule, which it trusts only for equal factors on all axes).  The
+    reference's external link for an equal shape and type becomes a copy."""
+    with h5io.File(src_path) as f:
+        src = f[src_name]
+    shape = (n, n, n // 2 + 1, 3)
+    if src.shape == shape:
+        dst[dst_name] = src.astype(ctype)
+        return
+    out = np.zeros(shape, ctype)
+    m = tuple(min(a, b) for a, b in zip(shape[:3], src.shape[:3]))
+    out[:m[0], :m[1], :m[2]] = src[:m[0], :m[1], :m[2]]
+    dst[dst_name] = out
+
+
+def find_checkpoint(simname, work_dir, iteration):
+    """The first <simname>_checkpoint_<k>.h5 holding vorticity/complex/<it>
+    (the search of DNS.launch_jobs, bfps/DNS.py:864-874)."""
+    k = 0
+    while True:
+        path = os.path.join(work_dir, "%s_checkpoint_%d.h5" % (simname, k))
+        if not os.path.exists(path):
+            raise FileNotFoundError("no checkpoint of %s holds iteration %d" % (simname, iteration))
+        with h5io.File(path) as f:
+            if "vorticity/complex/%d" % iteration in f:
+                return path
+        k += 1
+
+
+def launch_jobs(simname, parameters, dns_type="NSVEparticles", work_dir=".", src_simname=None, src_work_dir=".",
+                src_iteration=0, particle_rand_seed=None, precision="single"):
+    """DNS.launch_jobs (bfps/DNS.py:855-926) up to the job submission: when
+    <simname>.h5 does not exist yet, checkpoint 0 gets the initial vorticity
+    -- copied from iteration src_iteration of the run src_simname in
+    src_work_dir (``--src-simname``, ``--src-wd``, ``--src-iteration``) --
+    and, for the particle DNS types, tracers0/state/0 ([np][3]) and
+    tracers0/rhs/0 ([nsteps][np][3], zero) with the state drawn by
+    generate_tracer_state(particle_rand_seed) (DNS.write_par,
+    DNS.py:420-452; DNS.generate_tracer_state, DNS.py:711-733); then the
+    run file (write_par) and the tracer-sample file.  Returns the driver,
+    not yet initialised.  The reference's random vector-field generator
+    (used when no source run is named) is not part of this build: without
+    src_simname, checkpoint 0 must already hold vorticity/complex/0."""
+    p = parameters
+    ckpt0 = os.path.join(work_dir, simname + "_checkpoint_0.h5")
+    if not os.path.exists(os.path.join(work_dir, simname + ".h5")):
+        os.makedirs(work_dir, exist_ok=True)
+        ctype = np.complex64 if precision == "single" else np.complex128
+        with h5io.File(ckpt0, "a") as f:
+            if "vorticity/complex/0" not in f:
+                if src_simname is None:
+                    raise ValueError("no initial field: name a source run (src_simname) or write "
+                                     "vorticity/complex/0 into %s" % ckpt0)
+                src = find_checkpoint(src_simname, src_work_dir, src_iteration)
+                copy_complex_field(src, "vorticity/complex/%d" % src_iteration, f, "vorticity/complex/0",
+                                   p["nx"], ctype)
+            particles = dns_type in ("NSVEparticles", "NSVEparticles_no_output") and p.get("nparticles", 0) > 0
+            if particles:
+                np_ = p["nparticles"]
+                f["tracers0/rhs/0"] = np.zeros((p["tracers0_integration_steps"], np_, 3))
+                f["tracers0/state/0"] = generate_tracer_state(np_, rseed=particle_rand_seed)
+        write_par(simname, p, dns_type=dns_type, work_dir=work_dir)
+    return NSVEparticles(simname, p, precision=precision, work_dir=work_dir)
+
+
 def generate_tracer_state(nparticles, rseed=None, batch_size=10 ** 6):
     """DNS.generate_tracer_state (bfps/DNS.py:711-733): uniform positions in
     [0, 2 pi)^3 from numpy's legacy generator (seeded with rseed when
@@ -322,4 +391,4 @@ class NSVEparticles:
 
 
 __all__ = ["default_parameters", "launch_parameters", "kspace_info", "write_par", "read_iteration",
-           "generate_tracer_state", "NSVEparticles"]
+           "copy_complex_field", "find_checkpoint", "launch_jobs", "generate_tracer_state", "NSVEparticles"]

@@ -85,3 +85,37 @@ def test_write_par_run_file(tmp_path):
     bad = dict(p, niter_out=8, niter_stat=3)
     with pytest.raises(AssertionError):
         dns.write_par("bad", bad, work_dir=str(tmp_path))
+
+
+def test_launch_jobs_from_source_run(tmp_path, golden):
+    # DNS.launch_jobs with --src-simname B32p1e4 --src-iteration 0
+    # --particle-rand-seed 2 (the reference's test_bfps_NSVEparticles.py):
+    # checkpoint 0 holds the source field, the seeded tracer state and a
+    # zero history; the run file says iteration 0, checkpoint 0
+    import os
+    from bfps_amd import h5io
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    p = dns.launch_parameters(32, nparticles=10000, niter_todo=32, niter_out=32, niter_stat=1,
+                              checkpoints_per_file=3)
+    run = dns.launch_jobs("dns_nsveparticles", p, work_dir=str(tmp_path), src_simname="B32p1e4",
+                          src_work_dir=gold, src_iteration=0, particle_rand_seed=2)
+    assert (run.iteration0, run.checkpoint0) == (0, 0)
+    with h5io.File(os.path.join(str(tmp_path), "dns_nsveparticles_checkpoint_0.h5")) as f:
+        w = f["vorticity/complex/0"]
+        assert w.dtype == np.complex64 and np.array_equal(w, golden["vort_0"])
+        assert np.array_equal(f["tracers0/state/0"], golden["state_0"])
+        assert f["tracers0/rhs/0"].shape == (4, 10000, 3) and not f["tracers0/rhs/0"].any()
+    # a second launch of the same run changes nothing (the run file exists)
+    dns.launch_jobs("dns_nsveparticles", p, work_dir=str(tmp_path))
+    # another resolution: the source's leading block, zero elsewhere
+    q = dns.launch_parameters(48, nparticles=10, niter_todo=8, niter_out=8)
+    dns.launch_jobs("up", q, work_dir=str(tmp_path), src_simname="B32p1e4", src_work_dir=gold,
+                    particle_rand_seed=1)
+    with h5io.File(os.path.join(str(tmp_path), "up_checkpoint_0.h5")) as f:
+        w = f["vorticity/complex/0"]
+    assert w.shape == (48, 48, 25, 3)
+    assert np.array_equal(w[:32, :32, :17], golden["vort_0"]) and not w[32:].any() and not w[:, 32:].any()
+    # without a source the initial field must already be there
+    import pytest
+    with pytest.raises(ValueError):
+        dns.launch_jobs("none", q, work_dir=str(tmp_path / "x"))

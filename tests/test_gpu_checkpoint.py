@@ -162,3 +162,33 @@ def test_driver_main_loop_and_stop_file(B, tmp_path):
     with h5io.File(os.path.join(wd, "run_checkpoint_0.h5")) as f:
         assert f.keys("vorticity/complex") == ["0", "32", "33"]
         assert "tracers0/rhs/33" in f
+
+
+def test_reference_test_flow_through_launch_jobs(B, tmp_path):
+    # bfps/test/test_bfps_NSVEparticles.py through this package's launcher:
+    # two jobs of 32 iterations of a run launched from the golden file's
+    # iteration 0 (--src-simname B32p1e4 --src-iteration 0
+    # --particle-rand-seed 2 --niter_stat 1 --checkpoints_per_file 3), then
+    # the reference's comparison of iterations 0, 32 and 64 at 1e-5
+    from bfps_amd import dns, h5io
+    say("case: reference test flow through launch_jobs (2 jobs x 32 iterations)")
+    wd = str(tmp_path)
+    gold = os.path.dirname(REF)
+    p = dns.launch_parameters(32, nparticles=10000, niter_todo=32, niter_out=32, niter_stat=1, niter_part=32,
+                              checkpoints_per_file=3)
+    for job in range(2):
+        run = dns.launch_jobs("dns_nsveparticles", p, work_dir=wd, src_simname="B32p1e4", src_work_dir=gold,
+                              src_iteration=0, particle_rand_seed=2)
+        run.initialize()
+        run.main_loop()
+        it = run.iteration
+        run.finalize()
+        say("  job %d done at iteration %d" % (job, it))
+        assert it == 32 * (job + 1)
+    with h5io.File(REF) as f0, h5io.File(os.path.join(wd, "dns_nsveparticles_checkpoint_0.h5")) as f1:
+        for it in (0, 32, 64):
+            for name in ("vorticity/complex/%d", "tracers0/state/%d", "tracers0/rhs/%d"):
+                err = np.abs(f0[name % it] - f1[name % it]).max()
+                assert err < 1e-5, (name % it, err)
+    with h5io.File(os.path.join(wd, "dns_nsveparticles.h5")) as f:
+        assert int(f["iteration"]) == 64 and f["statistics/moments/velocity"].shape == (65, 10, 4)
