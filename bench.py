@@ -408,6 +408,14 @@ def main():
         "wall_ms_per_step": r32["wall_ms_per_step"],
         "device_gb": r32["device_bytes"] / 1e9,
     }
+    kp = merge_profile(r32["profile"])
+    if np_ > 0 and "interp_ab" in kp:
+        # the tracer kernel alone: interpolations (one tracer x one 3-component
+        # field) and stencil-point x component products per second
+        t_ia = kp["interp_ab"][0] / kp["interp_ab"][1] / 1e3
+        w = 2 * wl["nb"] + 2
+        out["tracer_kernel"] = {"ms": t_ia * 1e3, "interps_per_s": np_ / t_ia,
+                                "stencil_point_components_per_s": np_ * w ** 3 * 3 / t_ia}
     if not args.no_fp64:
         s64 = max(2, args.steps // 2)
         r64 = run_precision(B, ic, wl, "double", s64, 1, gradient=False)
