@@ -210,6 +210,11 @@ def roofline(prof, n, es, np_, nb, traffic_ok, steps):
     return out
 
 
+def progress(msg):
+    """One line per finished leg on stderr (stdout carries only the JSON line)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def accuracy(B, ic, wl, steps):
     """FP32 against FP64 from the same seed after `steps` NSVEparticles steps:
     relative L2 of the vorticity (half spectrum, nxmodes weights = real-space
@@ -222,6 +227,7 @@ def accuracy(B, ic, wl, steps):
         fs, ps = make_case(B, ic, wl, prec)
         for _ in range(steps):
             B.NSVEparticles_step(fs, dt)
+        progress("accuracy leg: %d %s steps done" % (steps, prec))
         w = fs.cvorticity.astype(np.complex128)
         spec = fs.cospectrum("velocity", "velocity")
         pos = ps.get()[0] if ps is not None else None
@@ -379,6 +385,7 @@ def main():
     n, np_ = wl["n"], wl["np"]
     r32 = run_precision(B, ic, wl, "single", args.steps, args.warmup)
     ms = r32["ms_per_step"]
+    progress("FP32 leg: %.3f ms per step" % ms)
     tb = tracer_bytes(np_)
     alg = step_bytes(n, 4, np_ > 0) + tb
     out = {
@@ -420,6 +427,7 @@ def main():
         s64 = max(2, args.steps // 2)
         r64 = run_precision(B, ic, wl, "double", s64, 1, gradient=False)
         ms64 = r64["ms_per_step"]
+        progress("FP64 leg: %.3f ms per step" % ms64)
         out["fp64"] = {"value": n ** 3 / (ms64 / 1e3), "ms_per_step": ms64,
                        "particle_interps_per_s": np_ / (ms64 / 1e3),
                        "hbm_frac_step": (step_bytes(n, 8, np_ > 0) + tb) / (ms64 / 1e3) / HBM_PEAK,
@@ -427,11 +435,14 @@ def main():
                        "kernel_ms": {k: v[0] / v[1] for k, v in merge_profile(r64["profile"]).items()}}
     if args.accuracy > 0 and not args.no_fp64:
         out["accuracy"] = accuracy(B, ic, wl, args.accuracy)
+        progress("accuracy leg done")
     if args.ensemble_steps > 0:
         out["ensemble"] = ensemble_bench(B, ic, args.ensemble_steps)
+        progress("ensemble leg done")
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(ic, wl, threads)
+        progress("CPU baseline done")
     else:
         out["cpu_baseline"] = None
     if args.copy_gib > 0:
