@@ -46,3 +46,25 @@ __host__ __device__ inline size_t sphys(int n, size_t idx) {
 
 // Wave-size facts for CDNA4.
 constexpr int WAVE = 64;
+
+// Cross-lane moves inside rows of 16 lanes (DPP, VALU only: no LDS traffic).
+// CTRL is a DPP control code: 0x150 + k broadcasts lane k of each row to the
+// whole row (row_newbcast), 0x101..0x10F shift the row by 1..15 towards
+// lane 0 (row_shl: lane i reads lane i + s; lanes past the row end read 0),
+// 0x121..0x12F rotate it (row_ror).
+template <int CTRL> BFPSH_DEV float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL> BFPSH_DEV double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// sum over the 16 lanes of each row, left in every lane of the row
+BFPSH_DEV double row16_sum(double v) {
+    v += dpp_d<0x128>(v);
+    v += dpp_d<0x124>(v);
+    v += dpp_d<0x122>(v);
+    v += dpp_d<0x121>(v);
+    return v;
+}

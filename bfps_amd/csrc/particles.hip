@@ -288,51 +288,72 @@ k_sample(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
 }
 
 // ---------------------------------------------------------------- LDS bricks
-// Tracers are bucketed by brick (KX x KY x KZ cells).  One workgroup per
-// brick stages the brick plus the stencil halo (n cells below, n+1 above, per
-// axis, periodic) of one field component at a time in LDS with contiguous
-// row reads, then 16-lane groups (lane = stencil column along x) accumulate
-// their tracer's stencil from LDS.  Every lane evaluates one weight per axis;
-// the group exchanges them with shuffles.
+// Tracers are bucketed by brick (KX x KY x KZ cells).  One workgroup per CU
+// walks a fixed list of bricks; for each brick it stages the brick plus the
+// stencil halo (n cells below, n+1 above, per axis, periodic) of one field
+// component at a time in LDS, and groups of LPT = 16 RY lanes accumulate one
+// tracer each from LDS: lane (jx, ry) holds stencil column x = jx and reads
+// rows y = ry, ry + RY, ... of each z plane.  The tile is double-buffered
+// and filled by LDS DMA (global_load_lds: no registers, all of a wave's loads
+// in flight at once): while the groups sum component c from one buffer, the
+// next component -- or the next brick's first -- arrives in the other.
+#ifndef BFPSH_BRICK_NT
+#define BFPSH_BRICK_NT 512
+#endif
+#ifndef BFPSH_BRICK_RY
+#define BFPSH_BRICK_RY 2
+#endif
+#ifndef BFPSH_BRICK_PAD
+#define BFPSH_BRICK_PAD 1
+#endif
 template <typename T, int NB>
 struct BrickCfg {
     static constexpr int W = 2 * NB + 2;
     static constexpr int KX = sizeof(T) == 4 ? 16 : 8;
     static constexpr int KY = sizeof(T) == 4 ? 16 : 8;
-    // 16 planes deep for FP32 stencils up to 10 wide (n <= 4): the tile
-    // (at most 25 x 25 x 25 floats, 62.5 KB) still lets two bricks share a
-    // CU and the halo per brick is amortised over twice the tracers
+    // 16 planes deep for FP32 stencils up to 10 wide (n <= 4): the halo per
+    // brick is amortised over twice the tracers
     static constexpr int KZ = (sizeof(T) == 4 && W <= 10) ? 16 : 8;
     static constexpr int WX = KX + W - 1, WY = KY + W - 1, WZ = KZ + W - 1;
-    static constexpr int TILE = WX * WY * WZ;
-    static constexpr size_t TILE_BYTES = size_t(TILE) * sizeof(T);
+    static constexpr int DW = int(sizeof(T) / 4);  // dwords per value
+    static constexpr int RY = BFPSH_BRICK_RY;      // y rows per LDS read
+    static constexpr int LPT = 16 * RY;            // lanes per tracer
+    static constexpr int NI = W / RY;              // y rows per lane
+    static_assert(W % RY == 0, "stencil rows split evenly over the lane rows");
+    // y-plane stride (values): with RY >= 2 the rows y and y+1 of a stencil
+    // start 16 dwords apart modulo 32 banks, so the two 16-lane rows of a
+    // read touch disjoint banks whatever the tracer's x offset
+    static constexpr int ys_of(int v) {
+        if (!BFPSH_BRICK_PAD || RY < 2) return v;
+        int s = v;
+        while ((s * DW) % 32 != 16) s++;
+        return s;
+    }
+    static constexpr int YS = ys_of(WZ * WX);
+    static constexpr int TILE = WY * YS;                    // values, pads included
+    static constexpr int CHUNKS = (TILE * DW + 63) / 64;    // 64-dword DMA chunks (one per wave load)
+    static constexpr size_t TILE_BYTES = size_t(CHUNKS) * 64 * 4;
+    static constexpr int NT = BFPSH_BRICK_NT;
+    static constexpr int GROUPS = NT / LPT;
+    // tracers per batch: a brick's tracers are normally one batch (512^3,
+    // 2^22 tracers: 64 per 16x16x8 brick on average); every batch refills
+    // the tiles of all components
+    static constexpr int PB = sizeof(T) == 4 ? 128 : 64;
+    static_assert(PB <= NT, "one tracer per thread in the batch prologue");
+    template <int NC, int MODE> static constexpr size_t res_doubles() {
+        return MODE == 0 ? size_t(PB) * NC : (size_t(PB) + 1) / 2;
+    }
+    // two tiles, then results [PB][NC] (or slots [PB], uint32) + in-cell
+    // positions [PB][3] (doubles) + cells [PB][3] (ints)
+    template <int NC, int MODE> static constexpr size_t lds_bytes() {
+        return 2 * TILE_BYTES + (res_doubles<NC, MODE>() + size_t(PB) * 3) * 8 + size_t(PB) * 3 * 4;
+    }
+    static constexpr size_t LDS_MAX = 160 * 1024;
 #ifdef BFPSH_NO_BRICKS
     static constexpr bool ok = false;
 #else
-    static constexpr bool ok = W <= 16 && TILE_BYTES <= 80 * 1024;
+    static constexpr bool ok = W <= 16 && lds_bytes<3, 0>() <= LDS_MAX && lds_bytes<9, 1>() <= LDS_MAX;
 #endif
-    static constexpr int LP = 16;
-    static constexpr int NT = 256;
-    static constexpr int GROUPS = NT / LP;
-    // tracers per batch, sized so that two bricks per CU fit in LDS.  The
-    // velocity + Adams-Bashforth form (MODE 0) keeps a batch's results in
-    // LDS for its epilogue; sampling (MODE 1) writes them straight to the
-    // output and keeps only the tracers' slots, so its FP32 batches hold a
-    // whole brick's tracers (each batch refills the tiles of all components)
-    template <int NC, int MODE> static constexpr int pb() {
-        if (MODE == 1) return sizeof(T) == 4 ? NT : 4 * GROUPS;
-        return (NC <= 3 && sizeof(T) == 4) ? 8 * GROUPS : 4 * GROUPS;
-    }
-    template <int NC, int MODE> static constexpr size_t res_doubles() {
-        return MODE == 0 ? size_t(pb<NC, MODE>()) * NC : (size_t(pb<NC, MODE>()) + 1) / 2;
-    }
-    template <int NC, int MODE> static constexpr size_t lds_bytes(int deg) {
-        // results [PB][NC] (or slots [PB], uint32) + frac [PB][3] + weight
-        // table (doubles), cells [PB][3] (ints, padded to doubles), tile
-        constexpr size_t P = pb<NC, MODE>();
-        return (res_doubles<NC, MODE>() + P * 3 + size_t(W) * (deg + 1) + (P * 3 + 1) / 2) * sizeof(double) +
-               TILE_BYTES;
-    }
 };
 
 // Visiting order of the bricks: each XCD walks a contiguous range of logical
@@ -378,12 +399,36 @@ __global__ void k_brick_bounds(const uint32_t* __restrict__ skeys, int64_t np, u
     if (i == np - 1 || skeys[i + 1] != k) bend[k] = uint32_t(i + 1);
 }
 
+// y weights of the lane rows and all z weights, spread within each 16-lane
+// row by DPP broadcasts (lane k of a row holds the weight of index k)
+template <int RY, int... I>
+BFPSH_DEV void bcast_rows(double (&w)[sizeof...(I)], double v, std::integer_sequence<int, I...>) {
+    ((w[I] = dpp_d<0x150 + I * RY>(v)), ...);
+}
+
+// Polynomial weight of one stencil index at in-cell position xr (Horner,
+// lowest-order coefficient first); the lane's table row is in registers.
+template <int DMAX>
+BFPSH_DEV double weight_of(const double (&row)[DMAX + 1], int deg, double xr) {
+    double a = 0.0;
+#pragma unroll
+    for (int k = DMAX; k >= 0; k--)
+        if (k <= deg) a = a * xr + row[k];
+    return a;
+}
+
 // MODE 0: velocity + Adams-Bashforth into the history ring; MODE 1: write
-// the NC sampled components to out[p][NC].
-// Per batch of up to PB tracers: a prologue loads each tracer's position once
-// (thread q owns tracer q of the batch) and leaves its cell offset in the
-// brick and in-cell fractions in LDS; the component passes then read only
-// LDS, so no dependent global load sits inside them.
+// the NC sampled components to out[p][NC].  One workgroup per CU: workgroup
+// g takes the logical bricks k G + rank(g), k = 0, 1, ... (rank: XCD-aware,
+// so that one XCD's workgroups hold neighbouring bricks at a time).  Work is
+// a sequence of phases (brick, batch of up to PB tracers, component); the
+// tile of phase i + 1 is requested before phase i is summed.  Per batch a
+// prologue loads each tracer's position once (thread q owns tracer q) and
+// leaves its cell offset in the brick and in-cell fractions in LDS.  All
+// control flow that reaches a barrier is uniform over the workgroup.
+// Arithmetic as the reference's (particles_field_computer.hpp:142-150):
+// FP64 weights, each FP32 or FP64 field value converted to double and
+// accumulated in double.
 template <typename T, int NB, int NC, int MODE>
 __global__ void __launch_bounds__((BrickCfg<T, NB>::NT))
 k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ rhs, double* __restrict__ out,
@@ -391,191 +436,225 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
         int nbx, int nby, int64_t np, int nsteps, int head, int order, int add_head, double dt,
         const double* __restrict__ tab, int deg, PartGeomD pg) {
     using BC = BrickCfg<T, NB>;
-    constexpr int W = BC::W, LP = BC::LP, NT = BC::NT, PB = BC::template pb<NC, MODE>();
-    constexpr int WX = BC::WX, WY = BC::WY, WZ = BC::WZ;
-    static_assert(PB <= NT, "one tracer per thread in the batch prologue");
+    constexpr int W = BC::W, NT = BC::NT, PB = BC::PB, LPT = BC::LPT, RY = BC::RY, NI = BC::NI;
+    constexpr int GROUPS = BC::GROUPS, WX = BC::WX, WY = BC::WY, WZ = BC::WZ, YS = BC::YS, DW = BC::DW;
+    constexpr int KX = BC::KX, KY = BC::KY, KZ = BC::KZ;
+    constexpr int DMAX = W - 1;  // highest polynomial degree of a (n, m) table
     extern __shared__ double smem[];
-    double* res = smem;                          // MODE 0: [PB][NC]
-    uint32_t* lslot = reinterpret_cast<uint32_t*>(smem);  // MODE 1: [PB] tracer slots
-    double* frac = res + BC::template res_doubles<NC, MODE>();  // [PB][3] in-cell positions
-    double* stab = frac + PB * 3;                // weight table, derivative 0
-    int* lcell = reinterpret_cast<int*>(stab + W * (deg + 1));  // [PB][3] cell in brick
-    T* tile = reinterpret_cast<T*>(lcell + ((PB * 3 + 1) / 2) * 2);
-    const int nbz = pg.n / BC::KZ;
-    const int64_t brick = brick_of(xcd_block(), nbx, nby, nbz);
-    if (brick >= int64_t(nbx) * nby * nbz) return;
-    const uint32_t s0 = bstart[brick], e0 = bend[brick];
-    if (s0 >= e0) return;  // uniform over the workgroup
-    const int tid = threadIdx.x;
-    const int grp = tid / LP, jx = tid % LP;
-    for (int i = tid; i < W * (deg + 1); i += NT) stab[i] = tab[i];
+    unsigned char* tiles = reinterpret_cast<unsigned char*>(smem);  // 2 x TILE_BYTES
+    double* res = smem + 2 * BC::TILE_BYTES / 8;                    // MODE 0: [PB][NC]
+    uint32_t* lslot = reinterpret_cast<uint32_t*>(res);             // MODE 1: [PB]
+    double* frac = res + BC::template res_doubles<NC, MODE>();      // [PB][3]
+    int* lcell = reinterpret_cast<int*>(frac + PB * 3);             // [PB][3]
+
     const int n = pg.n;
-    const int bz = int(brick / (int64_t(nbx) * nby)), by = int((brick / nbx) % nby), bx = int(brick % nbx);
-    const int x0 = bx * BC::KX - NB, y0 = by * BC::KY - NB, z0 = bz * BC::KZ - NB;
-    const int base[3] = {x0 + NB, y0 + NB, z0 + NB};
+    const int nbz = n / KZ;
+    const int64_t nbricks = int64_t(nbx) * nby * nbz;
+    const int G = gridDim.x;
+    const int rank = (G % 8 == 0) ? int(blockIdx.x % 8) * (G / 8) + int(blockIdx.x / 8) : int(blockIdx.x);
+    const int tid = threadIdx.x;
+    const int lane = tid % LPT, grp = tid / LPT, jx = lane & 15, ry = lane >> 4;
     // real planes 2 plane_size(n) reals apart, y rows ys reals apart (the
     // offsets within a plane fit 32 bits for n <= 1024)
     const size_t ps = 2 * plane_size(n);
     const uint32_t ys = uint32_t(2 * row_stride(n));
-    for (uint32_t bs = s0; bs < e0; bs += PB) {
+
+    // this lane's row of the weight table (index jx), in registers
+    double trow[DMAX + 1];
+    {
+        const int jj = jx < W ? jx : 0;
+#pragma unroll
+        for (int k = 0; k <= DMAX; k++) trow[k] = k <= deg ? tab[jj * (deg + 1) + k] : 0.0;
+    }
+
+    // next non-empty brick at or after logical round kr (uniform)
+    int64_t kr = 0, brick = 0;
+    uint32_t s0 = 0, e0 = 0;
+    auto find = [&](int64_t from, int64_t& kout, int64_t& bout, uint32_t& sout, uint32_t& eout) -> bool {
+        for (int64_t k = from;; k++) {
+            const int64_t L = k * G + rank;
+            if (L >= nbricks) return false;
+            const int64_t bb = brick_of(L, nbx, nby, nbz);
+            const uint32_t a = bstart[bb], e = bend[bb];
+            if (a < e) {
+                kout = k;
+                bout = bb;
+                sout = a;
+                eout = e;
+                return true;
+            }
+        }
+    };
+    // request the tile of component c of brick bb into buffer b (LDS DMA,
+    // 64 consecutive dwords per wave load; pad slots get a valid dummy)
+    auto fill = [&](int64_t bb, int c, int b) {
+        const int bz = int(bb / (int64_t(nbx) * nby)), by = int((bb / nbx) % nby), bx = int(bb % nbx);
+        const int x0 = bx * KX - NB, y0 = by * KY - NB, z0 = bz * KZ - NB;
+        const T* fc = f + c * ps;
+        uint32_t* tdw = reinterpret_cast<uint32_t*>(tiles + size_t(b) * BC::TILE_BYTES);
+        const int wv = tid / 64, ln = tid % 64;
+        for (int ch = wv; ch < BC::CHUNKS; ch += NT / 64) {
+            const int d = ch * 64 + ln;
+            const int e = d / DW, half = d % DW;
+            const int ly = e / YS, r = e % YS, lz = r / WX, lx = r % WX;
+            const unsigned char* src = reinterpret_cast<const unsigned char*>(fc);
+            if (ly < WY && lz < WZ) {
+                int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+                gx = gx < 0 ? gx + n : (gx >= n ? gx - n : gx);
+                gy = gy < 0 ? gy + n : (gy >= n ? gy - n : gy);
+                gz = gz < 0 ? gz + n : (gz >= n ? gz - n : gz);
+                src = reinterpret_cast<const unsigned char*>(fc + (uint32_t(gy) * ys + uint32_t(gz) * uint32_t(n) +
+                                                                   uint32_t(gx))) + 4 * half;
+            }
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src),
+                                             (__attribute__((address_space(3))) void*)(tdw + ch * 64), 4, 0, 0);
+        }
+    };
+
+    if (!find(0, kr, brick, s0, e0)) return;  // uniform
+    uint32_t bs = s0;
+    int comp = 0, buf = 0;
+    fill(brick, 0, 0);
+    int64_t myp = -1;
+    double myx[3] = {0.0, 0.0, 0.0};
+    for (;;) {
         const int cnt = int(min(uint32_t(PB), e0 - bs));
-        __syncthreads();  // previous batch done with frac / lcell / res
-        int64_t myp = -1;
-        double myx[3] = {0.0, 0.0, 0.0};
-        if (tid < cnt) {
-            myp = perm[bs + tid];
-            if constexpr (MODE == 1) lslot[tid] = uint32_t(myp);
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const double sp = pos[myp * 3 + d];
-                myx[d] = sp;
-                // get_norm_pos_in_cell / pbc_field_layer
-                const double rep = floor(sp / pg.width[d]);
-                const double inbox = sp - rep * pg.width[d];
-                const double ci = floor(inbox / pg.h[d]);
-                frac[tid * 3 + d] = (inbox - ci * pg.h[d]) / pg.h[d];
-                lcell[tid * 3 + d] = cell_of(sp, pg.h[d], n) - base[d];
-            }
-        }
-        for (int c = 0; c < NC; c++) {
-            __syncthreads();  // tile free; prologue visible
-            // tile fill in chunks; all loads of a chunk are issued before its
-            // LDS stores
-            constexpr int PERT = (BC::TILE + NT - 1) / NT;
-#ifndef BFPSH_FILL_CHUNK
-#define BFPSH_FILL_CHUNK 16
-#endif
-            constexpr int CH = PERT < BFPSH_FILL_CHUNK ? PERT : BFPSH_FILL_CHUNK;  // loads in flight per thread
-            const T* fc = f + c * ps;
-#pragma unroll
-            for (int k0 = 0; k0 < PERT; k0 += CH) {
-                T vals[CH];
-#pragma unroll
-                for (int k = 0; k < CH; k++) {
-                    const int i = tid + (k0 + k) * NT;
-                    if (k0 + k < PERT && i < BC::TILE) {
-                        const int lx = i % WX, lz = (i / WX) % WZ, ly = i / (WX * WZ);
-                        int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-                        gx = gx < 0 ? gx + n : (gx >= n ? gx - n : gx);
-                        gy = gy < 0 ? gy + n : (gy >= n ? gy - n : gy);
-                        gz = gz < 0 ? gz + n : (gz >= n ? gz - n : gz);
-                        vals[k] = fc[uint32_t(gy) * ys + uint32_t(gz) * uint32_t(n) + uint32_t(gx)];
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < CH; k++) {
-                    const int i = tid + (k0 + k) * NT;
-                    if (k0 + k < PERT && i < BC::TILE) tile[i] = vals[k];
-                }
-            }
-            __syncthreads();
-            for (int q = grp; q < cnt; q += NT / LP) {  // whole lane groups iterate together
-                // lane jx evaluates weight jx of each axis; the group shares them
-                double wv[3] = {0.0, 0.0, 0.0};
-                if (jx < W) {
-#pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        const double xr = frac[q * 3 + d];
-                        double a = 0.0;
-                        for (int k = deg; k >= 0; k--) a = a * xr + stab[jx * (deg + 1) + k];
-                        wv[d] = a;
-                    }
-                }
-                const int lc0 = lcell[q * 3 + 0], lc1 = lcell[q * 3 + 1], lc2 = lcell[q * 3 + 2];
-                double acc = 0.0;
-                // column base of this lane (x = lc0 + jx) at (y = lc1, z = lc2)
-                const T* tcol = tile + lc0 + (jx < W ? jx : 0) + (lc1 * WZ + lc2) * WX;
-                constexpr int YS = WZ * WX;  // y stride of the tile
-                if constexpr (sizeof(T) == 4) {
-                    // FP32 fields: the y sums of FP32 values run in FP32,
-                    // the z sums and the result in FP64
-                    float wyf[W];
-#pragma unroll
-                    for (int j = 0; j < W; j++) wyf[j] = __shfl(float(wv[1]), j, LP);
-                    if (jx < W) {
-                        for (int jz = 0; jz < W; jz++) {
-                            // z weight from lane jz; the W loads of the column
-                            // are issued together, then two FMA chains
-                            const double wzj = __shfl(wv[2], jz, LP);
-                            const T* tz = tcol + jz * WX;
-                            float t[W];
-#pragma unroll
-                            for (int jy = 0; jy < W; jy++) t[jy] = tz[jy * YS];
-                            float c0 = 0.0f, c1 = 0.0f;
-#pragma unroll
-                            for (int jy = 0; jy < W; jy += 2) {
-                                c0 = fmaf(wyf[jy], t[jy], c0);
-                                if (jy + 1 < W) c1 = fmaf(wyf[jy + 1], t[jy + 1], c1);
-                            }
-                            acc = fma(wzj, double(c0 + c1), acc);
-                        }
-                    }
-                } else {
-                    double wy[W];
-#pragma unroll
-                    for (int j = 0; j < W; j++) wy[j] = __shfl(wv[1], j, LP);
-                    if (jx < W) {
-                        for (int jz = 0; jz < W; jz++) {
-                            const double wzj = __shfl(wv[2], jz, LP);
-                            const T* tz = tcol + jz * WX;
-                            double t[W];
-#pragma unroll
-                            for (int jy = 0; jy < W; jy++) t[jy] = double(tz[jy * YS]);
-                            double c0 = 0.0, c1 = 0.0;
-#pragma unroll
-                            for (int jy = 0; jy < W; jy += 2) {
-                                c0 = fma(wy[jy], t[jy], c0);
-                                if (jy + 1 < W) c1 = fma(wy[jy + 1], t[jy + 1], c1);
-                            }
-                            acc = fma(wzj, c0 + c1, acc);
-                        }
-                    }
-                }
-                acc *= wv[0];
-#pragma unroll
-                for (int o = LP / 2; o > 0; o /= 2) acc += __shfl_xor(acc, o, LP);
-                if (jx == 0) {
-                    if constexpr (MODE == 1)
-                        out[size_t(lslot[q]) * NC + c] = acc;
-                    else
-                        res[q * NC + c] = acc;
-                }
-            }
-        }
-        __syncthreads();
-        if (tid < cnt) {
-            const int q = tid;
-            const int64_t p = myp;
-            if constexpr (MODE == 0) {
-                static_assert(NC == 3, "the Adams-Bashforth epilogue moves 3 coordinates");
-                double r0[3] = {res[q * NC + 0], res[q * NC + 1], res[q * NC + 2]};
-                double* h0 = rhs + size_t(head) * np * 3 + p * 3;
-                if (add_head) {
-                    r0[0] += h0[0];
-                    r0[1] += h0[1];
-                    r0[2] += h0[2];
-                }
-                h0[0] = r0[0];
-                h0[1] = r0[1];
-                h0[2] = r0[2];
+        if (comp == 0) {
+            // batch prologue (the previous batch ended with a barrier)
+            const int bz = int(brick / (int64_t(nbx) * nby)), by = int((brick / nbx) % nby),
+                      bx = int(brick % nbx);
+            const int base[3] = {bx * KX, by * KY, bz * KZ};
+            myp = -1;
+            if (tid < cnt) {
+                myp = perm[bs + tid];
+                if constexpr (MODE == 1) lslot[tid] = uint32_t(myp);
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    double r[6];
-                    r[0] = r0[d];
-                    for (int i = 1; i < order; i++) r[i] = rhs[size_t((head + i) % nsteps) * np * 3 + p * 3 + d];
-                    double incr;
-                    switch (order) {
-                    case 1: incr = dt * r[0]; break;
-                    case 2: incr = dt * (3. * r[0] - r[1]) / 2.; break;
-                    case 3: incr = dt * (23. * r[0] - 16. * r[1] + 5. * r[2]) / 12.; break;
-                    case 4: incr = dt * (55. * r[0] - 59. * r[1] + 37. * r[2] - 9. * r[3]) / 24.; break;
-                    case 5: incr = dt * (1901. * r[0] - 2774. * r[1] + 2616. * r[2] - 1274. * r[3] + 251. * r[4]) / 720.; break;
-                    default: incr = dt * (4277. * r[0] - 7923. * r[1] + 9982. * r[2] - 7298. * r[3] + 2877. * r[4] - 475. * r[5]) / 1440.; break;
-                    }
-                    pos[p * 3 + d] = myx[d] + incr;
+                    const double sp = pos[myp * 3 + d];
+                    myx[d] = sp;
+                    // get_norm_pos_in_cell / pbc_field_layer
+                    const double rep = floor(sp / pg.width[d]);
+                    const double inbox = sp - rep * pg.width[d];
+                    const double ci = floor(inbox / pg.h[d]);
+                    frac[tid * 3 + d] = (inbox - ci * pg.h[d]) / pg.h[d];
+                    lcell[tid * 3 + d] = cell_of(sp, pg.h[d], n) - base[d];
                 }
             }
         }
+        // the phase after this one
+        int64_t nkr = kr, nbrick = brick;
+        uint32_t ns0 = s0, ne0 = e0, nbs = bs;
+        int ncomp = comp + 1;
+        bool more = true;
+        if (ncomp == NC) {
+            ncomp = 0;
+            nbs = bs + PB;
+            if (nbs >= e0) {
+                more = find(kr + 1, nkr, nbrick, ns0, ne0);
+                nbs = ns0;
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA and prologue loads have landed
+        __syncthreads();                // the tile is whole; the other buffer is free
+        if (more) fill(nbrick, ncomp, buf ^ 1);
+
+        const T* tile = reinterpret_cast<const T*>(tiles + size_t(buf) * BC::TILE_BYTES);
+        for (int q = grp; q < cnt; q += GROUPS) {  // whole lane groups iterate together
+            // FP64 weights: lane jx evaluates index jx of each axis (the y
+            // weight of its row ry: index jx + ry, by a row shift); the
+            // group spreads them by DPP broadcasts
+            double wxo = 0.0, wyo = 0.0, wzo = 0.0;
+            if (jx < W) {
+                wxo = weight_of<DMAX>(trow, deg, frac[q * 3 + 0]);
+                wyo = weight_of<DMAX>(trow, deg, frac[q * 3 + 1]);
+                wzo = weight_of<DMAX>(trow, deg, frac[q * 3 + 2]);
+            }
+            double wyl = wyo;
+            if constexpr (RY >= 2) {
+                const double s1 = dpp_d<0x101>(wyo);
+                wyl = ry == 1 ? s1 : wyl;
+            }
+            if constexpr (RY >= 4) {
+                const double s2 = dpp_d<0x102>(wyo), s3 = dpp_d<0x103>(wyo);
+                wyl = ry == 2 ? s2 : (ry == 3 ? s3 : wyl);
+            }
+            double wyr[NI], wz[W];
+            bcast_rows<RY>(wyr, wyl, std::make_integer_sequence<int, NI>{});
+            bcast_rows<1>(wz, wzo, std::make_integer_sequence<int, W>{});
+            const int lc0 = lcell[q * 3 + 0], lc1 = lcell[q * 3 + 1], lc2 = lcell[q * 3 + 2];
+            // this lane's column: x = lc0 + jx (lanes past the stencil read
+            // column W - 1 with weight 0), rows lc1 + ry + i RY, plane lc2 + jz
+            const T* tcol = tile + (lc0 + (jx < W ? jx : W - 1)) + lc2 * WX + (lc1 + ry) * YS;
+            double acc = 0.0;
+#pragma unroll
+            for (int jz = 0; jz < W; jz++) {
+                const T* tz = tcol + jz * WX;
+                T t[NI];
+#pragma unroll
+                for (int i = 0; i < NI; i++) t[i] = tz[i * RY * YS];
+                double c0 = 0.0, c1 = 0.0;
+#pragma unroll
+                for (int i = 0; i < NI; i += 2) {
+                    c0 = fma(wyr[i], double(t[i]), c0);
+                    if (i + 1 < NI) c1 = fma(wyr[i + 1], double(t[i + 1]), c1);
+                }
+                acc = fma(wz[jz], c0 + c1, acc);
+            }
+            acc = row16_sum(acc * wxo);
+#pragma unroll
+            for (int o = 16; o < LPT; o *= 2) acc += __shfl_xor(acc, o, LPT);
+            if (lane == 0) {
+                if constexpr (MODE == 1)
+                    out[size_t(lslot[q]) * NC + comp] = acc;
+                else
+                    res[q * NC + comp] = acc;
+            }
+        }
+
+        if (comp == NC - 1) {
+            __syncthreads();  // the batch's results are complete; its LDS slots may be reused
+            if constexpr (MODE == 0) {
+                static_assert(NC == 3, "the Adams-Bashforth epilogue moves 3 coordinates");
+                if (tid < cnt) {
+                    const int q = tid;
+                    const int64_t p = myp;
+                    double r0[3] = {res[q * NC + 0], res[q * NC + 1], res[q * NC + 2]};
+                    double* h0 = rhs + size_t(head) * np * 3 + p * 3;
+                    if (add_head) {
+                        r0[0] += h0[0];
+                        r0[1] += h0[1];
+                        r0[2] += h0[2];
+                    }
+                    h0[0] = r0[0];
+                    h0[1] = r0[1];
+                    h0[2] = r0[2];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        double r[6];
+                        r[0] = r0[d];
+                        for (int i = 1; i < order; i++) r[i] = rhs[size_t((head + i) % nsteps) * np * 3 + p * 3 + d];
+                        double incr;
+                        switch (order) {
+                        case 1: incr = dt * r[0]; break;
+                        case 2: incr = dt * (3. * r[0] - r[1]) / 2.; break;
+                        case 3: incr = dt * (23. * r[0] - 16. * r[1] + 5. * r[2]) / 12.; break;
+                        case 4: incr = dt * (55. * r[0] - 59. * r[1] + 37. * r[2] - 9. * r[3]) / 24.; break;
+                        case 5: incr = dt * (1901. * r[0] - 2774. * r[1] + 2616. * r[2] - 1274. * r[3] + 251. * r[4]) / 720.; break;
+                        default: incr = dt * (4277. * r[0] - 7923. * r[1] + 9982. * r[2] - 7298. * r[3] + 2877. * r[4] - 475. * r[5]) / 1440.; break;
+                        }
+                        pos[p * 3 + d] = myx[d] + incr;
+                    }
+                }
+            }
+        }
+        if (!more) break;
+        kr = nkr;
+        brick = nbrick;
+        s0 = ns0;
+        e0 = ne0;
+        bs = nbs;
+        comp = ncomp;
+        buf ^= 1;
     }
 }
 
@@ -667,10 +746,14 @@ int launch_brick(bfpsh_ctx* ctx, const void* f, double* d_out, double dt, int or
     const int n = ctx->n;
     const int nbx = n / BC::KX, nby = n / BC::KY;
     const int64_t nb = int64_t(nbx) * nby * (n / BC::KZ);
-    const size_t lds = BC::template lds_bytes<NC, MODE>(ctx->deg);
+    const size_t lds = BC::template lds_bytes<NC, MODE>();
     auto kern = k_brick<T, NB, NC, MODE>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    const unsigned grid = unsigned((nb + 7) / 8 * 8);
+    // one workgroup per CU (each takes the LDS of a CU), persistent over
+    // its bricks
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const unsigned grid = unsigned(std::max<int64_t>(1, std::min<int64_t>(nb, ncu)));
     LAUNCH(name, (kern<<<grid, BC::NT, lds, ctx->stream>>>(
                      (const T*)f, ctx->d_pos, ctx->d_rhs, d_out, ctx->d_perm + ctx->np, ctx->d_bstart,
                      ctx->d_bstart + nb, nbx, nby, ctx->np, ctx->nsteps, ctx->head, order, ctx->add_head ? 1 : 0, dt,
