@@ -28,7 +28,10 @@ import numpy as np
 from . import capi
 from . import h5io
 
-__all__ = ["vorticity_equation", "kspace", "field", "particles_system", "NSVEparticles_step", "ensemble", "capi"]
+__version__ = "0.2.0"
+
+__all__ = ["vorticity_equation", "kspace", "field", "compute_gradient", "particles_system", "NSVEparticles_step",
+           "ensemble", "capi"]
 
 
 class kspace:
@@ -192,6 +195,22 @@ class vorticity_equation:
         """cvelocity = Biot-Savart(cvorticity) + symmetrize (spectral)."""
         return self.get_cfield(capi.CVELOCITY)
 
+    def impose_zero_modes(self):
+        """vorticity_equation::impose_zero_modes (vorticity_equation.cpp:39-46):
+        the k = 0 mode of cvelocity, cvorticity and the stage fields."""
+        capi.check(self._h, capi.load().bfpsh_impose_zero_modes(self._h), "impose_zero_modes")
+
+    def compute_vorticity(self, velocity=None):
+        """vorticity_equation::compute_vorticity (vorticity_equation.cpp:171-204):
+        cvorticity = i k x cvelocity for k^2 <= kM^2 (zero beyond), then
+        symmetrize.  `velocity` (an array in the complex layout) first sets
+        cvelocity; otherwise cvelocity is what the last call left there."""
+        lib = capi.load()
+        if velocity is not None:
+            a = np.ascontiguousarray(np.asarray(velocity).reshape(self.cshape()), dtype=self.cdtype)
+            capi.check(self._h, lib.bfpsh_put_cfield(self._h, capi.CVELOCITY, a.ctypes.data), "put_cfield")
+        capi.check(self._h, lib.bfpsh_compute_vorticity(self._h), "compute_vorticity")
+
     def velocity_rspace(self):
         capi.check(self._h, capi.load().bfpsh_velocity_rspace(self._h), "velocity_rspace")
 
@@ -263,6 +282,20 @@ class vorticity_equation:
         b = ctypes.c_size_t()
         capi.check(self._h, capi.load().bfpsh_device_bytes(self._h, ctypes.byref(b)), "device_bytes")
         return b.value
+
+
+def compute_gradient(kk, src, dst):
+    """The free function compute_gradient(kk, src, dst)
+    (bfps/cpp/field.cpp:1042-1101) on standalone fields of one context:
+    src ONE -> dst THREE or THREE -> THREExTHREE, dst component
+    j ncomp(src) + i = i k_j src_i for k^2 < kM^2, zero elsewhere.  `kk` is
+    the context's kspace (accepted for the reference's signature)."""
+    del kk
+    if src.fs is not dst.fs:
+        raise capi.BfpshError("compute_gradient needs two fields of one context")
+    capi.check(src.fs.handle, capi.load().bfpsh_field_compute_gradient(src.fs.handle, src.id, dst.id),
+               "compute_gradient")
+    dst.real_space_representation = False
 
 
 class field:

@@ -83,22 +83,33 @@ def build(jobs=None, sizes=SIZES, verbose=True, out=None, defines=()):
     return lib
 
 
-DRIVER_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "nsve_particles.cpp")
-DRIVER = os.path.join(os.path.dirname(HERE), "tests", "cpp", "nsve_particles")
+CPP = os.path.join(os.path.dirname(HERE), "tests", "cpp")
+DRIVER_SRC = os.path.join(CPP, "nsve_particles.cpp")
+DRIVER = os.path.join(CPP, "nsve_particles")
+# (source, binary, links the library): the array-fed driver, the file-based
+# NSVEparticles job, the HDF5-subset tool
+CPP_PROGRAMS = ((DRIVER_SRC, DRIVER, True),
+                (os.path.join(CPP, "nsve_particles_job.cpp"), os.path.join(CPP, "nsve_particles_job"), True),
+                (os.path.join(CPP, "h5_check.cpp"), os.path.join(CPP, "h5_check"), False))
 
 
 def build_driver(lib=None, out=None, verbose=True):
-    """Compile the C++ NSVEparticles-style driver (host code only, g++)
-    against include/bfps_hip.hpp and libbfps_hip.so."""
+    """Compile the C++ programs over include/bfps_hip.hpp (host code only,
+    g++): the NSVEparticles drivers against libbfps_hip.so, and the HDF5
+    subset tool.  `out` renames the array-fed driver only."""
     lib = lib or LIB
-    out = out or DRIVER
     inc = os.path.join(os.path.dirname(HERE), "include")
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I" + inc, DRIVER_SRC, "-o", out,
-           lib, "-Wl,-rpath," + os.path.dirname(os.path.abspath(lib))]
-    subprocess.check_call(cmd)
-    if verbose:
-        print("built %s" % out)
-    return out
+    for src, exe, links in CPP_PROGRAMS:
+        if out and src == DRIVER_SRC:
+            exe = out
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-Wno-stringop-truncation", "-I" + inc,
+               src, "-o", exe]
+        if links:
+            cmd += [lib, "-Wl,-rpath," + os.path.dirname(os.path.abspath(lib))]
+        subprocess.check_call(cmd)
+        if verbose:
+            print("built %s" % exe)
+    return out or DRIVER
 
 
 if __name__ == "__main__":

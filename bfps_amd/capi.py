@@ -33,6 +33,9 @@ SIGNATURES = {
     "bfpsh_get_cfield": (_I, [_P, _I, _P]),
     "bfpsh_get_rfield": (_I, [_P, _I, _P]),
     "bfpsh_load_projection": (_I, [_P]),
+    "bfpsh_compute_vorticity": (_I, [_P]),
+    "bfpsh_impose_zero_modes": (_I, [_P]),
+    "bfpsh_field_compute_gradient": (_I, [_P, _I, _I]),
     "bfpsh_step": (_I, [_P, _D]),
     "bfpsh_velocity_rspace": (_I, [_P]),
     "bfpsh_gradient_rspace": (_I, [_P]),

@@ -255,14 +255,42 @@ int bfpsh_kspace_info(bfpsh_ctx* ctx, double* out4, double* kshell, int64_t* nsh
 
 int bfpsh_put_cfield(bfpsh_ctx* ctx, int which, const void* host) {
     CTX_CHECK();
-    if (!host || which != BFPSH_CVORTICITY) return fail(ctx, BFPSH_EINVAL, "only the vorticity can be put");
+    if (!host || (which != BFPSH_CVORTICITY && which != BFPSH_CVELOCITY))
+        return fail(ctx, BFPSH_EINVAL, "only the vorticity or the velocity can be put");
+    void* dst = ctx->v0;
+    if (which == BFPSH_CVELOCITY) {
+        // cvelocity lives in work planes 0..2 (the step reuses them)
+        int r = ensure_work(ctx, 6);
+        if (r) return r;
+        dst = ctx->work;
+    }
     const size_t bytes = 3 * ctx->ps * 2 * ctx->es;
     std::vector<unsigned char> tmp(bytes);
     if (ctx->prec == BFPSH_FP32) host_to_planar<float>((const float*)host, (float*)tmp.data(), ctx->n);
     else host_to_planar<double>((const double*)host, (double*)tmp.data(), ctx->n);
-    HIPCHK(hipMemcpyAsync(ctx->v0, tmp.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    ctx->ru_valid = false;
+    if (which == BFPSH_CVORTICITY) ctx->ru_valid = false;
+    return BFPSH_OK;
+}
+
+int bfpsh_compute_vorticity(bfpsh_ctx* ctx) {
+    CTX_CHECK();
+    return PREC_CALL(compute_vorticity, ctx);
+}
+
+int bfpsh_impose_zero_modes(bfpsh_ctx* ctx) {
+    CTX_CHECK();
+    // the k = 0 mode is element 0 of each plane: cvelocity (work 0..2), v[0]
+    // (cvorticity), v[1], v[2]
+    const size_t cb = 2 * ctx->es, pb = ctx->ps * cb;
+    void* bufs[4] = {ctx->work, ctx->v0, ctx->v1, ctx->v2};
+    for (void* b : bufs) {
+        if (!b) continue;
+        for (int c = 0; c < 3; c++)
+            HIPCHK(hipMemsetAsync(static_cast<unsigned char*>(b) + c * pb, 0, cb, ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return BFPSH_OK;
 }
 
@@ -399,6 +427,26 @@ int bfpsh_field_symmetrize(bfpsh_ctx* ctx, int id) {
         LAUNCH("symmetrize", (k_symmetrize<float><<<(n * n + 255) / 256, 256, 0, ctx->stream>>>((float2*)ctx->fields[id], n, ctx->field_nc[id], 0)));
     else
         LAUNCH("symmetrize", (k_symmetrize<double><<<(n * n + 255) / 256, 256, 0, ctx->stream>>>((double2*)ctx->fields[id], n, ctx->field_nc[id], 0)));
+    return BFPSH_OK;
+}
+
+int bfpsh_field_compute_gradient(bfpsh_ctx* ctx, int src_id, int dst_id) {
+    CTX_CHECK();
+    if (!field_ok(ctx, src_id) || !field_ok(ctx, dst_id) || src_id == dst_id)
+        return fail(ctx, BFPSH_EINVAL, "bad field ids");
+    const int ncs = ctx->field_nc[src_id], ncd = ctx->field_nc[dst_id];
+    if (!((ncs == 1 && ncd == 3) || (ncs == 3 && ncd == 9)))
+        return fail(ctx, BFPSH_EINVAL, "gradient maps ONE to THREE or THREE to THREExTHREE");
+    if (ctx->field_real[src_id]) return fail(ctx, BFPSH_ESTATE, "source field is in real space (call dft first)");
+    const size_t cnt = size_t(ctx->n) * ctx->n * (ctx->n / 2);
+    const unsigned grid = unsigned((cnt + 255) / 256);
+    if (ctx->prec == BFPSH_FP32)
+        LAUNCH("gradient", (k_gradient<float><<<grid, 256, 0, ctx->stream>>>((const float2*)ctx->fields[src_id],
+                                                                             (float2*)ctx->fields[dst_id], geom(ctx), ncs)));
+    else
+        LAUNCH("gradient", (k_gradient<double><<<grid, 256, 0, ctx->stream>>>((const double2*)ctx->fields[src_id],
+                                                                              (double2*)ctx->fields[dst_id], geom(ctx), ncs)));
+    ctx->field_real[dst_id] = 0;
     return BFPSH_OK;
 }
 

@@ -217,6 +217,7 @@ struct OpsTable {
     int (*lagrangian_acceleration)(bfpsh_ctx*, void*);
     int (*normalize)(bfpsh_ctx*, void*, int);
     int (*rspace_stats)(bfpsh_ctx*, const void*, int, int, const double*, double*, int64_t*);
+    int (*compute_vorticity)(bfpsh_ctx*);
 };
 
 int create_ctx(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, double dkz, int precision,

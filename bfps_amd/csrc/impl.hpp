@@ -241,6 +241,17 @@ struct Impl {
         return symmetrize(ctx, ctx->work, 3);
     }
 
+    // compute_vorticity (vorticity_equation.cpp:171-204): cvorticity =
+    // symmetrize(i k x cvelocity), cvelocity in work planes 0..2
+    static int compute_vorticity(bfpsh_ctx* ctx) {
+        int r = ensure_work(ctx, 6);
+        if (r) return r;
+        const size_t ps = size_t(N) * N * NXH;
+        LAUNCH("curl", (k_curl<T><<<(ps + 255) / 256, 256, 0, ctx->stream>>>((const CT*)ctx->work, (CT*)ctx->v0, geom(ctx))));
+        ctx->ru_valid = false;
+        return symmetrize(ctx, ctx->v0, 3);
+    }
+
     // compute_pressure (vorticity_equation.cpp:441-520) of the velocity of
     // cvorticity into the spectral plane p: the real velocity is made here
     // (the reference takes `u` in real space), its products go through work

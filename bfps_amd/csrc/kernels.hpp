@@ -918,6 +918,49 @@ __global__ void k_biot_savart(const C2<T>* __restrict__ w, C2<T>* __restrict__ u
     }
 }
 
+// compute_vorticity (bfps/cpp/vorticity_equation.cpp:171-204): the curl
+// w = i k x u for k^2 <= kM^2 (the k = 0 mode included), zero beyond; the
+// caller symmetrizes.
+template <typename T>
+__global__ void k_curl(const C2<T>* __restrict__ u, C2<T>* __restrict__ w, Geom g) {
+    const int n = g.n, nxh = n / 2;
+    const size_t ps = plane_size(n);
+    const size_t lidx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lidx >= size_t(n) * n * nxh) return;
+    const size_t idx = sphys(n, lidx);
+    const int kxi = lidx % nxh, kz = (lidx / nxh) % n, ky = lidx / (size_t(nxh) * n);
+    const double kx = kxi * g.dkx, kyv = kwave(ky, n, g.dky), kzv = kwave(kz, n, g.dkz);
+    const double k2 = kx * kx + kyv * kyv + kzv * kzv;
+    const C2<T> a0 = u[idx], a1 = u[ps + idx], a2 = u[2 * ps + idx];
+    if (k2 <= g.kM2) {
+        w[idx] = mk<T>(T(-(kyv * a2.y - kzv * a1.y)), T(kyv * a2.x - kzv * a1.x));
+        w[ps + idx] = mk<T>(T(-(kzv * a0.y - kx * a2.y)), T(kzv * a0.x - kx * a2.x));
+        w[2 * ps + idx] = mk<T>(T(-(kx * a1.y - kyv * a0.y)), T(kx * a1.x - kyv * a0.x));
+    } else {
+        w[idx] = w[ps + idx] = w[2 * ps + idx] = mk<T>(T(0), T(0));
+    }
+}
+
+// compute_gradient (bfps/cpp/field.cpp:1042-1101) of an ncs-component
+// spectral field (ncs = 1 or 3): dst component j ncs + i = i k_j src_i for
+// k^2 < kM^2 (strict), zero elsewhere.
+template <typename T>
+__global__ void k_gradient(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, Geom g, int ncs) {
+    const int n = g.n, nxh = n / 2;
+    const size_t ps = plane_size(n);
+    const size_t lidx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lidx >= size_t(n) * n * nxh) return;
+    const size_t idx = sphys(n, lidx);
+    const int kxi = lidx % nxh, kz = (lidx / nxh) % n, ky = lidx / (size_t(nxh) * n);
+    const double kv[3] = {kxi * g.dkx, kwave(ky, n, g.dky), kwave(kz, n, g.dkz)};
+    const bool in = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2] < g.kM2;
+    for (int i = 0; i < ncs; i++) {
+        const C2<T> a = src[i * ps + idx];
+        for (int j = 0; j < 3; j++)
+            dst[(j * ncs + i) * ps + idx] = in ? mk<T>(T(-kv[j] * a.y), T(kv[j] * a.x)) : mk<T>(T(0), T(0));
+    }
+}
+
 // kspace::cospectrum (bfps/cpp/kspace.cpp:521-598): FP64 shell sums
 // S[s][i][j] += nxmodes (Re a_i Re b_j + Im a_i Im b_j), s = int(|k| / dk),
 // k^2 <= kM^2, nxmodes = 1 at kx = 0 else 2.

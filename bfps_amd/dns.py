@@ -13,14 +13,15 @@ bfps/DNS.py:372-452); the tracer samples ``<simname>_particles.h5``.  When
 
 Differences from files written through libhdf5: the statistics datasets
 are contiguous with fixed maximum dimensions rather than chunked and
-unlimited (they are grown by rewriting the file, as every append in
-``h5io`` is), and strings are fixed-length ASCII rather than h5py's
-variable-length strings.
+unlimited (a dataset is grown by appending a longer copy that replaces it
+in its group; a statistics row is written in place), and strings are
+fixed-length ASCII rather than h5py's variable-length strings.
 """
 import os
 
 import numpy as np
 
+from . import __version__
 from . import (NSVE_do_stats, NSVEparticles_do_stats, NSVEparticles_read_checkpoint, NSVEparticles_step,
                NSVEparticles_write_checkpoint, h5io, particles_system, vorticity_equation)
 
@@ -90,7 +91,7 @@ def kspace_info(p):
     return out
 
 
-def write_par(simname, parameters, dns_type="NSVEparticles", iter0=0, work_dir="."):
+def write_par(simname, parameters, dns_type="NSVEparticles", iter0=0, work_dir=".", precision="single"):
     """Creates the run file <simname>.h5 as _base.write_par + DNS.write_par
     do (bfps/_base.py:158-173, bfps/DNS.py:372-419): parameters/<name>,
     iteration, bfps_info/{solver_class, exec_name}, kspace/*, one row of
@@ -118,7 +119,8 @@ def write_par(simname, parameters, dns_type="NSVEparticles", iter0=0, work_dir="
             f["parameters/" + k] = v.encode("ascii") if isinstance(v, str) else v
         f["iteration"] = np.int64(iter0)
         f["bfps_info/solver_class"] = b"DNS"
-        f["bfps_info/exec_name"] = dns_type.encode("ascii")
+        # DNS.name (bfps/DNS.py:655): <dns_type>-<precision>-v<version>
+        f["bfps_info/exec_name"] = ("%s-%s-v%s" % (dns_type, precision, __version__)).encode("ascii")
         ks = kspace_info(p)
         for k, v in ks.items():
             f["kspace/" + k] = v
@@ -210,7 +212,7 @@ def launch_jobs(simname, parameters, dns_type="NSVEparticles", work_dir=".", src
                 np_ = p["nparticles"]
                 f["tracers0/rhs/0"] = np.zeros((p["tracers0_integration_steps"], np_, 3))
                 f["tracers0/state/0"] = generate_tracer_state(np_, rseed=particle_rand_seed)
-        write_par(simname, p, dns_type=dns_type, work_dir=work_dir)
+        write_par(simname, p, dns_type=dns_type, work_dir=work_dir, precision=precision)
     return NSVEparticles(simname, p, precision=precision, work_dir=work_dir)
 
 
@@ -226,6 +228,73 @@ def generate_tracer_state(nparticles, rseed=None, batch_size=10 ** 6):
         out[done:done + m] = rand((m, 3)) * 2 * np.pi
         done += m
     return out
+
+
+ENSEMBLE_CLASSES = ((32, 32), (32, 48), (24, 64), (8, 128))  # (boxes, N)
+
+
+def ensemble_manifest(seed=5, classes=ENSEMBLE_CLASSES, nu_range=(0.01, 0.05), k0_choices=(2.0, 3.0, 4.0),
+                      np_range=(1e3, 1e5), neighbours=1, smoothness=1, integration_steps=4, dtfactor=0.5):
+    """Parameters of an ensemble of independent boxes (no reference
+    counterpart: the reference runs one box per job).  One entry per box, in
+    the order of `classes` ((count, N) pairs): N, its own initial-field seed,
+    nu ~ U[nu_range], peak wavenumber k0 drawn from k0_choices, tracer count
+    log-uniform in np_range with its own position seed, dt = dtfactor / N,
+    linear band forcing (amplitude 0.5 on 2 <= |k| <= 4, the reference's
+    defaults, bfps/DNS.py:139-166) and the tracer scheme (neighbours,
+    smoothness, Adams-Bashforth steps; reference defaults 1, 1, 4).  The
+    draw is a numpy Generator seeded with `seed`, so a manifest is fixed by
+    its arguments."""
+    rng = np.random.default_rng(seed)
+    boxes = []
+    for cnt, n in classes:
+        for _ in range(int(cnt)):
+            i = len(boxes)
+            boxes.append({
+                "n": int(n), "seed": int(seed * 1000 + i), "nu": float(rng.uniform(*nu_range)),
+                "k0": float(rng.choice(k0_choices)), "dt": dtfactor / n,
+                "forcing_type": "linear", "famplitude": 0.5, "fk0": 2.0, "fk1": 4.0,
+                "nparticles": int(round(10 ** rng.uniform(np.log10(np_range[0]), np.log10(np_range[1])))),
+                "particle_seed": int(seed * 1000 + 500 + i), "neighbours": int(neighbours),
+                "smoothness": int(smoothness), "integration_steps": int(integration_steps)})
+    return boxes
+
+
+def write_ensemble_manifest(path, boxes):
+    """The manifest as JSON (one object per box)."""
+    import json
+    with open(path, "w") as f:
+        json.dump({"boxes": boxes}, f, indent=1)
+
+
+def read_ensemble_manifest(path):
+    import json
+    with open(path) as f:
+        return json.load(f)["boxes"]
+
+
+def build_ensemble(boxes, precision="single", dealias="SMOOTH", with_particles=True):
+    """An ``ensemble`` set up from a manifest: each box's physics, its seeded
+    solenoidal initial field (``bfps_amd.ic``, peak k0, energy 1/2) through
+    the checkpoint-read projections, and its seeded uniform tracers.
+    Returns (ensemble, particle systems (None for boxes without tracers),
+    dt array)."""
+    from . import ensemble, ic
+    ens = ensemble([b["n"] for b in boxes], precision=precision, dealias=dealias)
+    parts = []
+    for i, b in enumerate(boxes):
+        fs = ens.boxes[i]
+        fs.nu, fs.forcing_type = b["nu"], b["forcing_type"]
+        fs.famplitude, fs.fk0, fs.fk1 = b["famplitude"], b["fk0"], b["fk1"]
+        fs.io_checkpoint(read=True, data=ic.solenoidal_vorticity(b["n"], seed=b["seed"], k0=b["k0"],
+                                                                 dtype=fs.cdtype))
+        ps = None
+        if with_particles and b["nparticles"] > 0:
+            ps = particles_system(fs, b["nparticles"], b["neighbours"], b["smoothness"], b["integration_steps"],
+                                  step_idx=1)
+            ps.put(ic.uniform_positions(b["nparticles"], seed=b["particle_seed"]))
+        parts.append(ps)
+    return ens, parts, np.array([b["dt"] for b in boxes])
 
 
 class NSVEparticles:

@@ -24,10 +24,14 @@ the HDF5 file format those files use:
 
 The writer lays files out as the C library does for these objects (the
 same message set, flags and encodings, dataspaces with maximum dimensions,
-early space allocation): a fresh file is written in one pass; opening an
-existing file in mode "a" reads every dataset and rewrites the whole file
-with the additions (a cost proportional to the file).  Anything outside the
-subset raises ``H5FormatError`` so that a misread cannot pass silently.
+early space allocation).  A fresh file is written in one pass.  An existing
+file opened in mode "a" is appended to without moving anything in it: new
+datasets and groups go to the end of the file; a dataset assigned again
+with its type and shape is overwritten in place; each group whose entries
+changed gets a new heap, symbol-table nodes and B-tree at the end, and its
+symbol-table message (and the cached copy in its parent's entry) is patched
+in place.  Anything outside the subset raises ``H5FormatError`` so that a
+misread cannot pass silently.
 """
 import os
 import struct
@@ -132,6 +136,13 @@ def _decode_datatype(b, off):
 
 
 # ---------------------------------------------------------------- reading
+class _Group(dict):
+    """A group read from a file: its entries, plus where its symbol-table
+    message (stab_at) and the cached copy of its (B-tree, heap) addresses in
+    its parent's entry (scratch_at; the superblock's for the root) are."""
+    hdr = btree = heap = stab_at = scratch_at = None
+
+
 class _Dataset:
     def __init__(self, dtype, shape, address=None, data=None, compact=None):
         self.dtype, self.shape = np.dtype(dtype), tuple(int(s) for s in shape)
@@ -153,6 +164,7 @@ class _Parser:
         if head[13] != 8 or head[14] != 8:
             raise H5FormatError("offset/length sizes %d/%d" % (head[13], head[14]))
         self.root = _u(head, 56 + 8, 8)
+        self.eof = _u(head, 40, 8)
 
     def read(self, addr, n):
         self.f.seek(addr)
@@ -178,17 +190,19 @@ class _Parser:
                     blocks.append((_u(body, 0, 8), _u(body, 8, 8)))
                 if flags & 0x02 and mtype not in (0x00, 0x10):
                     raise H5FormatError("shared message type 0x%x" % mtype)
-                out.append((mtype, body))
+                out.append((mtype, body, start + p + 8))
                 p += 8 + msize
         return out
 
-    def object(self, addr):
+    def object(self, addr, scratch_at=None):
         msgs = self.messages(addr)
-        for t, body in msgs:
+        for t, body, at in msgs:
             if t == 0x11:
-                return self.group(_u(body, 0, 8), _u(body, 8, 8))
+                g = self.group(_u(body, 0, 8), _u(body, 8, 8))
+                g.hdr, g.stab_at, g.scratch_at = addr, at, scratch_at
+                return g
         dtype = shape = layout = None
-        for t, body in msgs:
+        for t, body, _ in msgs:
             if t == 0x01:
                 ver, rank, flags = body[0], body[1], body[2]
                 p = 8 if ver == 1 else 4
@@ -214,8 +228,11 @@ class _Parser:
         if dtype is None or shape is None or layout is None:
             raise H5FormatError("object at %d is neither a group nor a dataset" % addr)
         if layout[0] == "compact":
-            return _Dataset(dtype, shape, compact=layout[1])
-        return _Dataset(dtype, shape, address=None if layout[1] == UNDEF else layout[1])
+            ds = _Dataset(dtype, shape, compact=layout[1])
+        else:
+            ds = _Dataset(dtype, shape, address=None if layout[1] == UNDEF else layout[1])
+        ds.hdr = addr
+        return ds
 
     def group(self, btree, heap):
         h = self.read(heap, 32)
@@ -224,7 +241,11 @@ class _Parser:
         names = self.read(_u(h, 24, 8), _u(h, 8, 8))
         entries = {}
         self._btree(btree, names, entries)
-        return {name: self.object(addr) for name, addr in entries.items()}
+        g = _Group()
+        g.btree, g.heap = btree, heap
+        for name, (addr, at) in entries.items():
+            g[name] = self.object(addr, at)
+        return g
 
     def _btree(self, addr, names, out):
         h = self.read(addr, 24)
@@ -248,7 +269,8 @@ class _Parser:
         for e in range(n):
             noff, obj = _u(body, 40 * e, 8), _u(body, 40 * e + 8, 8)
             end = names.index(b"\x00", noff)
-            out[names[noff:end].decode()] = obj
+            # the entry's cached (B-tree, heap) pair sits 24 bytes in
+            out[names[noff:end].decode()] = (obj, addr + 8 + 40 * e + 24)
 
     def data(self, ds):
         n = int(np.prod(ds.shape, dtype=np.int64)) * ds.dtype.itemsize
@@ -391,6 +413,138 @@ def _write_file(path, tree):
     os.replace(tmp, path)
 
 
+class _Appender(_Writer):
+    """Adds to an existing file without moving anything in it: new datasets
+    (data, then object header) and new groups go to the end of the file; a
+    dataset assigned again with its type and shape is overwritten in place;
+    every group whose entries changed gets a new local heap, symbol-table
+    nodes and B-tree at the end, its symbol-table message and the cached
+    copy of it in its parent's entry (the superblock's for the root) being
+    patched in place.  Superseded group structures stay behind as unused
+    bytes."""
+
+    def __init__(self, fh, eof):
+        _Writer.__init__(self, fh)
+        self.pos = _pad8(eof)
+
+    def dataset(self, arr):
+        arr = np.array(arr, order="C")
+        if arr.dtype.byteorder == ">":
+            arr = arr.astype(arr.dtype.newbyteorder("<"))
+        daddr = self.alloc(arr.nbytes) if arr.nbytes else UNDEF
+        if arr.nbytes:
+            self.put(daddr, arr.tobytes())
+        h = _object_header(_dataset_messages(arr.shape, arr.dtype, daddr, arr.nbytes, self.mtime))
+        a = self.alloc(len(h))
+        self.put(a, h)
+        return _Dataset(arr.dtype, arr.shape, address=daddr if arr.nbytes else None), a
+
+    def structure(self, entries):
+        """heap + symbol-table nodes + B-tree of a group whose children are
+        already in the file: entries {name: (header, cache type, scratch)}."""
+        names = sorted(entries, key=lambda s: s.encode())
+        seg, offs = b"\x00" * 8, {}
+        for nm in names:
+            enc = nm.encode()
+            offs[nm] = len(seg)
+            seg += enc + b"\x00" * (_pad8(len(enc) + 1) - len(enc))
+        heap = self.alloc(32)
+        dseg = self.alloc(len(seg))
+        self.put(heap, b"HEAP" + bytes([0, 0, 0, 0]) + struct.pack("<QQQ", len(seg), UNDEF, dseg))
+        self.put(dseg, seg)
+        if not names:
+            a = self.alloc(24 + 8 * (4 * INTERNAL_K + 1))
+            self.put(a, b"TREE" + bytes([0, 0]) + struct.pack("<HQQ", 0, UNDEF, UNDEF) +
+                     b"\x00" * (8 * (4 * INTERNAL_K + 1)))
+            return a, heap, {}
+        scratch_at, leaves = {}, []
+        for i in range(0, len(names), 2 * LEAF_K):
+            chunk = names[i:i + 2 * LEAF_K]
+            body = b"".join(struct.pack("<QQI4x", offs[nm], entries[nm][0], entries[nm][1]) + entries[nm][2]
+                            for nm in chunk)
+            body += b"\x00" * (40 * 2 * LEAF_K - len(body))
+            a = self.alloc(8 + len(body))
+            self.put(a, b"SNOD" + bytes([1, 0]) + struct.pack("<H", len(chunk)) + body)
+            for k, nm in enumerate(chunk):
+                scratch_at[nm] = a + 8 + 40 * k + 24
+            leaves.append((a, offs[chunk[-1]]))
+        level, nodes = 0, leaves
+        while True:
+            parents = []
+            for i in range(0, len(nodes), 2 * INTERNAL_K):
+                chunk = nodes[i:i + 2 * INTERNAL_K]
+                body = struct.pack("<Q", 0)
+                for child, last in chunk:
+                    body += struct.pack("<QQ", child, last)
+                body += b"\x00" * (8 * (4 * INTERNAL_K + 1) - len(body))
+                a = self.alloc(24 + len(body))
+                self.put(a, b"TREE" + bytes([0, level]) + struct.pack("<HQQ", len(chunk), UNDEF, UNDEF) + body)
+                parents.append((a, chunk[-1][1]))
+            if len(parents) == 1:
+                return parents[0][0], heap, scratch_at
+            nodes, level = parents, level + 1
+
+    def flush(self, g, pending):
+        """Writes what changed below group g; returns g's replacement when
+        its structure was rewritten (a _Group), else None.  `pending` maps
+        id(group) to the set of its entries assigned since it was read."""
+        changed = not isinstance(g, _Group)
+        assigned = pending.get(id(g), set())
+        for name in list(g):
+            node = g[name]
+            if isinstance(node, dict):
+                new = self.flush(node, pending)
+                if new is not None:
+                    if not isinstance(node, _Group):  # a new group: its object header
+                        h = _object_header([_msg(0x11, struct.pack("<QQ", new.btree, new.heap))])
+                        new.hdr = self.alloc(len(h))
+                        self.put(new.hdr, h)
+                        new.stab_at = new.hdr + 16 + 8
+                        changed = True
+                    else:
+                        new.hdr, new.stab_at, new.scratch_at = node.hdr, node.stab_at, node.scratch_at
+                        self.patch(new)
+                    g[name] = new
+            elif name in assigned:
+                arr = np.array(node, order="C")
+                old = self.original(g, name)
+                if (isinstance(old, _Dataset) and old.address is not None and old.compact is None
+                        and old.dtype == arr.dtype and old.shape == arr.shape):
+                    self.put(old.address, arr.astype(arr.dtype.newbyteorder("<")).tobytes())
+                    g[name] = old
+                else:
+                    ds, hdr = self.dataset(arr)
+                    ds.hdr = hdr
+                    g[name] = ds
+                    changed = True
+        if not changed:
+            return None
+        entries = {}
+        for name, node in g.items():
+            if isinstance(node, dict):
+                entries[name] = (node.hdr, 1, struct.pack("<QQ", node.btree, node.heap))
+            else:
+                entries[name] = (node.hdr, 0, b"\x00" * 16)
+        btree, heap, scratch_at = self.structure(entries)
+        out = _Group(g)
+        out.btree, out.heap = btree, heap
+        out.hdr, out.stab_at, out.scratch_at = getattr(g, "hdr", None), getattr(g, "stab_at", None), \
+            getattr(g, "scratch_at", None)
+        for name, node in out.items():
+            if isinstance(node, _Group):
+                node.scratch_at = scratch_at[name]
+        return out
+
+    def original(self, g, name):
+        return self.originals.get((id(g), name))
+
+    def patch(self, g):
+        b = struct.pack("<QQ", g.btree, g.heap)
+        self.put(g.stab_at, b)
+        if g.scratch_at is not None:
+            self.put(g.scratch_at, b)
+
+
 # ---------------------------------------------------------------- interface
 class File:
     """A checkpoint file: ``File(path, "r" | "a" | "w")``.  Datasets are
@@ -403,9 +557,12 @@ class File:
             raise ValueError("mode must be r, a or w")
         self.path, self.mode = path, mode
         self._tree, self._parser, self._dirty = {}, None, mode == "w"
+        # entries assigned since the file was read: names per parent group,
+        # and the dataset each replaced (for overwriting in place)
+        self._assigned, self._originals = {}, {}
         if mode == "r" or (mode == "a" and os.path.exists(path)):
             self._parser = _Parser(path)
-            self._tree = self._parser.object(self._parser.root)
+            self._tree = self._parser.object(self._parser.root, 80)
 
     def __enter__(self):
         return self
@@ -473,10 +630,33 @@ class File:
             arr = arr.astype("S1")
         if arr.dtype.kind not in "fiucS":
             raise H5FormatError("dtype %s" % arr.dtype)
+        if isinstance(parent.get(last), dict):
+            raise KeyError("%s is a group" % path)
+        key = (id(parent), last)
+        if key not in self._originals:
+            self._originals[key] = parent.get(last)
+        self._assigned.setdefault(id(parent), set()).add(last)
         parent[last] = arr
         self._dirty = True
 
     def close(self):
+        if self.mode != "r" and self._dirty and self._parser is not None:
+            # an existing file: append, rewriting only the changed groups
+            eof = max(self._parser.eof, os.path.getsize(self.path))
+            self._parser.f.close()
+            self._parser = None
+            with open(self.path, "r+b") as fh:
+                ap = _Appender(fh, eof)
+                ap.originals = self._originals
+                root = ap.flush(self._tree, self._assigned)
+                if root is not None:
+                    ap.patch(root)
+                    self._tree = root
+                if ap.pos != _pad8(eof):
+                    ap.put(40, struct.pack("<Q", ap.pos))
+            self._dirty = False
+            self._assigned, self._originals = {}, {}
+            return
         if self.mode != "r" and self._dirty:
             def materialise(node):
                 if isinstance(node, dict):

@@ -19,9 +19,21 @@ def wavenumbers(n):
 
 
 def solenoidal_vorticity(n, seed, k0=3.0, energy=0.5, kM=None, dtype=np.complex64, kcut=None):
-    """Modes are drawn for |kx|, |ky|, |kz| <= kc = min(kM, ceil(8 k0)) only:
-    beyond that E(k) ~ exp(-2 (k/k0)^2) < exp(-128), i.e. zero in double
-    precision, so large grids cost no more than small ones."""
+    """The curl i k x u of solenoidal_velocity(n, seed, ...), computed here on
+    the host.  Modes are drawn for |kx|, |ky|, |kz| <= kc = min(kM,
+    ceil(8 k0)) only: beyond that E(k) ~ exp(-2 (k/k0)^2) < exp(-128), i.e.
+    zero in double precision, so large grids cost no more than small ones."""
+    return _solenoidal(n, seed, k0, energy, kM, dtype, kcut, curl=True)
+
+
+def solenoidal_velocity(n, seed, k0=3.0, energy=0.5, kM=None, dtype=np.complex64, kcut=None):
+    """The Gaussian random solenoidal velocity u_hat itself (same draw as
+    solenoidal_vorticity); a solver's compute_vorticity turns it into the
+    initial vorticity on the device."""
+    return _solenoidal(n, seed, k0, energy, kM, dtype, kcut, curl=False)
+
+
+def _solenoidal(n, seed, k0, energy, kM, dtype, kcut, curl):
     kM = (n // 2 - 1) if kM is None else kM
     kc = int(min(kM, np.ceil(8 * k0))) if kcut is None else int(kcut)
     rng = np.random.default_rng(seed)
@@ -40,7 +52,7 @@ def solenoidal_vorticity(n, seed, k0=3.0, energy=0.5, kM=None, dtype=np.complex6
     E = 0.5 * float((nxm * (np.abs(u) ** 2).sum(axis=0)).sum())
     if E > 0:
         u *= np.sqrt(energy / E)
-    w = 1j * np.cross(kv, u, axis=0)  # vorticity = i k x u_hat
+    w = 1j * np.cross(kv, u, axis=0) if curl else u  # vorticity = i k x u_hat
     out = np.zeros((n, n, n // 2 + 1, 3), dtype=dtype)
     iy = np.where(k1 >= 0, k1, k1 + n).astype(np.int64)
     sub = np.moveaxis(w, 0, -1).astype(dtype)

@@ -27,6 +27,8 @@
 #ifndef BFPS_HIP_HPP
 #define BFPS_HIP_HPP
 
+#include <algorithm>
+#include <cmath>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -36,6 +38,7 @@
 #include <vector>
 
 #include "bfpsh.h"
+#include "bfps_hip_h5.hpp"
 
 namespace bfps_hip {
 
@@ -49,6 +52,16 @@ constexpr int ncomp(field_components fc) { return fc == ONE ? 1 : (fc == THREE ?
 class error : public std::runtime_error {
    public:
     using std::runtime_error::runtime_error;
+};
+
+/* A group of an open HDF5 file (the statistics group of the run file):
+ * what the reference's statistics writers take as an hid_t. */
+struct hdf5_group {
+    h5::File* file;
+    std::string path;
+    std::string at(const std::string& name) const {
+        return (path.empty() || path == "/") ? name : path + "/" + name;
+    }
 };
 
 namespace detail {
@@ -101,6 +114,9 @@ inline box* find_box(int nx, int ny, int nz, int prec) {
 enum class role { standalone, vorticity, velocity };
 
 }  // namespace detail
+
+template <field_backend be, kspace_dealias_type dt>
+class kspace;
 
 /* field<rnumber, HIP, fc> (bfps/cpp/field.hpp:53-289). */
 template <typename rnumber, field_backend be, field_components fc>
@@ -179,6 +195,89 @@ class field {
                                                           hist.data()),
                       "rspace_stats");
     }
+    /* field::compute_rspace_stats (field.cpp:661-850) into the statistics
+     * group: row toffset of moments/<dset_name> [t][10][nvals] and of
+     * histograms/<dset_name> [t][nbins][nvals]; nbins is the histogram
+     * dataset's second dimension, as there. */
+    void compute_rspace_stats(const hdf5_group& group, const std::string& dset_name, uint64_t toffset,
+                              const std::vector<double>& max_estimate) {
+        const std::vector<uint64_t> hs = group.file->shape(group.at("histograms/" + dset_name));
+        if (hs.size() < 2) throw error("histograms/" + dset_name + " has no bin dimension");
+        std::vector<double> moments;
+        std::vector<int64_t> hist;
+        compute_rspace_stats(max_estimate, int(hs[1]), moments, hist);
+        group.file->write_rows(group.at("moments/" + dset_name), toffset, 1, moments.data());
+        group.file->write_rows(group.at("histograms/" + dset_name), toffset, 1, hist.data());
+    }
+    /* field::compute_stats (field.cpp:984-1040): real-space statistics and
+     * the cospectrum of the field with itself (spectra/<name>_<name>), row
+     * toffset of the statistics group; the field ends in the representation
+     * it did not start in, with a forward transform normalised, as there. */
+    template <kspace_dealias_type dt>
+    void compute_stats(kspace<be, dt>* kk, const hdf5_group& group, const std::string& dset_name, uint64_t toffset,
+                       double max_estimate) {
+        std::vector<double> mev;
+        if (fc == ONE) mev.assign(1, max_estimate);
+        else if (fc == THREE) mev = {max_estimate, max_estimate, max_estimate, max_estimate * std::sqrt(3.0)};
+        else mev.assign(9, max_estimate);
+        bool did_rspace = false;
+        if (real_space_representation) {
+            compute_rspace_stats(group, dset_name, toffset, mev);
+            did_rspace = true;
+            dft();
+            normalize();
+        }
+        kk->cospectrum(*this, *this, group, dset_name + "_" + dset_name, toffset);
+        if (!did_rspace) {
+            ift();
+            compute_rspace_stats(group, dset_name, toffset, mev);
+        }
+    }
+    /* field::io (field.cpp:164-356): dataset <field_name>/complex/<it> (or
+     * /real/<it>, by the current representation) of fname; a complex read
+     * is symmetrized, as there.  Real data are stored unpadded, [nz][ny][nx]
+     * followed by the component dimensions. */
+    void io(const std::string& fname, const std::string& field_name, int iteration, bool read) {
+        const std::string dset = field_name + (real_space_representation ? "/real/" : "/complex/") +
+                                 std::to_string(iteration);
+        const size_t nc = ncomp(fc);
+        if (read) {
+            h5::File f(fname, 'r');
+            const std::vector<rnumber> v = f.read<rnumber>(dset);
+            if (real_space_representation) {
+                if (v.size() != size_t(npoints) * nc) throw error(dset + " has the wrong size");
+                host_.assign(rsize(), rnumber(0));
+                for (size_t zy = 0; zy < size_t(nz) * ny; zy++)
+                    std::memcpy(&host_[zy * (nx + 2) * nc], &v[zy * nx * nc], sizeof(rnumber) * nx * nc);
+            } else {
+                if (v.size() != csize()) throw error(dset + " has the wrong size");
+                host_ = v;
+            }
+            upload();
+            if (!real_space_representation && role_ == detail::role::standalone) symmetrize();
+            return;
+        }
+        download();
+        std::vector<uint64_t> shape;
+        h5::dtype t{'f', sizeof(rnumber)};
+        std::vector<rnumber> dense;
+        const rnumber* src = host_.data();
+        if (real_space_representation) {
+            shape = {uint64_t(nz), uint64_t(ny), uint64_t(nx)};
+            dense.resize(size_t(npoints) * nc);
+            for (size_t zy = 0; zy < size_t(nz) * ny; zy++)
+                std::memcpy(&dense[zy * nx * nc], &host_[zy * (nx + 2) * nc], sizeof(rnumber) * nx * nc);
+            src = dense.data();
+        } else {
+            shape = {uint64_t(ny), uint64_t(nz), uint64_t(nx / 2 + 1)};
+            t = {'c', 2 * sizeof(rnumber)};
+        }
+        if (fc == THREE) shape.push_back(3);
+        if (fc == THREExTHREE) shape.insert(shape.end(), {3, 3});
+        h5::File f(fname, 'a');
+        f.write(dset, t, shape, src);
+        f.close();
+    }
     /* field::symmetrize (field.cpp:900-978). */
     void symmetrize() {
         if (role_ != detail::role::standalone) throw error("symmetrize of a solver field: it is kept symmetric");
@@ -212,7 +311,8 @@ class field {
         } else if (role_ == detail::role::standalone) {
             detail::check(box_->ctx, bfpsh_field_put(box_->ctx, id_, 0, host_.data()), "field_put");
         } else {
-            throw error("cvelocity is computed from cvorticity");
+            // the `u` member compute_vorticity reads
+            detail::check(box_->ctx, bfpsh_put_cfield(box_->ctx, BFPSH_CVELOCITY, host_.data()), "put_cfield");
         }
     }
 
@@ -339,6 +439,25 @@ class kspace {
         }
     }
 
+    /* kspace::cospectrum into the statistics group (kspace.cpp:521-598):
+     * row toffset of spectra/<dset_name>, [t][nshells][3][3]. */
+    template <typename rnumber>
+    void cospectrum(field<rnumber, be, THREE>& a, field<rnumber, be, THREE>& b, const hdf5_group& group,
+                    const std::string& dset_name, uint64_t toffset) {
+        std::vector<double> spec(size_t(nshells) * 9);
+        cospectrum(a, b, spec.data());
+        group.file->write_rows(group.at("spectra/" + dset_name), toffset, 1, spec.data());
+    }
+
+    /* kspace::store (kspace.cpp:178-230): kspace/kshell, nshell, kM and dk
+     * of the run file */
+    void store(h5::File& stat_file) {
+        stat_file.write<double>("kspace/kshell", {uint64_t(nshells)}, kshell.data());
+        stat_file.write<int64_t>("kspace/nshell", {uint64_t(nshells)}, nshell.data());
+        stat_file.write<double>("kspace/kM", {}, &kM);
+        stat_file.write<double>("kspace/dk", {}, &dk);
+    }
+
     /* CLOOP / CLOOP_K2 / CLOOP_K2_NXMODES (kspace.hpp:117-192) on the host. */
     template <class func_type>
     void CLOOP(func_type expression) {
@@ -437,13 +556,70 @@ class vorticity_equation {
         cvelocity->real_space_representation = false;
     }
 
-    /* io_checkpoint(read=true) without the file (vorticity_equation.hpp:107-128):
-     * cvorticity <- data, then symmetrize, low_pass(kM), force_divfree. */
-    void io_checkpoint(bool read, const typename field<rnumber, be, THREE>::cnumber* data = nullptr) {
-        if (!read) throw error("checkpoint writing: read cvorticity->get_cdata()");
-        if (!data) throw error("io_checkpoint(read) needs the vorticity data (no HDF5 in this backend)");
+    /* get_current_fname (vorticity_equation.hpp:98-104) */
+    std::string get_current_fname() const { return name + "_checkpoint_" + std::to_string(checkpoint) + ".h5"; }
+
+    /* update_checkpoint (vorticity_equation.cpp:49-108): the next file once
+     * the current one holds checkpoints_per_file fields and not this
+     * iteration's; a missing file is created with the group
+     * vorticity/complex. */
+    void update_checkpoint() {
+        const std::string fname = get_current_fname();
+        std::FILE* probe = std::fopen(fname.c_str(), "rb");
+        if (probe) {
+            std::fclose(probe);
+            h5::File f(fname, 'r');
+            const std::vector<std::string> stored = f.exists("vorticity/complex") ? f.keys("vorticity/complex")
+                                                                                    : std::vector<std::string>();
+            bool here = false;
+            for (const std::string& k : stored) here = here || k == std::to_string(iteration);
+            if (int(stored.size()) >= checkpoints_per_file && !here) checkpoint++;
+        } else {
+            h5::File f(fname, 'w');
+            f.create_group("vorticity/complex");
+            f.close();
+        }
+    }
+
+    /* io_checkpoint (vorticity_equation.hpp:107-128): read = true loads
+     * vorticity/complex/<iteration> of the current checkpoint file, then
+     * symmetrize, low_pass(kM), force_divfree; read = false updates the
+     * checkpoint index and writes cvorticity there. */
+    void io_checkpoint(bool read = true) {
+        if (cvorticity->real_space_representation) throw error("io_checkpoint of a real-space vorticity");
+        if (!read) update_checkpoint();
+        const std::string fname = get_current_fname();
+        if (read) {
+            h5::File f(fname, 'r');
+            const std::vector<rnumber> v = f.read<rnumber>("vorticity/complex/" + std::to_string(iteration));
+            if (v.size() != size_t(box_.ny) * box_.nz * (box_.nx / 2 + 1) * 3 * 2)
+                throw error("vorticity/complex/" + std::to_string(iteration) + " has the wrong size");
+            io_checkpoint(true, reinterpret_cast<const typename field<rnumber, be, THREE>::cnumber*>(v.data()));
+        } else {
+            cvorticity->io(fname, "vorticity", iteration, false);
+        }
+    }
+    /* the read of io_checkpoint from an array in the complex layout
+     * instead of the file (no reference counterpart) */
+    void io_checkpoint(bool read, const typename field<rnumber, be, THREE>::cnumber* data) {
+        if (!read) throw error("io_checkpoint(false, data): call io_checkpoint(false)");
+        if (!data) throw error("io_checkpoint(true, data) needs data");
         *cvorticity = data;
         detail::check(box_.ctx, bfpsh_load_projection(box_.ctx), "io_checkpoint");
+    }
+
+    /* compute_vorticity (vorticity_equation.cpp:171-204): cvorticity = i k x
+     * cvelocity (k^2 <= kM^2), symmetrized; cvelocity is what was last
+     * assigned to it (*cvelocity = data) */
+    void compute_vorticity() {
+        box_.flush();
+        detail::check(box_.ctx, bfpsh_compute_vorticity(box_.ctx), "compute_vorticity");
+        cvorticity->real_space_representation = false;
+    }
+    /* impose_zero_modes (vorticity_equation.cpp:39-46) */
+    void impose_zero_modes() {
+        box_.flush();
+        detail::check(box_.ctx, bfpsh_impose_zero_modes(box_.ctx), "impose_zero_modes");
     }
 
     /* compute_pressure (vorticity_equation.cpp:441-520) of the velocity of
@@ -646,6 +822,132 @@ std::unique_ptr<abstract_particles_system<long long int, double>> particles_syst
     std::unique_ptr<abstract_particles_system<long long int, double>> out(ps);
     if (state) ps->put(state, rhs, nullptr);
     return out;
+}
+
+/* particles_system_builder (particles_system_builder.hpp:238-257): the
+ * tracer state [np][3] and history [nsteps][np][3] are read from datasets
+ * state_dset and rhs_dset of fname (particles_input_hdf5; tracer i of the
+ * file gets global index i).  A tracer count or history length that
+ * differs from the parameters throws here (the reference constructs that
+ * error without throwing it, particles_system_builder.hpp:217-225). */
+template <typename rnumber, typename comm_t>
+std::unique_ptr<abstract_particles_system<long long int, double>> particles_system_builder(
+    field<rnumber, HIP, THREE>* fs_field, kspace<HIP, SMOOTH>* fs_kk, int nsteps, long long int nparticles,
+    const std::string& fname, const std::string& state_dset, const std::string& rhs_dset, int interpolation_size,
+    int spline_mode, comm_t comm, int in_current_iteration) {
+    h5::File f(fname, 'r');
+    const std::vector<uint64_t> ss = f.shape(state_dset), rs = f.shape(rhs_dset);
+    if (ss.empty() || ss.back() != 3) throw error(state_dset + " is not [..][3]");
+    uint64_t np_file = 1;
+    for (size_t i = 0; i + 1 < ss.size(); i++) np_file *= ss[i];
+    if ((long long int)np_file != nparticles)
+        throw error("Nb particles is " + std::to_string(nparticles) + " in the parameters but " +
+                    std::to_string(np_file) + " in the particles file.");
+    if (rs.empty() || (long long int)rs[0] != nsteps)
+        throw error("Nb steps is " + std::to_string(nsteps) + " in the parameters but " +
+                    std::to_string(rs.empty() ? 0 : rs[0]) + " in the particles file.");
+    const std::vector<double> state = f.read<double>(state_dset);
+    const std::vector<double> rhs = f.read<double>(rhs_dset);
+    if (rhs.size() != size_t(nsteps) * np_file * 3) throw error(rhs_dset + " has the wrong size");
+    return particles_system_builder(fs_field, fs_kk, nsteps, nparticles, state.data(), rhs.data(), interpolation_size,
+                                    spline_mode, comm, in_current_iteration);
+}
+
+/* particles_output_hdf5<partsize_t, real_number, 3, 3>
+ * (particles/particles_output_hdf5.hpp, abstract_particles_output.hpp:144-190):
+ * open_file creates <species>/state and <species>/rhs when missing; save
+ * writes <species>/state/<it> [np][3] and <species>/rhs/<it>
+ * [nb_rhs][np][3], tracers in global-index order. */
+template <class partsize_t, class real_number, int size_particle_positions, int size_particle_rhs>
+class particles_output_hdf5 {
+   public:
+    template <typename comm_t>
+    particles_output_hdf5(comm_t, const std::string& ps_name, partsize_t total_nb_particles, int nb_rhs,
+                          bool = false)
+        : name_(ps_name), np_(total_nb_particles), nb_rhs_(nb_rhs) {}
+
+    int open_file(const std::string& filename) {
+        file_.reset(new h5::File(filename, 'a'));
+        file_->create_group(name_ + "/state");
+        file_->create_group(name_ + "/rhs");
+        return 0;
+    }
+    void save(const real_number positions[], const std::unique_ptr<real_number[]> rhs[], const partsize_t index[],
+              partsize_t nb_particles, int idx_time_step) {
+        if (!file_) throw error("particles_output_hdf5::save before open_file");
+        if (nb_particles != np_) throw error("particles_output_hdf5: one rank holds every tracer here");
+        std::vector<partsize_t> order(static_cast<size_t>(nb_particles));
+        for (partsize_t i = 0; i < nb_particles; i++) order[size_t(i)] = i;
+        std::sort(order.begin(), order.end(), [&](partsize_t a, partsize_t b) { return index[a] < index[b]; });
+        std::vector<real_number> st(size_t(nb_particles) * size_particle_positions);
+        std::vector<real_number> rh(size_t(nb_rhs_) * nb_particles * size_particle_rhs);
+        for (partsize_t i = 0; i < nb_particles; i++) {
+            const partsize_t src = order[size_t(i)];
+            for (int c = 0; c < size_particle_positions; c++)
+                st[size_t(i) * size_particle_positions + c] = positions[src * size_particle_positions + c];
+            for (int r = 0; r < nb_rhs_; r++)
+                for (int c = 0; c < size_particle_rhs; c++)
+                    rh[(size_t(r) * nb_particles + i) * size_particle_rhs + c] = rhs[r][src * size_particle_rhs + c];
+        }
+        const std::string it = std::to_string(idx_time_step);
+        file_->write<real_number>(name_ + "/state/" + it, {uint64_t(nb_particles), uint64_t(size_particle_positions)},
+                                  st.data());
+        file_->write<real_number>(name_ + "/rhs/" + it,
+                                  {uint64_t(nb_rhs_), uint64_t(nb_particles), uint64_t(size_particle_rhs)}, rh.data());
+    }
+    int close_file() {
+        if (file_) file_->close();
+        file_.reset();
+        return 0;
+    }
+
+   private:
+    std::string name_;
+    partsize_t np_;
+    int nb_rhs_;
+    std::unique_ptr<h5::File> file_;
+};
+
+/* sample_from_particles_system (particles/particles_sampling.hpp:15-50): the
+ * field at the tracers into <parent>/<fname>/<step_idx> of filename,
+ * [1][np][ncomp] in global-index order; an existing dataset is kept. */
+template <class partsize_t, class particles_rnumber, class rnumber, field_components fc>
+void sample_from_particles_system(const field<rnumber, HIP, fc>& in_field,
+                                  std::unique_ptr<abstract_particles_system<partsize_t, particles_rnumber>>& ps,
+                                  const std::string& filename, const std::string& parent_groupname,
+                                  const std::string& fname) {
+    const std::string dset = parent_groupname + "/" + fname + "/" + std::to_string(ps->get_step_idx());
+    {
+        std::FILE* probe = std::fopen(filename.c_str(), "rb");
+        if (probe) {
+            std::fclose(probe);
+            h5::File f(filename, 'r');
+            if (f.exists(dset)) return;
+        }
+    }
+    const partsize_t np = ps->getLocalNbParticles();
+    const int nc = ncomp(fc);
+    std::vector<particles_rnumber> sample(size_t(np) * nc, 0);
+    ps->sample_compute_field(in_field, sample.data());  // rows in global-index order
+    h5::File f(filename, 'a');
+    f.write<particles_rnumber>(dset, {1, uint64_t(np), uint64_t(nc)}, sample.data());
+    f.close();
+}
+
+/* compute_gradient (field.cpp:1042-1101) of standalone fields of one box:
+ * ONE -> THREE or THREE -> THREExTHREE, dst[j ncomp(src) + i] = i k_j
+ * src_i for k^2 < kM^2 (dst spectral). */
+template <typename rnumber, field_backend be, field_components fc1, field_components fc2, kspace_dealias_type dt>
+int compute_gradient(kspace<be, dt>*, field<rnumber, be, fc1>* src, field<rnumber, be, fc2>* dst) {
+    static_assert((fc1 == ONE && fc2 == THREE) || (fc1 == THREE && fc2 == THREExTHREE), "gradient shapes");
+    if (src->owner() != dst->owner() || src->handle() < 0 || dst->handle() < 0)
+        throw error("compute_gradient needs two standalone fields of one box");
+    if (src->real_space_representation) throw error("compute_gradient of a real-space field");
+    src->owner()->flush();
+    detail::check(src->owner()->ctx, bfpsh_field_compute_gradient(src->owner()->ctx, src->handle(), dst->handle()),
+                  "compute_gradient");
+    dst->real_space_representation = false;
+    return 0;
 }
 
 }  // namespace bfps_hip

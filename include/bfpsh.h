@@ -76,7 +76,9 @@ int bfpsh_set_physics(bfpsh_ctx* ctx, double nu, int forcing, double famplitude,
 int bfpsh_kspace_info(bfpsh_ctx* ctx, double* out4, double* kshell, int64_t* nshell);
 
 /* Replaces `*cvorticity = host_data` / field::io read of vorticity/complex
- * (bfps/cpp/field.hpp:193-200, field.cpp:336-341) without the projection. */
+ * (bfps/cpp/field.hpp:193-200, field.cpp:336-341) without the projection;
+ * which = BFPSH_CVELOCITY sets `*cvelocity` (the `u` member the
+ * vorticity_equation reads in compute_vorticity). */
 int bfpsh_put_cfield(bfpsh_ctx* ctx, int which, const void* host);
 /* Replaces reading field::get_cdata() of cvorticity / cvelocity
  * (bfps/cpp/field.hpp:140-146).  The kx = nx/2 plane is returned as zero. */
@@ -89,6 +91,16 @@ int bfpsh_get_rfield(bfpsh_ctx* ctx, int which, void* host);
  * (bfps/cpp/vorticity_equation.hpp:107-128): symmetrize (field.cpp:340),
  * low_pass(kM), force_divfree. */
 int bfpsh_load_projection(bfpsh_ctx* ctx);
+/* Replaces vorticity_equation::compute_vorticity()
+ * (bfps/cpp/vorticity_equation.cpp:171-204): cvorticity = i k x cvelocity
+ * for k^2 <= kM^2, zero beyond, then symmetrize.  cvelocity is what the last
+ * bfpsh_put_cfield(BFPSH_CVELOCITY) set (a step or a velocity computation
+ * in between overwrites it, as they overwrite the reference's `u`). */
+int bfpsh_compute_vorticity(bfpsh_ctx* ctx);
+/* Replaces vorticity_equation::impose_zero_modes()
+ * (bfps/cpp/vorticity_equation.cpp:39-46): the k = 0 mode of cvelocity,
+ * cvorticity and the two stage fields set to zero. */
+int bfpsh_impose_zero_modes(bfpsh_ctx* ctx);
 /* Replaces vorticity_equation::step(dt) (bfps/cpp/vorticity_equation.cpp:358-439). */
 int bfpsh_step(bfpsh_ctx* ctx, double dt);
 /* Replaces `compute_velocity(cvorticity); cvelocity->ift();`
@@ -121,6 +133,11 @@ int bfpsh_field_symmetrize(bfpsh_ctx* ctx, int id);
  * [nz][ny][nx][ncomp]; 0 -> rnumber {re, im} [ny][nz][nx/2+1][ncomp]
  * (field::get_rdata / get_cdata, bfps/cpp/field.hpp:135-146). */
 int bfpsh_field_put(bfpsh_ctx* ctx, int id, int real_space, const void* host);
+/* Replaces the free function compute_gradient(kk, src, dst)
+ * (bfps/cpp/field.cpp:1042-1101) on standalone fields: src ONE -> dst THREE
+ * or src THREE -> dst THREExTHREE (component j ncomp(src) + i = i k_j src_i
+ * for k^2 < kM^2, zero elsewhere); src spectral, dst left spectral. */
+int bfpsh_field_compute_gradient(bfpsh_ctx* ctx, int src_id, int dst_id);
 int bfpsh_field_get(bfpsh_ctx* ctx, int id, int real_space, void* host);
 /* Replaces `*tmp_vec_field = fs->cvorticity->get_cdata()` and the same for
  * cvelocity (bfps/cpp/full_code/NSVE.cpp:115,124): which = BFPSH_CVORTICITY

@@ -6,6 +6,8 @@ n = 32 -- the GPU test that runs the golden case through the driver checks
 the trajectory they produce at the reference's 1e-5."""
 import numpy as np
 
+import bfps_amd
+
 from bfps_amd import dns
 
 NU = (2.0 * 2 / 32) ** (4. / 3)
@@ -56,7 +58,7 @@ def test_write_par_run_file(tmp_path):
         assert f["parameters/nu"] == NU and f["parameters/nu"].dtype == np.float64
         assert f["parameters/nx"] == 32 and f["parameters/nx"].dtype == np.int64
         assert f["parameters/forcing_type"] == b"linear"
-        assert f["bfps_info/exec_name"] == b"NSVEparticles"
+        assert f["bfps_info/exec_name"] == ("NSVEparticles-single-v%s" % bfps_amd.__version__).encode()
         # wavenumbers: k_i = i up to the Nyquist index (positive), i - n above
         n = 32
         signed = np.array([i if i <= n // 2 else i - n for i in range(n)], dtype=np.float64)

@@ -192,3 +192,62 @@ def test_reference_test_flow_through_launch_jobs(B, tmp_path):
                 assert err < 1e-5, (name % it, err)
     with h5io.File(os.path.join(wd, "dns_nsveparticles.h5")) as f:
         assert int(f["iteration"]) == 64 and f["statistics/moments/velocity"].shape == (65, 10, 4)
+
+
+def test_cpp_job_reproduces_the_reference_test(B, tmp_path):
+    """The reference's test flow with the job itself in C++
+    (tests/cpp/nsve_particles_job.cpp: direct_numerical_simulation, NSVE and
+    NSVEparticles call for call over include/bfps_hip.hpp, files through
+    include/bfps_hip_h5.hpp): the launcher prepares the run from the golden
+    iteration 0, the C++ job runs twice for 32 iterations, and iterations
+    32 and 64 of its checkpoint file match the golden file at the
+    reference's 1e-5 and the Python driver's files bit for bit; the run
+    file holds the iteration, the checkpoint index and 65 statistics rows,
+    the sample file the velocity and acceleration at the tracers."""
+    import subprocess
+    from bfps_amd import build as hb
+    from bfps_amd import dns, h5io
+    say("case: C++ NSVEparticles job, 2 jobs x 32 iterations from the golden file")
+    job = os.path.join(HERE, "cpp", "nsve_particles_job")
+    if not os.path.exists(job):
+        hb.build_driver()
+    gold = os.path.dirname(REF)
+    p = dns.launch_parameters(32, nparticles=10000, niter_todo=32, niter_out=32, niter_stat=1, niter_part=32,
+                              checkpoints_per_file=3)
+    wd_c, wd_py = str(tmp_path / "cpp"), str(tmp_path / "py")
+    for j in range(2):
+        dns.launch_jobs("dns_nsveparticles", p, work_dir=wd_c, src_simname="B32p1e4", src_work_dir=gold,
+                        src_iteration=0, particle_rand_seed=2)
+        r = subprocess.run([job, "dns_nsveparticles", "single"], cwd=wd_c, capture_output=True, text=True,
+                           timeout=300)
+        say("  C++ job %d: rc %d %s" % (j, r.returncode, r.stderr.strip()[-300:]))
+        assert r.returncode == 0
+        run = dns.launch_jobs("dns_nsveparticles", p, work_dir=wd_py, src_simname="B32p1e4", src_work_dir=gold,
+                              src_iteration=0, particle_rand_seed=2)
+        run.initialize()
+        run.main_loop()
+        run.finalize()
+    ck_c = os.path.join(wd_c, "dns_nsveparticles_checkpoint_0.h5")
+    with h5io.File(REF) as f0, h5io.File(ck_c) as fc, \
+            h5io.File(os.path.join(wd_py, "dns_nsveparticles_checkpoint_0.h5")) as fp:
+        for it in (0, 32, 64):
+            for name in ("vorticity/complex/%d", "tracers0/state/%d", "tracers0/rhs/%d"):
+                err = np.abs(f0[name % it] - fc[name % it]).max()
+                say("  %s: %.3e vs golden" % (name % it, err))
+                assert err < 1e-5, (name % it, err)
+                assert np.array_equal(fc[name % it], fp[name % it]), name % it
+    with h5io.File(os.path.join(wd_c, "dns_nsveparticles.h5")) as fc, \
+            h5io.File(os.path.join(wd_py, "dns_nsveparticles.h5")) as fp:
+        assert int(fc["iteration"]) == 64 and int(fc["checkpoint"]) == 0
+        for key in ("statistics/moments/velocity", "statistics/spectra/vorticity_vorticity",
+                    "statistics/histograms/velocity"):
+            a, b = fc[key], fp[key]
+            assert a.shape == b.shape == ((65,) + a.shape[1:])
+            assert np.allclose(a, b, rtol=1e-9, atol=0), key
+    with h5io.File(os.path.join(wd_c, "dns_nsveparticles_particles.h5")) as fc, \
+            h5io.File(os.path.join(wd_py, "dns_nsveparticles_particles.h5")) as fp:
+        for name in ("velocity", "acceleration"):
+            assert fc.keys("tracers0/" + name) == fp.keys("tracers0/" + name) == ["1", "33", "65"]
+            for k in ("1", "33", "65"):
+                key = "tracers0/%s/%s" % (name, k)
+                assert np.allclose(fc[key], fp[key], rtol=1e-9, atol=1e-12), key

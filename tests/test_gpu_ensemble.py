@@ -89,3 +89,51 @@ def test_ensemble_errors(B):
         B.ensemble([32, 40])
     with pytest.raises(B.capi.BfpshError):
         B.ensemble([])
+
+
+def test_ensemble_full_manifest(B, oracle):
+    """The 96-box ensemble at its full size (32 x 32^3, 32 x 48^3, 24 x 64^3,
+    8 x 128^3; per-box nu, k0 and log-uniform tracer counts, n1 m1 AB4):
+    three batched graph steps against the same manifest advanced box by box
+    (same kernels: fields and tracers bit for bit), and the first 128^3 box
+    against the FP32 oracle (vorticity within 3e-6 relative L2, tracer
+    positions within 1e-7, velocities within 1e-5 of their maximum)."""
+    from bfps_amd import dns, ic
+    boxes = dns.ensemble_manifest(seed=5)
+    assert len(boxes) == 96 and sum(b["n"] ** 3 for b in boxes) == 27656192
+    say("case: 96-box ensemble, %d tracers" % sum(b["nparticles"] for b in boxes))
+    ens, pe, dt = dns.build_ensemble(boxes)
+    sep, psep, _ = dns.build_ensemble(boxes)
+    nsteps = 3
+    for s in range(nsteps):
+        ens.step(dt)
+        sep.step_separate(dt)
+    ens.synchronize()
+    sep.synchronize()
+    for i in range(len(boxes)):
+        a, b = ens.boxes[i].cvorticity, sep.boxes[i].cvorticity
+        assert np.array_equal(a, b), (i, np.abs(a - b).max())
+        if pe[i] is not None:
+            assert np.array_equal(pe[i].get()[0], psep[i].get()[0]), i
+    say("  batched == box by box for all 96 boxes")
+    i = next(k for k, b in enumerate(boxes) if b["n"] == 128)
+    b = boxes[i]
+    o = oracle.Solver(128, precision="single")
+    o.set_physics(b["nu"], "linear", b["famplitude"], b["fk0"], b["fk1"])
+    o.set_cfield("vorticity", ic.solenoidal_vorticity(128, seed=b["seed"], k0=b["k0"]))
+    o.io_projection()
+    op = oracle.Particles(o, b["nparticles"], b["neighbours"], b["smoothness"], b["integration_steps"], step_idx=1)
+    op.set_state(ic.uniform_positions(b["nparticles"], seed=b["particle_seed"]))
+    for s in range(nsteps):
+        oracle.nsve_particles_step(o, op, b["dt"])
+    ref = o.get_cfield("vorticity")
+    e = np.linalg.norm(ens.boxes[i].cvorticity - ref) / np.linalg.norm(ref)
+    gp, grhs, _, gsi = pe[i].get()
+    opos, orhs, osi = op.get_state()
+    ep, er = np.abs(gp - opos).max(), np.abs(grhs - orhs).max() / np.abs(orhs).max()
+    say("  box %d (128^3, %d tracers) vs oracle: vorticity %.3e pos %.3e rhs %.3e"
+        % (i, b["nparticles"], e, ep, er))
+    ens.close()
+    sep.close()
+    assert gsi == osi == nsteps + 1
+    assert e < 3e-6 and ep < 1e-7 and er < 1e-5

@@ -200,7 +200,9 @@ def test_cospectrum(B, oracle):
     assert np.allclose(fs.kk.kshell, ks) and np.array_equal(fs.kk.nshell, ns)
 
 
-@pytest.mark.parametrize("nb,m", [(1, 0), (1, 1), (2, 2), (4, 1), (6, 2), (3, 0)])
+# n <= 6 run the LDS-brick kernel; n >= 7 (a tile beyond 80 KB) the
+# global-gather kernel
+@pytest.mark.parametrize("nb,m", [(1, 0), (1, 1), (2, 2), (4, 1), (6, 2), (3, 0), (5, 1), (7, 1), (8, 2), (10, 1)])
 def test_tracers_vs_oracle(B, oracle, nb, m):
     n = 32
     say("case: tracers n=%d m=%d" % (nb, m))
@@ -270,10 +272,11 @@ def test_errors_are_reported(B):
         B.particles_system(fs, 10, 11, 1)
 
 
-@pytest.mark.parametrize("n,nb,m", [(64, 6, 2), (64, 4, 1), (48, 1, 1)])
+@pytest.mark.parametrize("n,nb,m", [(64, 6, 2), (64, 4, 1), (48, 1, 1), (64, 5, 2), (48, 7, 1), (32, 10, 2)])
 def test_tracers_fp32_bricks(B, oracle, n, nb, m):
-    """FP32 fields, many tracers per brick and several bricks per axis; the
-    interpolated velocities / gradients are FP64 sums of FP32 field values."""
+    """FP32 fields, many tracers per brick and several bricks per axis (n <=
+    6; n >= 7 the global-gather kernel); the interpolated velocities /
+    gradients are FP64 sums of FP32 field values."""
     from bfps_amd import ic
     say("case: fp32 tracers n=%d nb=%d m=%d" % (n, nb, m))
     w0 = ic.solenoidal_vorticity(n, seed=21, k0=4.0)
@@ -432,4 +435,68 @@ def test_largest_grid_beltrami_and_tracers(B, n):
     assert len(idx[0]) == 10
     assert e_modes < 1e-6 and e_rest < 1e-6
     del got
+    fs.close()
+
+
+@pytest.mark.parametrize("n,precision,tol", [(48, "double", 1e-14), (64, "single", 1e-6), (32, "double", 1e-14)])
+def test_compute_vorticity_vs_oracle(B, oracle, n, precision, tol):
+    """compute_vorticity (the curl of cvelocity, cut at kM, symmetrized) of a
+    broadband velocity against the oracle; the ABC velocity is its own
+    curl."""
+    say("case: compute_vorticity n=%d %s" % (n, precision))
+    u = random_vorticity(n, seed=31 + n, k0=n / 4.0)
+    fs = B.vorticity_equation("t", n, precision=precision)
+    fs.compute_vorticity(velocity=u)
+    o = oracle.Solver(n, precision=precision)
+    o.set_cfield("velocity", u)
+    o.compute_vorticity()
+    e = rel(fs.cvorticity, o.get_cfield("vorticity"))
+    say("  rel vs oracle %.3e" % e)
+    assert e < tol
+    w = abc_modes(n, fs.cdtype)
+    fs.compute_vorticity(velocity=w)
+    e = rel(fs.cvorticity, w.astype(np.complex128))
+    say("  ABC: curl u = u, rel %.3e" % e)
+    assert e < tol
+    fs.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-12), ("single", 2e-6)])
+def test_field_compute_gradient_vs_oracle(B, oracle, precision, tol):
+    """compute_gradient(kk, src, dst) on standalone fields (THREE ->
+    THREExTHREE) of the velocity, transformed to real space, against the
+    oracle's velocity gradient."""
+    n = 48
+    say("case: field compute_gradient %s" % precision)
+    w0 = random_vorticity(n, seed=12)
+    fs = B.vorticity_equation("t", n, precision=precision)
+    fs.io_checkpoint(read=True, data=w0)
+    o = oracle.Solver(n, precision=precision)
+    o.set_cfield("vorticity", w0)
+    o.io_projection()
+    o.gradient_rspace()
+    src = B.field(fs, "THREE")
+    src.put(fs.compute_velocity(), real_space=False)
+    dst = B.field(fs, "THREExTHREE")
+    B.compute_gradient(fs.kk, src, dst)
+    dst.ift()
+    e = rel(dst.get(), o.get_rfield("gradient"))
+    say("  rel %.3e" % e)
+    assert e < tol
+    src.close()
+    dst.close()
+    fs.close()
+
+
+def test_impose_zero_modes(B):
+    n = 32
+    w = random_vorticity(n, seed=13)
+    w[0, 0, 0] = [1.0, -2.0, 0.5]
+    fs = B.vorticity_equation("t", n, precision="double")
+    fs.cvorticity = w
+    fs.impose_zero_modes()
+    got = fs.cvorticity
+    assert np.all(got[0, 0, 0] == 0)
+    got[0, 0, 0] = w[0, 0, 0]
+    assert np.array_equal(got[..., :n // 2, :], w[..., :n // 2, :])
     fs.close()

@@ -150,3 +150,36 @@ def test_errors(tmp_path):
     with pytest.raises(h5io.H5FormatError):
         with h5io.File(str(tmp_path / "y.h5"), "w") as f:
             f["s"] = np.array(["text"])
+
+
+def test_append_costs_only_the_new_bytes(tmp_path, h5mini):
+    """Mode "a" on an existing file appends: the file grows by the new data
+    plus a few KB of group structure, a dataset of the same type and shape
+    is overwritten in place (no growth), and the reference file's datasets,
+    the new ones and the replaced one read equal through both readers."""
+    path = str(tmp_path / "ck.h5")
+    shutil.copy(REF, path)
+    size0 = os.path.getsize(path)
+    w = (np.arange(32 * 32 * 17 * 3) * (1 + 0.5j)).astype(np.complex64).reshape(32, 32, 17, 3)
+    st = np.linspace(0, 1, 30000).reshape(10000, 3)
+    with h5io.File(path, "a") as f:
+        f["vorticity/complex/96"] = w
+        f["tracers0/state/96"] = st
+        f["new/deep/x"] = np.arange(5)
+    grown = os.path.getsize(path) - size0
+    assert w.nbytes + st.nbytes <= grown < w.nbytes + st.nbytes + 64 * 1024
+    size1 = os.path.getsize(path)
+    with h5io.File(path, "a") as f:
+        f["tracers0/state/96"] = -st  # same type and shape: in place
+    assert os.path.getsize(path) == size1
+    with h5io.File(path, "a") as f:
+        f["new/deep/x"] = np.arange(7)  # another shape: a new dataset
+    g = np.load(os.path.join(GOLD, "B32p1e4.npz"))
+    with h5io.File(path) as f:
+        for read in (lambda n: f[n], lambda n: mini_read(h5mini, path, n)):
+            assert np.array_equal(read("vorticity/complex/96"), w)
+            assert np.array_equal(read("tracers0/state/96"), -st)
+            assert np.array_equal(read("new/deep/x"), np.arange(7))
+            assert np.array_equal(read("vorticity/complex/64"), g["vort_64"])
+            assert np.array_equal(read("tracers0/rhs/32"), g["rhs_32"])
+        assert f.keys("vorticity/complex") == ["0", "32", "64", "96"]

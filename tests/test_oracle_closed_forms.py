@@ -456,3 +456,56 @@ def test_rspace_stats_unit_magnitude(oracle):
     b = int(np.floor(1.0 * 32 / np.sqrt(3.0)))            # magnitude bins of width sqrt 3 / 32
     assert hist[b, 3] == n ** 3 and hist[:, 3].sum() == n ** 3
     assert hist[16, 2] == n ** 3                          # u_z = 0 -> bin 16 of 32 on [-1, 1)
+
+
+# ---------------------------------------------------------------- compute_vorticity
+def test_compute_vorticity_closed_forms(oracle):
+    """compute_vorticity (vorticity_equation.cpp:171-204) against curls fixed
+    by the mathematics: the ABC flow is its own curl; u = (sin 2y cos z, 0,
+    cos x) has curl (0, sin x - sin 2y sin z, -2 cos 2y cos z) (transformed
+    by numpy, not by the oracle)."""
+    n = 16
+    o = oracle.Solver(n, precision="double")
+    u = abc_vorticity(n)
+    o.set_cfield("velocity", u)
+    o.compute_vorticity()
+    assert rel(o.get_cfield("vorticity"), u) < 1e-14
+    x, y, z = grid(n)
+    ur = np.stack([np.sin(2 * y) * np.cos(z), np.zeros_like(x), np.cos(x)], axis=-1)
+    wr = np.stack([np.zeros_like(x), np.sin(x) - np.sin(2 * y) * np.sin(z), -2 * np.cos(2 * y) * np.cos(z)], axis=-1)
+    o.set_cfield("velocity", spectral(ur))
+    o.compute_vorticity()
+    assert rel(o.get_cfield("vorticity"), spectral(wr)) < 1e-14
+
+
+@pytest.mark.parametrize("dealias", ["SMOOTH", "TWO_THIRDS"])
+def test_compute_vorticity_inverts_biot_savart(oracle, dealias):
+    """compute_velocity(compute_vorticity(u)) = u for a solenoidal u with a
+    zero mean and modes only below kM; modes at k^2 > kM^2 of the curl of a
+    broadband u are zero."""
+    n = 24
+    o = oracle.Solver(n, precision="double", dealias=dealias)
+    rng = np.random.default_rng(4)
+    shp = (n, n, n // 2 + 1, 3)
+    u = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+    kx = np.arange(n // 2 + 1)
+    k1 = np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n)
+    K = np.stack(np.meshgrid(k1, k1, kx, indexing="ij"), -1)[..., [2, 0, 1]].astype(float)  # (kx, ky, kz)
+    k2 = (K ** 2).sum(-1)
+    # broadband input: the curl vanishes beyond kM
+    o.set_cfield("velocity", u)
+    o.compute_vorticity()
+    w = o.get_cfield("vorticity")
+    assert np.abs(w[k2 > o.kM2]).max() == 0.0
+    # band-limited solenoidal input (projected here, and made Hermitian by
+    # the field's own symmetrize): the Biot-Savart of the curl returns it
+    u = u * (k2 < o.kM2 * 0.8)[..., None]
+    u -= K * ((K * u).sum(-1) / np.where(k2 > 0, k2, 1))[..., None]
+    u[0, 0, 0] = 0
+    u = oracle.symmetrize(u, n)
+    o.set_cfield("velocity", u)
+    o.compute_vorticity()
+    w = o.get_cfield("vorticity")
+    o.set_cfield("vorticity", w)
+    o.compute_velocity()
+    assert rel(o.get_cfield("velocity"), u) < 1e-14
