@@ -303,11 +303,39 @@ def ensemble_bench(B, ic, steps, warmup=2, seed=5):
             "speedup": ms_s / ms_b}
 
 
-def cpu_baseline(ic, wl, threads):
+def cpu_info():
+    """CPU model and logical-core counts of this host (nproc: the cores this
+    process may run on)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return {"model": model, "nproc": nproc, "cpu_count": os.cpu_count()}
+
+
+def cpu_threads_default():
+    """All the cores this process may use, within the OpenMP thread limit
+    the host sets (OMP_NUM_THREADS), which the CPU restatement obeys."""
+    nproc = cpu_info()["nproc"]
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(nproc, int(omp)) if omp.isdigit() and int(omp) > 0 else nproc
+
+
+def cpu_baseline(ic, wl, threads, single_n=256):
     """The oracle (CPU restatement, C++/OpenMP) on `threads` host threads, on a
-    bounded sample of the same workload: one FP32 vorticity step and one
-    velocity transform at full size, and the tracer loop on 2^18 of the
-    tracers (scaled to the full count)."""
+    sample of the same workload: after one untimed warm-up step (first touch
+    of the buffers, twiddle tables), one NSVEparticles step at full size --
+    the velocity transform, the tracer loop over all tracers, the FP32
+    vorticity step.  Then one FP32 vorticity step of a `single_n`^3 box on a
+    single thread, for per-core throughput."""
     from oracle import oracle as O
     n = wl["n"]
     O.build()
@@ -317,28 +345,50 @@ def cpu_baseline(ic, wl, threads):
     s.set_cfield("vorticity", ic.solenoidal_vorticity(n, seed=1, k0=wl["k0"]))
     s.io_projection()
     dt = 0.5 / n
+    p = None
+    if wl["np"]:
+        p = O.Particles(s, wl["np"], wl["nb"], wl["m"], 4, step_idx=1)
+        p.set_state(ic.uniform_positions(wl["np"], seed=2))
+    t0 = time.perf_counter()
+    O.nsve_particles_step(s, p, dt) if p is not None else s.step(dt)
+    t_warm = time.perf_counter() - t0
+    progress("CPU baseline: warm-up step %.1f s" % t_warm)
     t0 = time.perf_counter()
     s.velocity_rspace()
     t_vel = time.perf_counter() - t0
-    nsamp = min(wl["np"], 2 ** 18)
     t_tr = 0.0
-    if nsamp:
-        p = O.Particles(s, nsamp, wl["nb"], wl["m"], 4, step_idx=1)
-        p.set_state(ic.uniform_positions(nsamp, seed=2))
+    if p is not None:
         t0 = time.perf_counter()
         p.complete_loop(dt)
-        t_tr = (time.perf_counter() - t0) * wl["np"] / nsamp
-        del p
+        t_tr = time.perf_counter() - t0
     t0 = time.perf_counter()
     s.step(dt)
     t_step = time.perf_counter() - t0
-    del s
+    del p, s
     total = t_vel + t_tr + t_step
+    progress("CPU baseline: timed step %.1f s on %d threads" % (total, threads))
+    # one core
+    O.set_threads(1)
+    s1 = O.Solver(single_n, precision="single")
+    s1.set_physics(wl["nu"], wl["forcing"], wl["famp"], wl["fk0"], wl["fk1"])
+    s1.set_cfield("vorticity", ic.solenoidal_vorticity(single_n, seed=1, k0=wl["k0"]))
+    s1.io_projection()
+    s1.step(0.5 / single_n)
+    t0 = time.perf_counter()
+    s1.step(0.5 / single_n)
+    t_one = time.perf_counter() - t0
+    del s1
+    O.set_threads(threads)
     return {"value": n ** 3 / total, "unit": "grid-pt*steps/s", "cores": threads, "kind": "port",
-            "sample": ("one NSVEparticles step of the %d^3 FP32 workload by the CPU oracle: vorticity step %.1f s, "
-                       "velocity transform %.1f s, tracer loop on %d of %d tracers %.1f s scaled to %.1f s"
-                       % (n, t_step, t_vel, nsamp, wl["np"], t_tr * nsamp / max(wl["np"], 1), t_tr)),
-            "particle_interps_per_s": (wl["np"] / t_tr) if t_tr else None}
+            "sample": ("one NSVEparticles step of the %d^3 FP32 workload with all %d tracers by the CPU oracle "
+                       "(after one untimed warm-up step): velocity transform %.1f s, tracer loop %.1f s, "
+                       "vorticity step %.1f s" % (n, wl["np"], t_vel, t_tr, t_step)),
+            "cpu": cpu_info(),
+            "particle_interps_per_s": (wl["np"] / t_tr) if t_tr else None,
+            "vorticity_step_s": t_step,
+            "single_core": {"n": single_n, "vorticity_step_s": t_one,
+                            "grid_pt_steps_per_s_per_core": single_n ** 3 / t_one,
+                            "note": "FP32 vorticity step (no tracers) of a %d^3 box on one thread" % single_n}}
 
 
 def copy_calibration(nbytes):
@@ -401,8 +451,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "precision_note": "f32 field storage, FFTs and cross product; f64 factors, reductions, weights, "
-                          "tracer positions and z-sums",
+        "precision_note": "f32 field storage, FFTs and cross product; f64 integrating factors, forcing "
+                          "amplitudes, reductions, interpolation weights and stencil sums (every f32 field value "
+                          "is converted to f64 before it is accumulated), tracer positions and history",
         "data": "synthetic: seeded solenoidal field E(k)~k^4 exp(-2(k/%g)^2), seeded uniform tracers" % wl["k0"],
         "config": {"workload": wl["name"], "n": n, "nu": wl["nu"], "dt": 0.5 / n, "forcing": wl["forcing"],
                    "tracers": np_, "neighbours": wl["nb"], "smoothness": wl["m"]},
@@ -441,7 +492,7 @@ def main():
         out["ensemble"] = ensemble_bench(B, ic, args.ensemble_steps)
         progress("ensemble leg done")
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or cpu_threads_default()
         out["cpu_baseline"] = cpu_baseline(ic, wl, threads)
         progress("CPU baseline done")
     else:

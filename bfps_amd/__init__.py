@@ -58,12 +58,15 @@ class kspace:
 class vorticity_equation:
     """GPU-resident ``vorticity_equation<rnumber, HIP>``.
 
-    precision: 'single' (rnumber = float) or 'double'.
-    dealias:   'SMOOTH' (the reference solver's fixed choice) or 'TWO_THIRDS'.
+    precision:  'single' (rnumber = float) or 'double'.
+    dealias:    'SMOOTH' (the reference solver's fixed choice) or 'TWO_THIRDS'.
+    plan_rigor: 'FFTW_ESTIMATE', 'FFTW_MEASURE' (the reference constructor's
+                default) or 'FFTW_PATIENT': how the strided FFT passes pick
+                their tile shapes (``set_plan_rigor``).
     """
 
     def __init__(self, name, nx, ny=None, nz=None, dkx=1.0, dky=1.0, dkz=1.0,
-                 precision="single", dealias="SMOOTH", device=0, _handle=None):
+                 precision="single", dealias="SMOOTH", device=0, plan_rigor="FFTW_MEASURE", _handle=None):
         ny = nx if ny is None else ny
         nz = nx if nz is None else nz
         self.name = name
@@ -88,6 +91,8 @@ class vorticity_equation:
                 raise capi.BfpshError("bfpsh_create failed (%d): %s" % (rc, msg.decode() if msg else ""))
         self._h = h
         self.kk = kspace(h, nx, ny, nz, dkx, dky, dkz)
+        if self._owned:
+            self.set_plan_rigor(plan_rigor)
         # physical parameters, initialised as the reference constructor does
         # (bfps/cpp/vorticity_equation.cpp:149-155)
         self.nu = 0.1
@@ -118,6 +123,31 @@ class vorticity_equation:
             self.close()
         except Exception:
             pass
+
+    def set_plan_rigor(self, rigor):
+        """Choose the tile shape of each strided FFT pass (the reference's
+        plan rigor, bfps/cpp/field.cpp:105-118): 'FFTW_ESTIMATE' takes the
+        wisdom or the built-in shapes, 'FFTW_MEASURE' / 'FFTW_PATIENT' time
+        the compiled shapes unless the wisdom holds a choice of that rigor."""
+        if rigor not in capi.PLAN_RIGOR:
+            raise capi.BfpshError("unknown plan rigor %r" % (rigor,))
+        capi.check(self._h, capi.load().bfpsh_set_plan_rigor(self._h, capi.PLAN_RIGOR[rigor]), "set_plan_rigor")
+
+    def pass_shapes(self):
+        """{kind: (values per thread, columns, shape index, shapes compiled)}
+        of the strided passes (kinds z_inv, y_inv, z_fwd, y_fwd)."""
+        out = {}
+        for name, k in capi.PASS_KINDS.items():
+            v = [ctypes.c_int() for _ in range(4)]
+            capi.check(self._h, capi.load().bfpsh_get_pass_shape(self._h, k, *[ctypes.byref(x) for x in v]),
+                       "get_pass_shape")
+            out[name] = tuple(x.value for x in v)
+        return out
+
+    def set_pass_shape(self, kind, index):
+        """Force compiled shape `index` for pass `kind` (name or number)."""
+        k = capi.PASS_KINDS.get(kind, kind)
+        capi.check(self._h, capi.load().bfpsh_set_pass_shape(self._h, int(k), int(index)), "set_pass_shape")
 
     def _push_physics(self):
         rc = capi.load().bfpsh_set_physics(self._h, float(self.nu), capi.FORCING[self.forcing_type],
@@ -580,6 +610,25 @@ def NSVEparticles_step(fs, dt):
     """NSVEparticles::step: tracers advance with u(t), then the fluid step."""
     fs._push_physics()
     capi.check(fs.handle, capi.load().bfpsh_nsve_particles_step(fs.handle, float(dt)), "nsve_particles_step")
+
+
+def wisdom_import(path):
+    """Load pass wisdom from a text file (fftwf_import_wisdom_from_filename,
+    bfps/cpp/full_code/main_code.hpp:101-104); a missing file is no error."""
+    rc = capi.load().bfpsh_wisdom_import(os.fsencode(path))
+    if rc != 0:
+        raise capi.BfpshError("wisdom_import failed (%d): %s" % (rc, path))
+
+
+def wisdom_export(path):
+    """Save the pass wisdom (fftwf_export_wisdom_to_filename, main_code.hpp:145-149)."""
+    rc = capi.load().bfpsh_wisdom_export(os.fsencode(path))
+    if rc != 0:
+        raise capi.BfpshError("wisdom_export failed (%d): %s" % (rc, path))
+
+
+def wisdom_forget():
+    capi.load().bfpsh_wisdom_forget()
 
 
 class ensemble:

@@ -40,14 +40,36 @@ def _compile(args):
     return obj, r.returncode, errs, time.time() - t0
 
 
+def _deps(src, seen=None):
+    """The file and the quoted includes it reaches (transitively)."""
+    seen = set() if seen is None else seen
+    src = os.path.normpath(src)
+    if src in seen or not os.path.exists(src):
+        return seen
+    seen.add(src)
+    for line in open(src):
+        t = line.strip()
+        if t.startswith("#include") and '"' in t:
+            _deps(os.path.join(os.path.dirname(src), t.split('"')[1]), seen)
+    return seen
+
+
+def _stale(src, obj):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in _deps(src))
+
+
 def sources():
     out = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC))
            if f.endswith((".hip", ".hpp", ".h"))]
     return out
 
 
-def build(jobs=None, sizes=SIZES, verbose=True, out=None, defines=()):
-    """Compile and link; `out` / `defines` build an experimental variant."""
+def build(jobs=None, sizes=SIZES, verbose=True, out=None, defines=(), incremental=False):
+    """Compile and link; `out` / `defines` build an experimental variant;
+    `incremental` recompiles only objects older than their sources."""
     lib = out or LIB
     bdir = BUILD if not defines else BUILD + "_" + "_".join(d.replace("=", "") for d in defines)
     os.makedirs(bdir, exist_ok=True)
@@ -62,8 +84,9 @@ def build(jobs=None, sizes=SIZES, verbose=True, out=None, defines=()):
     jobs = jobs or min(8, os.cpu_count() or 4)
     t0 = time.time()
     failed = []
+    todo = [t for t in tasks if not incremental or _stale(t[0], t[1])]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        for obj, rc, errs, dt in ex.map(_compile, tasks):
+        for obj, rc, errs, dt in ex.map(_compile, todo):
             if verbose:
                 print("  %-28s %5.1fs%s" % (os.path.basename(obj), dt, "" if rc == 0 else "  FAILED"))
             if rc != 0:
@@ -118,8 +141,9 @@ if __name__ == "__main__":
     ap.add_argument("--sizes", default=None)
     ap.add_argument("--out", default=None, help="library path (experimental variants)")
     ap.add_argument("-D", dest="defines", action="append", default=[], help="NAME=VALUE")
+    ap.add_argument("-i", "--incremental", action="store_true", help="recompile only stale objects")
     a = ap.parse_args()
     sz = tuple(int(s) for s in a.sizes.split(",")) if a.sizes else SIZES
-    lib = build(a.jobs, sz, out=a.out, defines=tuple(a.defines))
+    lib = build(a.jobs, sz, out=a.out, defines=tuple(a.defines), incremental=a.incremental)
     if not a.out:
         build_driver(lib)

@@ -14,6 +14,11 @@ FP32, FP64 = 0, 1
 TWO_THIRDS, SMOOTH = 0, 1
 FORCING = {"none": 0, "linear": 1, "Kolmogorov": 2, "fixed_energy_injection_rate": 3}
 CVORTICITY, CVELOCITY, RVELOCITY, RGRADIENT = 0, 1, 5, 6
+# plan rigor (the reference's FFTW planner flags) and strided pass kinds
+ESTIMATE, MEASURE, PATIENT = 0, 1, 2
+PLAN_RIGOR = {"FFTW_ESTIMATE": ESTIMATE, "FFTW_MEASURE": MEASURE, "FFTW_PATIENT": PATIENT,
+              ESTIMATE: ESTIMATE, MEASURE: MEASURE, PATIENT: PATIENT}
+PASS_KINDS = {"z_inv": 0, "y_inv": 1, "z_fwd": 2, "y_fwd": 3}
 
 # every exported entry point: name -> (restype, argtypes)
 _P = ctypes.c_void_p
@@ -49,6 +54,12 @@ SIGNATURES = {
     "bfpsh_field_get": (_I, [_P, _I, _I, _P]),
     "bfpsh_field_copy_from": (_I, [_P, _I, _I]),
     "bfpsh_field_cospectrum": (_I, [_P, _I, _I, _DP]),
+    "bfpsh_set_plan_rigor": (_I, [_P, _I]),
+    "bfpsh_get_pass_shape": (_I, [_P, _I, _IP, _IP, _IP, _IP]),
+    "bfpsh_set_pass_shape": (_I, [_P, _I, _I]),
+    "bfpsh_wisdom_import": (_I, [ctypes.c_char_p]),
+    "bfpsh_wisdom_export": (_I, [ctypes.c_char_p]),
+    "bfpsh_wisdom_forget": (_I, []),
     "bfpsh_field_normalize": (_I, [_P, _I]),
     "bfpsh_field_rspace_stats": (_I, [_P, _I, _I, _DP, _DP, _LP]),
     "bfpsh_compute_pressure": (_I, [_P, _I]),

@@ -1,4 +1,6 @@
 // C ABI implementation of the MI355X backend (see include/bfpsh.h).
+#include <mutex>
+
 #include "ctx.hpp"
 #include "interp_tables.h"
 
@@ -93,6 +95,7 @@ static void planar_to_host_real(const T* p, T* h, int n, int nc) {
     } while (0)
 
 #define PREC_CALL(fn, ...) (ops_for(ctx)->fn(__VA_ARGS__))
+
 
 extern "C" {
 
@@ -512,6 +515,7 @@ int bfpsh_field_cospectrum(bfpsh_ctx* ctx, int id_a, int id_b, double* out) {
     return PREC_CALL(cospectrum, ctx, ctx->fields[id_a], ctx->fields[id_b], out);
 }
 
+
 int bfpsh_field_normalize(bfpsh_ctx* ctx, int id) {
     CTX_CHECK();
     if (!field_ok(ctx, id)) return fail(ctx, BFPSH_EINVAL, "bad field id");
@@ -769,6 +773,128 @@ int bfpsh_time_steps(bfpsh_ctx* ctx, int nsteps, double dt, int with_particles, 
 
 }  // extern "C"
 
+// ------------------------------------------------------------------ pass wisdom
+namespace {
+std::mutex wisdom_mu;
+std::map<std::string, PassWisdom> wisdom;
+std::map<int, std::string> arch_names;  // device -> architecture (gcnArchName up to ':')
+const char* const kPassKindName[4] = {"z_inv", "y_inv", "z_fwd", "y_fwd"};
+}  // namespace
+
+std::string pass_wisdom_key(const bfpsh_ctx* ctx, int kind) {
+    std::string arch;
+    {
+        std::lock_guard<std::mutex> lk(wisdom_mu);
+        auto it = arch_names.find(ctx->device);
+        if (it != arch_names.end()) arch = it->second;
+    }
+    if (arch.empty()) {
+        hipDeviceProp_t prop;
+        arch = "unknown";
+        if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) {
+            arch = prop.gcnArchName;
+            arch = arch.substr(0, arch.find(':'));
+        }
+        std::lock_guard<std::mutex> lk(wisdom_mu);
+        arch_names[ctx->device] = arch;
+    }
+    return arch + "/" + (ctx->prec == BFPSH_FP32 ? "f32" : "f64") + "/" + std::to_string(ctx->n) + "/" +
+           kPassKindName[kind & 3];
+}
+
+bool pass_wisdom_lookup(const std::string& key, PassWisdom* w) {
+    std::lock_guard<std::mutex> lk(wisdom_mu);
+    auto it = wisdom.find(key);
+    if (it == wisdom.end()) return false;
+    *w = it->second;
+    return true;
+}
+
+void pass_wisdom_store(const std::string& key, const PassWisdom& w) {
+    std::lock_guard<std::mutex> lk(wisdom_mu);
+    wisdom[key] = w;
+}
+
+extern "C" {
+
+int bfpsh_set_plan_rigor(bfpsh_ctx* ctx, int rigor) {
+    CTX_CHECK();
+    if (rigor < BFPSH_ESTIMATE || rigor > BFPSH_PATIENT) return fail(ctx, BFPSH_EINVAL, "bad plan rigor");
+    return PREC_CALL(tune_passes, ctx, rigor);
+}
+
+int bfpsh_get_pass_shape(bfpsh_ctx* ctx, int kind, int* e, int* b, int* index, int* nshapes) {
+    CTX_CHECK();
+    if (kind < 0 || kind > 3) return fail(ctx, BFPSH_EINVAL, "bad pass kind");
+    const PassShapes sh = ops_for(ctx)->pass_shapes();
+    const int v = ctx->pass_var[kind] >= 0 && ctx->pass_var[kind] < sh.n ? ctx->pass_var[kind] : 0;
+    if (e) *e = sh.v[v].e;
+    if (b) *b = sh.v[v].b;
+    if (index) *index = v;
+    if (nshapes) *nshapes = sh.n;
+    return BFPSH_OK;
+}
+
+int bfpsh_set_pass_shape(bfpsh_ctx* ctx, int kind, int index) {
+    CTX_CHECK();
+    if (kind < 0 || kind > 3) return fail(ctx, BFPSH_EINVAL, "bad pass kind");
+    const PassShapes sh = ops_for(ctx)->pass_shapes();
+    if (index < 0 || index >= sh.n) return fail(ctx, BFPSH_EINVAL, "pass shape index out of range");
+    ctx->pass_var[kind] = index;
+    return BFPSH_OK;
+}
+
+int bfpsh_wisdom_import(const char* path) {
+    if (!path) return BFPSH_EINVAL;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return BFPSH_OK;  // no wisdom yet
+    std::map<std::string, PassWisdom> got;
+    char line[512];
+    int rc = BFPSH_OK;
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        char key[256];
+        PassWisdom w;
+        if (std::sscanf(line, "%255s %d %d %lf %d", key, &w.e, &w.b, &w.ms, &w.rigor) != 5 || w.e <= 0 || w.b <= 0 ||
+            w.rigor < BFPSH_ESTIMATE || w.rigor > BFPSH_PATIENT) {
+            rc = BFPSH_EINVAL;
+            break;
+        }
+        got[key] = w;
+    }
+    std::fclose(f);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(wisdom_mu);
+    for (auto& kv : got) wisdom[kv.first] = kv.second;
+    return BFPSH_OK;
+}
+
+int bfpsh_wisdom_export(const char* path) {
+    if (!path) return BFPSH_EINVAL;
+    std::map<std::string, PassWisdom> snap;
+    {
+        std::lock_guard<std::mutex> lk(wisdom_mu);
+        snap = wisdom;
+    }
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "w");
+    if (!f) return BFPSH_EINVAL;
+    std::fprintf(f, "# bfps_amd pass wisdom: <arch>/<precision>/<N>/<pass kind> <values per thread> <columns> "
+                    "<ms per plane> <rigor>\n");
+    for (auto& kv : snap)
+        std::fprintf(f, "%s %d %d %.6f %d\n", kv.first.c_str(), kv.second.e, kv.second.b, kv.second.ms, kv.second.rigor);
+    if (std::fclose(f) != 0 || std::rename(tmp.c_str(), path) != 0) return BFPSH_EINVAL;
+    return BFPSH_OK;
+}
+
+int bfpsh_wisdom_forget(void) {
+    std::lock_guard<std::mutex> lk(wisdom_mu);
+    wisdom.clear();
+    return BFPSH_OK;
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------------ ensembles
 // Independent boxes advanced together: boxes of one size form a class whose
 // 18-plane buffers (v0 | v1 | v2 | work | ru) sit in one slab, so each field
@@ -794,6 +920,7 @@ struct bfpsh_ensemble {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool captured_ru = false;          // the graph writes the real velocity
+    std::vector<char> captured_fixed;  // per class: the graph holds the band-amplitude launches
     static constexpr int NPB = 4;      // neighbour counts batched in one step
     PartBatch pb[NPB];                 // tracer-loop tables, one per neighbour count
 };
@@ -847,6 +974,22 @@ static int ensemble_params(bfpsh_ensemble* e, const double* dt, bool* changed) {
     return BFPSH_OK;
 }
 
+// some box of the class uses the fixed energy-injection forcing, whose band
+// amplitudes the graph must then compute each stage
+static bool class_any_fixed(const bfpsh_ensemble::Cls& c) {
+    for (const StepParams& p : c.sp)
+        if (p.forcing == BFPSH_FORCING_FIXED_ENERGY) return true;
+    return false;
+}
+
+// the captured graph no longer matches the boxes' forcing choices
+static bool fixed_changed(const bfpsh_ensemble* e) {
+    if (e->captured_fixed.size() != e->cls.size()) return true;
+    for (size_t ci = 0; ci < e->cls.size(); ci++)
+        if (bool(e->captured_fixed[ci]) != class_any_fixed(e->cls[ci])) return true;
+    return false;
+}
+
 static int ensemble_capture(bfpsh_ensemble* e, bool with_ru) {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
@@ -860,14 +1003,16 @@ static int ensemble_capture(bfpsh_ensemble* e, bool with_ru) {
     }
     EHIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = BFPSH_OK;
-    for (auto& c : e->cls) {
+    e->captured_fixed.assign(e->cls.size(), 0);
+    for (size_t ci = 0; ci < e->cls.size(); ci++) {
+        auto& c = e->cls[ci];
         bfpsh_ctx* b0 = e->boxes[c.members[0]];
         BatchDesc d;
         d.nbox = int(c.members.size());
         d.bstride = c.bstride;
         d.spb = c.d_sp;
-        d.any_fixed = false;
-        for (const StepParams& p : c.sp) d.any_fixed |= p.forcing == BFPSH_FORCING_FIXED_ENERGY;
+        d.any_fixed = class_any_fixed(c);
+        e->captured_fixed[ci] = d.any_fixed;
         d.v0 = b0->v0;
         d.v1 = b0->v1;
         d.v2 = b0->v2;
@@ -977,7 +1122,9 @@ int bfpsh_ensemble_step(bfpsh_ensemble* e, const double* dt, int with_particles)
     if (r) return r;
     bool any_part = false;
     for (bfpsh_ctx* c : e->boxes) any_part |= with_particles && c->nsteps > 0 && c->np > 0;
-    if (!e->exec || e->captured_ru != any_part) {
+    // parameter values reach the graph through the device table; a box
+    // switching to or from the fixed-injection forcing changes its launches
+    if (!e->exec || e->captured_ru != any_part || (changed && fixed_changed(e))) {
         r = ensemble_capture(e, any_part);
         if (r) return r;
     }

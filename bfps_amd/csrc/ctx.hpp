@@ -71,6 +71,10 @@ struct bfpsh_ctx {
     int64_t nbricks = 0;
     int sort_every = 1;            // re-sort period in tracer steps
     int sort_age = 1 << 30;        // steps since the last sort
+    // strided-pass tile shapes: index into pass_shapes<T, N>() per pass kind
+    // (BFPSH_PASS_*), chosen by the pass autotuner (bfpsh_set_plan_rigor)
+    int pass_var[4] = {0, 0, 0, 0};
+    int plan_rigor = BFPSH_ESTIMATE;
     // profiling
     bool prof = false;
     std::vector<std::string> prof_names;
@@ -218,7 +222,22 @@ struct OpsTable {
     int (*normalize)(bfpsh_ctx*, void*, int);
     int (*rspace_stats)(bfpsh_ctx*, const void*, int, int, const double*, double*, int64_t*);
     int (*compute_vorticity)(bfpsh_ctx*);
+    int (*tune_passes)(bfpsh_ctx*, int);
+    PassShapes (*pass_shapes)();
 };
+
+// Process-wide pass wisdom (bfpsh.hip): the tile shape the autotuner chose
+// for one (GPU architecture, precision, N, pass kind), with its time per
+// plane and the rigor of the search that chose it.  Keys are
+// "arch/f32|f64/N/kind".
+struct PassWisdom {
+    int e = 0, b = 0;
+    double ms = 0.0;
+    int rigor = BFPSH_MEASURE;  // the plan rigor that chose it
+};
+std::string pass_wisdom_key(const bfpsh_ctx* ctx, int kind);
+bool pass_wisdom_lookup(const std::string& key, PassWisdom* w);
+void pass_wisdom_store(const std::string& key, const PassWisdom& w);
 
 int create_ctx(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, double dkz, int precision,
                int dealias, int device, hipStream_t stream, void* slab);

@@ -1,6 +1,9 @@
 // Launch sequences of the vorticity step, transforms and tracer loop for one
 // (precision, N) pair.  Instantiated once per pair by impl_inst.hip.
 #pragma once
+#include <array>
+#include <utility>
+
 #include "ctx.hpp"
 #include "stats.hpp"
 
@@ -25,6 +28,119 @@ struct Impl {
         StepParams sp;
         Geom g;
     };
+    // ---------------------------------------------------------- strided passes
+    // compiled tile shapes of k_pass for this (T, N); ctx->pass_var[kind]
+    // selects one per pass kind
+    static constexpr PassShapes SH = pass_shapes<T, N>();
+    typedef void (*PassFn)(hipStream_t, CT*, size_t, size_t, const CT*, int, size_t, int);
+    template <int SIGN, int I>
+    static void pass_launch(hipStream_t st, CT* d, size_t es, size_t os, const CT* tw, int planes, size_t bs,
+                            int nbox) {
+        constexpr PassShape p = SH.v[I < SH.n ? I : 0];
+        k_pass<T, N, SIGN, p.e, p.b><<<dim3(NXH / p.b, N, planes * nbox), (N / p.e) * p.b, 0, st>>>(
+            d, es, os, tw, planes, bs);
+    }
+    template <int SIGN, size_t... I>
+    static constexpr std::array<PassFn, kMaxPassShapes> pass_table(std::index_sequence<I...>) {
+        return {{&pass_launch<SIGN, int(I)>...}};
+    }
+    static PassFn pass_fn(int kind, int v) {
+        static constexpr std::array<PassFn, kMaxPassShapes> inv = pass_table<+1>(std::make_index_sequence<kMaxPassShapes>{});
+        static constexpr std::array<PassFn, kMaxPassShapes> fwd = pass_table<-1>(std::make_index_sequence<kMaxPassShapes>{});
+        if (v < 0 || v >= SH.n) v = 0;
+        return (kind == BFPSH_PASS_Z_INV || kind == BFPSH_PASS_Y_INV) ? inv[v] : fwd[v];
+    }
+    // element and tile strides of a pass kind: y passes walk ky rows, z
+    // passes the kz rows of one ky slab
+    static size_t pass_estride(int kind) {
+        return (kind == BFPSH_PASS_Y_INV || kind == BFPSH_PASS_Y_FWD) ? row_stride(N) : size_t(NXH);
+    }
+    static size_t pass_ostride(int kind) {
+        return (kind == BFPSH_PASS_Y_INV || kind == BFPSH_PASS_Y_FWD) ? size_t(NXH) : row_stride(N);
+    }
+    // one in-place strided pass of `planes` planes of each of nbox boxes
+    // (bstride complex elements apart)
+    static int pass(bfpsh_ctx* ctx, int kind, CT* d, int planes, const char* name, int nbox = 1,
+                    size_t bstride = 0) {
+        const PassFn f = pass_fn(kind, ctx->pass_var[kind]);
+        LAUNCH(name, f(ctx->stream, d, pass_estride(kind), pass_ostride(kind), (const CT*)ctx->tw, planes, bstride,
+                       nbox));
+        return BFPSH_OK;
+    }
+    static PassShapes shapes() { return SH; }
+
+    // The pass autotuner: for each pass kind, the shape held by the wisdom
+    // when a search of at least this rigor chose it (any wisdom for
+    // ESTIMATE), else the fastest of all compiled shapes timed on one
+    // scratch plane (ESTIMATE without wisdom: the built-in shape), as FFTW's
+    // planner searches its algorithms under the reference's plan rigor
+    // (bfps/cpp/fftw_interface.hpp:30-35) and reuses its wisdom.
+    static int tune_passes(bfpsh_ctx* ctx, int rigor) {
+        ctx->plan_rigor = rigor;
+        void* scratch = nullptr;
+        hipEvent_t a = nullptr, b = nullptr;
+        int rc = BFPSH_OK;
+        for (int kind = 0; kind < 4 && rc == BFPSH_OK; kind++) {
+            ctx->pass_var[kind] = 0;
+            if (SH.n == 1) continue;
+            const std::string key = pass_wisdom_key(ctx, kind);
+            PassWisdom w;
+            if (pass_wisdom_lookup(key, &w) && w.rigor >= rigor) {
+                for (int v = 0; v < SH.n; v++)
+                    if (SH.v[v].e == w.e && SH.v[v].b == w.b) ctx->pass_var[kind] = v;
+                continue;
+            }
+            if (rigor == BFPSH_ESTIMATE) continue;
+            if (!scratch) {
+                const size_t bytes = ctx->ps * sizeof(CT);
+                if (hipMalloc(&scratch, bytes) != hipSuccess || hipMemsetAsync(scratch, 0, bytes, ctx->stream) != hipSuccess ||
+                    hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+                    rc = fail(ctx, BFPSH_EHIP, "pass autotuner: scratch plane or events");
+                    break;
+                }
+            }
+            const int reps = rigor >= BFPSH_PATIENT ? 10 : 3;
+            double best = 1e300;
+            int bestv = 0;
+            for (int v = 0; v < SH.n && rc == BFPSH_OK; v++) {
+                const PassFn f = pass_fn(kind, v);
+                CT* d = static_cast<CT*>(scratch);
+                const CT* tw = (const CT*)ctx->tw;
+                f(ctx->stream, d, pass_estride(kind), pass_ostride(kind), tw, 1, 0, 1);  // warm-up
+                (void)hipEventRecord(a, ctx->stream);
+                for (int r = 0; r < reps; r++) f(ctx->stream, d, pass_estride(kind), pass_ostride(kind), tw, 1, 0, 1);
+                (void)hipEventRecord(b, ctx->stream);
+                hipError_t e = hipGetLastError();
+                if (e == hipSuccess) e = hipEventSynchronize(b);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+                if (e != hipSuccess) {
+                    rc = fail(ctx, BFPSH_EHIP, std::string("pass autotuner: ") + hipGetErrorString(e));
+                    break;
+                }
+                if (ms / reps < best) {
+                    best = ms / reps;
+                    bestv = v;
+                }
+            }
+            if (rc == BFPSH_OK) {
+                ctx->pass_var[kind] = bestv;
+                w.e = SH.v[bestv].e;
+                w.b = SH.v[bestv].b;
+                w.ms = best;
+                w.rigor = rigor;
+                pass_wisdom_store(key, w);
+            }
+        }
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        if (scratch) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(scratch);
+        }
+        return rc;
+    }
+
     static Batch single(bfpsh_ctx* ctx, double dt) {
         Batch b;
         b.v0 = (CT*)ctx->v0;
@@ -78,16 +194,20 @@ struct Impl {
             int r = bs_pass<0>(ctx, src, w, tw, g, "inv_z_bs", nb, bs);
             if (r) return r;
         }
-        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 6 * nb), NT, 0, ctx->stream>>>(
-                             w, row_stride(N), size_t(NXH), tw, 6, bs)));
+        {
+            int r = pass(ctx, BFPSH_PASS_Y_INV, w, 6, "inv_y", nb, bs);
+            if (r) return r;
+        }
         if constexpr (RowsW<T, N>::ok && !BFPSH_ROWS_TILE)
             LAUNCH("rows_cross", (k_rows_cross_w<T, N><<<dim3(N * N / 2 / RowsW<T, N>::RPB, nb), RowsW<T, N>::NT, 0,
                                                          ctx->stream>>>(w, ru, tw, T(double(N) * N * N), bs)));
         else
             LAUNCH("rows_cross", (k_rows_cross<T, N><<<dim3(N * N / 2 / CF::RP, nb), CF::NT_ROW, 0, ctx->stream>>>(
                                       w, ru, tw, T(double(N) * N * N), bs)));
-        LAUNCH("fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, 3 * nb), NT, 0, ctx->stream>>>(
-                             w, size_t(NXH), row_stride(N), tw, 3, bs)));
+        {
+            int r = pass(ctx, BFPSH_PASS_Z_FWD, w, 3, "fwd_z", nb, bs);
+            if (r) return r;
+        }
         const dim3 tiles3(NXH / CF::B_Y3, N, nb);
         const char* nm = s == 1 ? "fwd_y_update1" : (s == 2 ? "fwd_y_update2" : "fwd_y_update3");
         const CT* vp = s == 1 ? v1 : (s == 2 ? v1 : v2);
@@ -185,7 +305,8 @@ struct Impl {
         const CT* tw = (const CT*)ctx->tw;
         r = bs_pass<1>(ctx, (const CT*)ctx->v0, w, tw, geom(ctx), "inv_z_bs_u");
         if (r) return r;
-        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 3), NT, 0, ctx->stream>>>(w, row_stride(N), size_t(NXH), tw, 3, 0)));
+        r = pass(ctx, BFPSH_PASS_Y_INV, w, 3, "inv_y");
+        if (r) return r;
         LAUNCH("rows_c2r3", (k_rows_c2r<T, N, 3><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->ru, tw)));
         ctx->ru_valid = true;
         return BFPSH_OK;
@@ -200,7 +321,8 @@ struct Impl {
         const CT* tw = (const CT*)ctx->tw;
         r = bs_pass<2>(ctx, (const CT*)ctx->v0, w, tw, geom(ctx), "inv_z_bs_grad");
         if (r) return r;
-        LAUNCH("inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, 9), NT, 0, ctx->stream>>>(w, row_stride(N), size_t(NXH), tw, 9, 0)));
+        r = pass(ctx, BFPSH_PASS_Y_INV, w, 9, "inv_y");
+        if (r) return r;
         LAUNCH("rows_c2r9", (k_rows_c2r<T, N, 9><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(w, (T*)ctx->rgrad, tw)));
         return BFPSH_OK;
     }
@@ -211,8 +333,10 @@ struct Impl {
     static int field_ift_nc(bfpsh_ctx* ctx, void* data) {
         CT* d = (CT*)data;
         const CT* tw = (const CT*)ctx->tw;
-        LAUNCH("field_inv_z", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), row_stride(N), tw, NC, 0)));
-        LAUNCH("field_inv_y", (k_pass<T, N, +1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, row_stride(N), size_t(NXH), tw, NC, 0)));
+        int r = pass(ctx, BFPSH_PASS_Z_INV, d, NC, "field_inv_z");
+        if (r) return r;
+        r = pass(ctx, BFPSH_PASS_Y_INV, d, NC, "field_inv_y");
+        if (r) return r;
         LAUNCH("field_rows_c2r", (k_rows_c2r<T, N, NC><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>(d, (T*)d, tw)));
         return BFPSH_OK;
     }
@@ -221,8 +345,10 @@ struct Impl {
         CT* d = (CT*)data;
         const CT* tw = (const CT*)ctx->tw;
         LAUNCH("field_rows_r2c", (k_rows_r2c<T, N, NC><<<N * N / 2 / CF::RP, CF::NT_ROW, 0, ctx->stream>>>((const T*)d, d, tw)));
-        LAUNCH("field_fwd_z", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, size_t(NXH), row_stride(N), tw, NC, 0)));
-        LAUNCH("field_fwd_y", (k_pass<T, N, -1><<<dim3(NXH / B, N, NC), NT, 0, ctx->stream>>>(d, row_stride(N), size_t(NXH), tw, NC, 0)));
+        int r = pass(ctx, BFPSH_PASS_Z_FWD, d, NC, "field_fwd_z");
+        if (r) return r;
+        r = pass(ctx, BFPSH_PASS_Y_FWD, d, NC, "field_fwd_y");
+        if (r) return r;
         return BFPSH_OK;
     }
     static int field_ift(bfpsh_ctx* ctx, void* data, int nc) {

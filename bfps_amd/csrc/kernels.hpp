@@ -246,14 +246,17 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
 // element (n, b) of tile (blockIdx.x, blockIdx.y) of plane p of box q
 // (blockIdx.z = q * planes + p) at
 //   base + q * bstride + p * plane_size(N) + blockIdx.y * ostride + n * estride + kx0 + b
-template <typename T, int N, int SIGN>
-__global__ void __launch_bounds__((Cfg<T, N>::NT_PASS))
+// Tile shape: B columns, E complex values per thread (N / E threads per
+// column); the shapes a context may use are listed by pass_shapes() and
+// picked by the pass autotuner.
+template <typename T, int N, int SIGN, int E, int B>
+__global__ void __launch_bounds__((N / E) * B)
 k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __restrict__ tw, int planes,
        size_t bstride) {
-    using CF = Cfg<T, N>;
-    constexpr int B = CF::B, NT = CF::NT_PASS, E = CF::E_PASS, NXH = CF::NXH;
-    constexpr size_t PS = plane_size(N), YS = row_stride(N);
+    constexpr int NT = (N / E) * B;
     using F = RegFft<T, N, B, B, NT, SIGN>;
+    static_assert(F::E == E, "tile shape");
+    constexpr size_t PS = plane_size(N);
     __shared__ C2<T> lds[N * B];
     const int tid = threadIdx.x;
     const int b = tid % B, grp = tid / B;
@@ -266,6 +269,46 @@ k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __
     F::run(v, lds, tw, grp, b);
 #pragma unroll
     for (int e = 0; e < E; e++) base[F::out_row(grp, e) * estride] = v[e];
+}
+
+// Tile shapes (E values per thread, B columns) of the plain strided pass
+// for one (precision, N): the built-in choice (Cfg) first, then, for
+// N >= 256 (below that a box is cache-resident and launch-bound), every
+// shape with E in {8, 12, 16, 24, 32} dividing N with a radix plan, B
+// dividing N/2, 64..1024 threads, at least 32-byte row segments and at most
+// 128 KB of LDS -- up to kMaxPassShapes in all.
+struct PassShape {
+    int e, b;
+};
+constexpr int kMaxPassShapes = 8;
+struct PassShapes {
+    int n;
+    PassShape v[kMaxPassShapes];
+};
+template <typename T, int N>
+constexpr PassShapes pass_shapes() {
+    using CF = Cfg<T, N>;
+    PassShapes s{};
+    s.n = 0;
+    s.v[s.n++] = PassShape{CF::E_PASS, CF::B};
+    if (N < 256) return s;
+    constexpr int es[5] = {8, 12, 16, 24, 32};
+    constexpr int bs[6] = {2, 4, 8, 16, 32, 64};
+    for (int i = 0; i < 5; i++) {
+        const int e = es[i];
+        if (N % e != 0 || make_plan_e(N, e).ns <= 0) continue;
+        for (int j = 0; j < 6; j++) {
+            const int b = bs[j];
+            const long nt = long(N / e) * b;
+            if ((N / 2) % b != 0 || nt < 64 || nt > 1024) continue;
+            if (long(N) * b * long(sizeof(C2<T>)) > 131072 || b * int(sizeof(C2<T>)) < 32) continue;
+            bool dup = false;
+            for (int k = 0; k < s.n; k++) dup |= s.v[k].e == e && s.v[k].b == b;
+            if (dup || s.n == kMaxPassShapes) continue;
+            s.v[s.n++] = PassShape{e, b};
+        }
+    }
+    return s;
 }
 
 // ------------------------------------------------------------ x passes on row pairs

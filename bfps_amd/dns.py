@@ -23,7 +23,8 @@ import numpy as np
 
 from . import __version__
 from . import (NSVE_do_stats, NSVEparticles_do_stats, NSVEparticles_read_checkpoint, NSVEparticles_step,
-               NSVEparticles_write_checkpoint, h5io, particles_system, vorticity_equation)
+               NSVEparticles_write_checkpoint, h5io, particles_system, vorticity_equation, wisdom_export,
+               wisdom_import)
 
 
 def default_parameters(dns_type="NSVEparticles"):
@@ -325,12 +326,20 @@ class NSVEparticles:
     def _path(self, name):
         return os.path.join(self.work_dir, name)
 
+    def _wisdom_path(self):
+        return self._path(self.simname + "_hip_wisdom.txt")
+
     def initialize(self):
         p = self.p
         name = self._path(self.simname)
+        # the transform wisdom of earlier jobs of this run (main_code.hpp:101-104
+        # imports <simname>_fftw_wisdom.txt), then the drivers' default rigor
+        # (DEFAULT_FFTW_FLAG = FFTW_PATIENT, fftw_interface.hpp:30-35)
+        wisdom_import(self._wisdom_path())
         fs = vorticity_equation(name, p["nx"], p["ny"], p["nz"], p["dkx"], p["dky"], p["dkz"],
                                 precision=self.precision,
-                                dealias="SMOOTH" if p["dealias_type"] == 1 else "TWO_THIRDS")
+                                dealias="SMOOTH" if p["dealias_type"] == 1 else "TWO_THIRDS",
+                                plan_rigor="FFTW_PATIENT")
         fs.nu, fs.fmode, fs.famplitude = p["nu"], p["fmode"], p["famplitude"]
         fs.fk0, fs.fk1, fs.forcing_type = p["fk0"], p["fk1"], p["forcing_type"]
         fs.checkpoints_per_file = p["checkpoints_per_file"]
@@ -457,6 +466,7 @@ class NSVEparticles:
         if self.fs is not None:
             self.fs.close()
             self.fs = self.ps = None
+            wisdom_export(self._wisdom_path())  # main_code.hpp:145-149
 
 
 __all__ = ["default_parameters", "launch_parameters", "kspace_info", "write_par", "read_iteration",

@@ -49,6 +49,18 @@ enum field_components { ONE, THREE, THREExTHREE };
 
 constexpr int ncomp(field_components fc) { return fc == ONE ? 1 : (fc == THREE ? 3 : 9); }
 
+/* Planner flags of the constructors, as fftw3's FFTW_ESTIMATE /
+ * FFTW_MEASURE / FFTW_PATIENT, and the drivers' default of
+ * fftw_interface.hpp:30-35 (PATIENT unless USE_HIPESTIMATE is defined):
+ * they pick how the strided FFT passes choose their tile shapes
+ * (bfpsh_set_plan_rigor). */
+constexpr unsigned HIP_ESTIMATE = BFPSH_ESTIMATE, HIP_MEASURE = BFPSH_MEASURE, HIP_PATIENT = BFPSH_PATIENT;
+#ifdef USE_HIPESTIMATE
+#define DEFAULT_HIP_FLAG bfps_hip::HIP_ESTIMATE
+#else
+#define DEFAULT_HIP_FLAG bfps_hip::HIP_PATIENT
+#endif
+
 class error : public std::runtime_error {
    public:
     using std::runtime_error::runtime_error;
@@ -501,7 +513,8 @@ class vorticity_equation {
 
     /* (vorticity_equation.cpp:110-156); dealias and device extend it. */
     vorticity_equation(const char* NAME, int nx, int ny, int nz, double DKX = 1.0, double DKY = 1.0,
-                       double DKZ = 1.0, unsigned = 0, int dealias = BFPSH_SMOOTH, int device = 0)
+                       double DKZ = 1.0, unsigned FFTW_PLAN_RIGOR = HIP_MEASURE, int dealias = BFPSH_SMOOTH,
+                       int device = 0)
         : name(NAME) {
         std::strncpy(forcing_type, "linear", sizeof(forcing_type));
         box_.nx = nx;
@@ -511,6 +524,7 @@ class vorticity_equation {
         detail::check(nullptr,
                       bfpsh_create(&box_.ctx, nx, ny, nz, DKX, DKY, DKZ, box_.prec, dealias, device),
                       "vorticity_equation (unsupported size or no GPU)");
+        detail::check(box_.ctx, bfpsh_set_plan_rigor(box_.ctx, int(FFTW_PLAN_RIGOR)), "vorticity_equation plan rigor");
         detail::boxes().push_back(&box_);
         cvorticity = new field<rnumber, be, THREE>(&box_, detail::role::vorticity);
         cvelocity = new field<rnumber, be, THREE>(&box_, detail::role::velocity);

@@ -62,6 +62,20 @@ enum {
  * device: HIP device ordinal.  All fields start at zero. */
 int bfpsh_create(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, double dkz,
                  int precision, int dealias, int device);
+/* Plan rigor of the transforms: replaces the `plan_rigor` argument of the
+ * field / vorticity_equation constructors, which picks FFTW's plan search
+ * (FFTW_ESTIMATE / FFTW_MEASURE / FFTW_PATIENT; bfps/cpp/field.cpp:105-118,
+ * bfps/cpp/fftw_interface.hpp:30-35).  Here it picks how the tile shape of
+ * each strided FFT pass kind is chosen among the compiled shapes:
+ * ESTIMATE takes the shape the pass wisdom holds, else the built-in one;
+ * MEASURE and PATIENT take the wisdom's shape when a search of at least
+ * their rigor chose it, else time every compiled shape (3 launches each on
+ * one scratch plane for MEASURE, 10 for PATIENT) and keep the fastest in
+ * the wisdom.  Boxes below 256^3 have one shape. */
+enum { BFPSH_ESTIMATE = 0, BFPSH_MEASURE = 1, BFPSH_PATIENT = 2 };
+/* strided pass kinds: inverse and forward transforms along kz (z) and ky (y) */
+enum { BFPSH_PASS_Z_INV = 0, BFPSH_PASS_Y_INV = 1, BFPSH_PASS_Z_FWD = 2, BFPSH_PASS_Y_FWD = 3 };
+
 /* Replaces ~vorticity_equation (bfps/cpp/vorticity_equation.cpp:158-169). */
 int bfpsh_destroy(bfpsh_ctx* ctx);
 const char* bfpsh_last_error(const bfpsh_ctx* ctx);
@@ -237,6 +251,29 @@ int bfpsh_profile_enable(bfpsh_ctx* ctx, int enable);
 int bfpsh_profile_get(bfpsh_ctx* ctx, int i, char* name, int namelen, double* total_ms, int* count);
 /* Bytes of device memory held by the context. */
 int bfpsh_device_bytes(bfpsh_ctx* ctx, size_t* bytes);
+
+/* Replaces the plan creation with `plan_rigor` in the field constructor
+ * (bfps/cpp/field.cpp:105-118): chooses the tile shape of each strided pass
+ * kind of this context by `rigor` (BFPSH_ESTIMATE / MEASURE / PATIENT, see
+ * above).  Measuring allocates one scratch plane for the time it runs;
+ * the context's fields are left untouched. */
+int bfpsh_set_plan_rigor(bfpsh_ctx* ctx, int rigor);
+/* The tile shape in use for pass `kind` (BFPSH_PASS_*): e complex values per
+ * thread, b columns per tile; nshapes = shapes compiled for this box size.
+ * No reference counterpart (FFTW plans are opaque). */
+int bfpsh_get_pass_shape(bfpsh_ctx* ctx, int kind, int* e, int* b, int* index, int* nshapes);
+/* Forces shape `index` (0 <= index < nshapes; 0 is the built-in one) for
+ * pass `kind`; for tests of every shape.  No reference counterpart. */
+int bfpsh_set_pass_shape(bfpsh_ctx* ctx, int kind, int index);
+/* Replace fftwf_import_wisdom_from_filename / fftwf_export_wisdom_to_filename
+ * (bfps/cpp/full_code/main_code.hpp:101-104,145-149) and fftw_forget_wisdom:
+ * the process-wide pass wisdom as a text file, one line per (architecture,
+ * precision, N, pass kind): "<arch>/<f32|f64>/<N>/<kind> <e> <b> <ms> <rigor>".
+ * Importing a missing file is not an error (as the reference's start of a
+ * new simulation); lines that do not parse are rejected. */
+int bfpsh_wisdom_import(const char* path);
+int bfpsh_wisdom_export(const char* path);
+int bfpsh_wisdom_forget(void);
 
 #ifdef __cplusplus
 }

@@ -135,7 +135,7 @@ class NSVE : public direct_numerical_simulation {
             std::cerr << data_file_problem << " problems growing file datasets.\ntrying to exit now." << std::endl;
             return EXIT_FAILURE;
         }
-        this->fs = new vorticity_equation<rnumber, HIP>(simname.c_str(), nx, ny, nz, dkx, dky, dkz, 0);
+        this->fs = new vorticity_equation<rnumber, HIP>(simname.c_str(), nx, ny, nz, dkx, dky, dkz, DEFAULT_HIP_FLAG);
         this->tmp_vec_field = new field<rnumber, HIP, THREE>(nx, ny, nz, this->comm, 0);
 
         this->fs->checkpoints_per_file = checkpoints_per_file;
@@ -285,10 +285,15 @@ class NSVEparticles : public NSVE<rnumber> {
 // ---------------------------------------------------------------- main (main_code.hpp:121-132)
 template <class DNS>
 static int run(const std::string& simname) {
+    // pass wisdom around the run, as main_code.hpp:101-104,145-149 does with
+    // the FFTW wisdom
+    const std::string wisdom = simname + "_hip_wisdom.txt";
+    if (bfpsh_wisdom_import(wisdom.c_str()) != BFPSH_OK) std::cerr << "ignoring unreadable " << wisdom << "\n";
     DNS c(MPI_COMM_WORLD, simname);
     if (c.initialize() != EXIT_SUCCESS) return EXIT_FAILURE;
     c.main_loop();
     c.finalize();
+    if (bfpsh_wisdom_export(wisdom.c_str()) != BFPSH_OK) std::cerr << "could not write " << wisdom << "\n";
     return EXIT_SUCCESS;
 }
 

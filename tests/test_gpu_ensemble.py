@@ -81,6 +81,17 @@ def test_ensemble_matches_separate_boxes(B, precision):
         ref[i].step(dt2[i])
     for i in range(len(sizes)):
         assert np.array_equal(ens.boxes[i].cvorticity, ref[i].cvorticity), i
+    # a box of the 48^3 class (no fixed-injection box so far) switches to
+    # that forcing: the graph must gain the band-amplitude launches
+    for f in (ens.boxes[1], ref[1]):
+        f.forcing_type, f.famplitude, f.fk0, f.fk1 = "fixed_energy_injection_rate", 0.2, 1.5, 3.5
+    for step in range(2):
+        ens.step(dt2, with_particles=False)
+        for i in range(len(sizes)):
+            ref[i].step(dt2[i])
+        for i in range(len(sizes)):
+            assert np.array_equal(ens.boxes[i].cvorticity, ref[i].cvorticity), (step, i)
+    say("  forcing switched to fixed injection: fields identical")
     ens.close()
 
 

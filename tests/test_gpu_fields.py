@@ -205,3 +205,107 @@ def test_sample_standalone_fields(B, precision):
         assert np.all(out == 7.0)
     else:
         assert np.abs(out[:, 0] - ref[:, 0]).max() <= 1e-12 * np.abs(ref).max()
+
+
+# ------------------------------------------------------------ pass shapes (autotuner)
+def _set_all(fs, idx):
+    for k in ("z_inv", "y_inv", "z_fwd", "y_fwd"):
+        fs.set_pass_shape(k, idx)
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_every_pass_shape_vs_oracle_256(B, oracle, precision):
+    """Every compiled tile shape of the strided passes gives the transforms
+    of the oracle at 256^3 (one component): dft and ift within the
+    tolerances of the default shape's tests."""
+    n = 256
+    rng = np.random.default_rng(256)
+    r = rng.standard_normal((n, n, n, 1))
+    ref = drop_nyquist(oracle.fft3d_r2c(r))
+    back = oracle.fft3d_c2r(ref, n)
+    fs = B.vorticity_equation("t", n, precision=precision, plan_rigor="FFTW_ESTIMATE")
+    nsh = fs.pass_shapes()["y_inv"][3]
+    say("case: %d pass shapes at 256^3 %s" % (nsh, precision))
+    assert nsh > 1
+    f = B.field(fs, 1)
+    tol = 1e-13 if precision == "double" else 1e-6
+    for idx in range(nsh):
+        _set_all(fs, idx)
+        e, b = fs.pass_shapes()["y_inv"][:2]
+        f.put(r, real_space=True)
+        f.dft()
+        c = f.get()
+        e_f = rel(c, ref)
+        f.ift()
+        e_b = rel(f.get(), back)
+        say("  shape %d (E=%d, B=%d): dft rel %.3e, ift(dft) rel %.3e" % (idx, e, b, e_f, e_b))
+        assert e_f < tol and e_b < tol
+    f.close()
+    fs.close()
+
+
+@pytest.mark.parametrize("n,precision", [(512, "single"), (512, "double"), (1024, "single")])
+def test_every_pass_shape_matches_default(B, n, precision):
+    """At 512^3 and 1024^3 every shape's dft and ift equal the built-in
+    shape's (validated against the oracle by the step and transform tests)
+    within rounding: FP64 1e-13, FP32 1e-6 relative L2."""
+    rng = np.random.default_rng(n)
+    dt = np.float32 if precision == "single" else np.float64
+    r = rng.standard_normal((n, n, n, 1), dtype=dt)
+    fs = B.vorticity_equation("t", n, precision=precision, plan_rigor="FFTW_ESTIMATE")
+    nsh = fs.pass_shapes()["y_inv"][3]
+    say("case: %d pass shapes at %d^3 %s" % (nsh, n, precision))
+    f = B.field(fs, 1)
+    tol = 1e-13 if precision == "double" else 1e-6
+    c0 = r0 = None
+    for idx in range(nsh):
+        _set_all(fs, idx)
+        e, b = fs.pass_shapes()["y_inv"][:2]
+        f.put(r, real_space=True)
+        f.dft()
+        c = f.get()
+        f.ift()
+        rr = f.get()
+        if idx == 0:
+            c0, r0 = c, rr
+            say("  shape 0 (E=%d, B=%d): reference" % (e, b))
+            continue
+        e_f, e_b = rel(c, c0), rel(rr, r0)
+        say("  shape %d (E=%d, B=%d): dft rel %.3e, ift rel %.3e" % (idx, e, b, e_f, e_b))
+        assert e_f < tol and e_b < tol
+        del c, rr
+    f.close()
+    fs.close()
+
+
+def test_plan_rigor_and_wisdom(B, tmp_path):
+    """MEASURE picks a compiled shape per pass kind and records it in the
+    wisdom; a new context of the same size with ESTIMATE then takes the
+    wisdom's shapes; the exported wisdom file reads back."""
+    B.wisdom_forget()
+    fs = B.vorticity_equation("t", 256, precision="single", plan_rigor="FFTW_MEASURE")
+    got = fs.pass_shapes()
+    say("case: plan rigor at 256^3: %s" % got)
+    for k, (e, b, idx, nsh) in got.items():
+        assert 0 <= idx < nsh and e > 0 and b > 0
+    path = str(tmp_path / "w.txt")
+    B.wisdom_export(path)
+    lines = [l for l in open(path) if not l.startswith("#")]
+    assert len(lines) == 4 and all("/f32/256/" in l for l in lines)
+    B.wisdom_forget()
+    fs2 = B.vorticity_equation("t2", 256, precision="single", plan_rigor="FFTW_ESTIMATE")
+    assert all(v[2] == 0 for v in fs2.pass_shapes().values())  # no wisdom: built-in shapes
+    fs2.close()
+    B.wisdom_import(path)
+    fs3 = B.vorticity_equation("t3", 256, precision="single", plan_rigor="FFTW_ESTIMATE")
+    assert fs3.pass_shapes() == got
+    # the step runs with the tuned shapes (the step tests cover its values)
+    fs3.io_checkpoint(read=True, data=np.zeros((256, 256, 129, 3), dtype=np.complex64))
+    fs3.step(1e-3)
+    assert np.abs(fs3.cvorticity).max() == 0.0
+    fs3.close()
+    fs.close()
+    with pytest.raises(B.capi.BfpshError):
+        fs4 = B.vorticity_equation("t4", 32, precision="single", plan_rigor="FFTW_WHATEVER")
+        fs4.close()
+    B.wisdom_forget()
