@@ -391,6 +391,74 @@ def cpu_baseline(ic, wl, threads, single_n=256):
                             "note": "FP32 vorticity step (no tracers) of a %d^3 box on one thread" % single_n}}
 
 
+def rocfft_leg(n, reps=5):
+    """rocFFT, the vendor FFT library, timed as a reference only: the
+    3-component FP32 r2c and c2r of an n^3 box (batched 3D transforms, not
+    in place, default plans, on one stream), as one vorticity step needs 6
+    vector c2r and 3 vector r2c (2 inverse and 1 forward per RK stage).
+    The library is loaded through ctypes; nothing of this build calls it."""
+    import ctypes
+    try:
+        rf = ctypes.CDLL("librocfft.so")
+    except OSError as e:
+        return {"error": "librocfft not loadable: %s" % e}
+    hip = ctypes.CDLL("libamdhip64.so")
+    P, S = ctypes.c_void_p, ctypes.c_size_t
+    rf.rocfft_setup()
+    ver = ctypes.create_string_buffer(64)
+    rf.rocfft_get_version_string(ver, S(64))
+    stream = P()
+    hip.hipStreamCreate(ctypes.byref(stream))
+    nr, nc = n ** 3, n * n * (n // 2 + 1)
+    rbuf, cbuf = P(), P()
+    if hip.hipMalloc(ctypes.byref(rbuf), S(3 * nr * 4)) or hip.hipMalloc(ctypes.byref(cbuf), S(3 * nc * 8)):
+        return {"error": "hipMalloc failed"}
+    hip.hipMemset(rbuf, 0, S(3 * nr * 4))
+    lengths = (S * 3)(n, n, n)
+    out = {"version": ver.value.decode(), "n": n, "components": 3, "precision": "f32"}
+    for name, ttype, src, dst in (("r2c", 2, rbuf, cbuf), ("c2r", 3, cbuf, rbuf)):
+        plan, info, work = P(), P(), P()
+        if rf.rocfft_plan_create(ctypes.byref(plan), 1, ttype, 0, S(3), lengths, S(3), None):
+            return {"error": "rocfft_plan_create failed (%s)" % name}
+        wsz = S(0)
+        rf.rocfft_plan_get_work_buffer_size(plan, ctypes.byref(wsz))
+        rf.rocfft_execution_info_create(ctypes.byref(info))
+        rf.rocfft_execution_info_set_stream(info, stream)
+        if wsz.value:
+            hip.hipMalloc(ctypes.byref(work), wsz)
+            rf.rocfft_execution_info_set_work_buffer(info, work, wsz)
+        ins, outs = (P * 1)(src), (P * 1)(dst)
+        for _ in range(2):
+            rf.rocfft_execute(plan, ins, outs, info)
+        a, b = P(), P()
+        hip.hipEventCreate(ctypes.byref(a))
+        hip.hipEventCreate(ctypes.byref(b))
+        hip.hipEventRecord(a, stream)
+        for _ in range(reps):
+            rf.rocfft_execute(plan, ins, outs, info)
+        hip.hipEventRecord(b, stream)
+        hip.hipEventSynchronize(b)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), a, b)
+        t = ms.value / reps
+        byt = 3 * (nr * 4 + nc * 8)
+        out[name] = {"ms": t, "bytes": byt, "achieved_gbps": byt / (t / 1e3) / 1e9,
+                     "frac": byt / (t / 1e3) / HBM_PEAK, "work_buffer_bytes": wsz.value}
+        hip.hipEventDestroy(a)
+        hip.hipEventDestroy(b)
+        rf.rocfft_execution_info_destroy(info)
+        rf.rocfft_plan_destroy(plan)
+        if wsz.value:
+            hip.hipFree(work)
+    hip.hipFree(rbuf)
+    hip.hipFree(cbuf)
+    hip.hipStreamDestroy(stream)
+    out["transforms_per_step_ms"] = 6 * out["c2r"]["ms"] + 3 * out["r2c"]["ms"]
+    out["note"] = ("6 vector c2r + 3 vector r2c: the transforms of one vorticity step alone, without its "
+                   "pointwise work, the fused Biot-Savart / cross product / update or the tracer velocity")
+    return out
+
+
 def copy_calibration(nbytes):
     """A plain device-to-device copy of known size: rocprofv3's FETCH_SIZE /
     WRITE_SIZE of this copy calibrate the counters of the other kernels."""
@@ -417,6 +485,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=20,
                     help="steps of the 96-box ensemble leg (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rocfft", action="store_true", help="skip the rocFFT reference leg")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--copy-gib", type=float, default=0.0,
@@ -491,6 +560,9 @@ def main():
     if args.ensemble_steps > 0:
         out["ensemble"] = ensemble_bench(B, ic, args.ensemble_steps)
         progress("ensemble leg done")
+    if not args.no_rocfft:
+        out["rocfft_reference"] = rocfft_leg(n)
+        progress("rocFFT reference leg done")
     if not args.no_cpu:
         threads = args.cpu_threads or cpu_threads_default()
         out["cpu_baseline"] = cpu_baseline(ic, wl, threads)

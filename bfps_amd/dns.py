@@ -192,9 +192,9 @@ def launch_jobs(simname, parameters, dns_type="NSVEparticles", work_dir=".", src
     generate_tracer_state(particle_rand_seed) (DNS.write_par,
     DNS.py:420-452; DNS.generate_tracer_state, DNS.py:711-733); then the
     run file (write_par) and the tracer-sample file.  Returns the driver,
-    not yet initialised.  The reference's random vector-field generator
-    (used when no source run is named) is not part of this build: without
-    src_simname, checkpoint 0 must already hold vorticity/complex/0."""
+    not yet initialised.  Without a source run (and no vorticity/complex/0
+    in checkpoint 0 yet) the initial vorticity is generate_vector_field with
+    the reference's seed 7547, slope 2 and amplitude 0.05."""
     p = parameters
     ckpt0 = os.path.join(work_dir, simname + "_checkpoint_0.h5")
     if not os.path.exists(os.path.join(work_dir, simname + ".h5")):
@@ -203,11 +203,13 @@ def launch_jobs(simname, parameters, dns_type="NSVEparticles", work_dir=".", src
         with h5io.File(ckpt0, "a") as f:
             if "vorticity/complex/0" not in f:
                 if src_simname is None:
-                    raise ValueError("no initial field: name a source run (src_simname) or write "
-                                     "vorticity/complex/0 into %s" % ckpt0)
-                src = find_checkpoint(src_simname, src_work_dir, src_iteration)
-                copy_complex_field(src, "vorticity/complex/%d" % src_iteration, f, "vorticity/complex/0",
-                                   p["nx"], ctype)
+                    # DNS.launch_jobs without a source run (DNS.py:880-884)
+                    f["vorticity/complex/0"] = generate_vector_field(p, spectra_slope=2.0, amplitude=0.05,
+                                                                     precision=precision)
+                else:
+                    src = find_checkpoint(src_simname, src_work_dir, src_iteration)
+                    copy_complex_field(src, "vorticity/complex/%d" % src_iteration, f, "vorticity/complex/0",
+                                       p["nx"], ctype)
             particles = dns_type in ("NSVEparticles", "NSVEparticles_no_output") and p.get("nparticles", 0) > 0
             if particles:
                 np_ = p["nparticles"]
@@ -229,6 +231,91 @@ def generate_tracer_state(nparticles, rseed=None, batch_size=10 ** 6):
         out[done:done + m] = rand((m, 3)) * 2 * np.pi
         done += m
     return out
+
+
+# ---------------------------------------------------------------- initial fields
+def generate_data_3D(n0, n1, n2, dtype=np.complex128, p=1.5, amplitude=0.5, rng=None):
+    """tools.generate_data_3D (bfps/tools.py:96-144): a scalar Gaussian
+    random field in the FFTW layout [y][z][x/2+1] (n0, n1, n2 the z, y, x
+    real-space sizes): amplitude (g1 + i g2) with g1, g2 standard normal
+    draws (the real parts first, then the imaginary parts, from numpy's
+    legacy global generator, or `rng` -- a RandomState -- when given),
+    divided by |k|^p (|k| of the signed wavenumbers, 1 at k = 0); the k = 0
+    mode set to 0; the kx = 0 plane made Hermitian in the reference's order
+    (ky axis, kz axis, then every (ky, kz) with 1 <= ky < n1/2 taking the
+    conjugate of (n1-ky, n0-kz)'s mirror); every mode with |k| > min(n)/3
+    set to 0."""
+    if n0 % 2 or n1 % 2 or n2 % 2:
+        raise ValueError("grid sizes must be even")
+    randn = rng.randn if rng is not None else np.random.randn
+    shape = (n1, n0, n2 // 2 + 1)
+    a = np.zeros(shape, dtype=dtype)
+    re = randn(*shape)
+    im = randn(*shape)
+    a[:] = amplitude * (re + 1j * im)
+    # signed wavenumbers: index i <-> i for i <= n/2, i - n above (the
+    # reference builds -n/2+1 .. n/2 and rolls it by n/2+1)
+    ky = np.where(np.arange(n1) <= n1 // 2, np.arange(n1), np.arange(n1) - n1)
+    kz = np.where(np.arange(n0) <= n0 // 2, np.arange(n0), np.arange(n0) - n0)
+    kx = np.arange(n2 // 2 + 1)
+    k = np.sqrt(ky[:, None, None] ** 2 + kz[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    k[0, 0, 0] = 1
+    a /= k ** p
+    a[0, 0, 0] = 0
+    for y in range(1, n1 // 2):
+        a[n1 - y, 0, 0] = np.conj(a[y, 0, 0])
+    for z in range(1, n0 // 2):
+        a[0, n0 - z, 0] = np.conj(a[0, z, 0])
+    for y in range(1, n1 // 2):
+        a[n1 - y, n0 - np.arange(1, n0), 0] = np.conj(a[y, 1:n0, 0])
+    a[k > min(n0, n1, n2) / 3.] = 0
+    return a
+
+
+def generate_data_3D_uniform(n0, n1, n2, dtype=np.complex128, p=1.5, amplitude=1.0, rng=None):
+    """tools.generate_data_3D_uniform (bfps/tools.py:61-94): every mode of
+    the ky = 0 slab set to `amplitude` (p and rng are ignored)."""
+    if n0 % 2 or n1 % 2 or n2 % 2:
+        raise ValueError("grid sizes must be even")
+    a = np.zeros((n1, n0, n2 // 2 + 1), dtype=dtype)
+    a[0] = amplitude
+    return a
+
+
+def padd_with_zeros(a, n0, n1, n2, odtype=None):
+    """tools.padd_with_zeros (bfps/tools.py:169-204): the modes of `a`
+    ([y][z][x/2+1], possibly with trailing component axes) placed into a
+    zero field of real-space size (n0, n1, n2): the four (ky, kz) corner
+    blocks of half the source sizes, kx up to m2//2 (m2 = a.shape[2])."""
+    odtype = a.dtype if odtype is None else odtype
+    if not (a.shape[0] <= n0 and a.shape[1] <= n1 and a.shape[2] <= n2 // 2 + 1):
+        raise ValueError("source larger than the target grid")
+    b = np.zeros((n0, n1, n2 // 2 + 1) + a.shape[3:], dtype=odtype)
+    m0, m1, m2 = a.shape[1], a.shape[0], a.shape[2]
+    b[:m1 // 2, :m0 // 2, :m2 // 2 + 1] = a[:m1 // 2, :m0 // 2, :m2 // 2 + 1]
+    b[:m1 // 2, n0 - m0 // 2:, :m2 // 2 + 1] = a[:m1 // 2, m0 - m0 // 2:, :m2 // 2 + 1]
+    b[n1 - m1 // 2:, :m0 // 2, :m2 // 2 + 1] = a[m1 - m1 // 2:, :m0 // 2, :m2 // 2 + 1]
+    b[n1 - m1 // 2:, n0 - m0 // 2:, :m2 // 2 + 1] = a[m1 - m1 // 2:, m0 - m0 // 2:, :m2 // 2 + 1]
+    return b
+
+
+def generate_vector_field(parameters, rseed=7547, spectra_slope=1., amplitude=1., precision="single",
+                          scalar_generator=generate_data_3D):
+    """DNS.generate_vector_field (bfps/DNS.py:734-809): numpy's legacy
+    generator seeded with rseed, three scalar fields of half the grid
+    (nz/2, ny/2, nx/2) from `scalar_generator` with slope spectra_slope and
+    `amplitude`, cast to the run's complex type, stacked as components and
+    padded with zeros to the full grid: [ny][nz][nx/2+1][3].  Not
+    divergence free (the load projection of io_checkpoint makes it so)."""
+    p = parameters
+    ctype = np.complex64 if precision == "single" else np.complex128
+    rng = np.random.RandomState(rseed)
+    comps = [scalar_generator(p["nz"] // 2, p["ny"] // 2, p["nx"] // 2, p=spectra_slope, amplitude=amplitude,
+                              rng=rng).astype(ctype) for _ in range(3)]
+    k0 = np.zeros(comps[0].shape + (3,), dtype=comps[0].dtype)
+    for c in range(3):
+        k0[..., c] = comps[c]
+    return padd_with_zeros(k0, p["nz"], p["ny"], p["nx"])
 
 
 ENSEMBLE_CLASSES = ((32, 32), (32, 48), (24, 64), (8, 128))  # (boxes, N)
@@ -470,4 +557,5 @@ class NSVEparticles:
 
 
 __all__ = ["default_parameters", "launch_parameters", "kspace_info", "write_par", "read_iteration",
-           "copy_complex_field", "find_checkpoint", "launch_jobs", "generate_tracer_state", "NSVEparticles"]
+           "copy_complex_field", "find_checkpoint", "launch_jobs", "generate_tracer_state", "NSVEparticles",
+           "generate_data_3D", "generate_data_3D_uniform", "padd_with_zeros", "generate_vector_field"]

@@ -117,7 +117,72 @@ def test_launch_jobs_from_source_run(tmp_path, golden):
         w = f["vorticity/complex/0"]
     assert w.shape == (48, 48, 25, 3)
     assert np.array_equal(w[:32, :32, :17], golden["vort_0"]) and not w[32:].any() and not w[:, 32:].any()
-    # without a source the initial field must already be there
-    import pytest
-    with pytest.raises(ValueError):
-        dns.launch_jobs("none", q, work_dir=str(tmp_path / "x"))
+    # without a source: DNS.generate_vector_field(spectra_slope=2, amplitude=0.05),
+    # seed 7547 (DNS.py:880-884)
+    dns.launch_jobs("gen", q, work_dir=str(tmp_path / "x"), particle_rand_seed=1)
+    with h5io.File(os.path.join(str(tmp_path / "x"), "gen_checkpoint_0.h5")) as f:
+        w = f["vorticity/complex/0"]
+    want = dns.generate_vector_field(q, spectra_slope=2.0, amplitude=0.05)
+    assert w.dtype == np.complex64 and np.array_equal(w, want) and np.abs(w).max() > 0
+
+
+def _signed(n):
+    return np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n)
+
+
+def test_generate_data_3D_structure():
+    """tools.generate_data_3D (bfps/tools.py:96-144): the draws of numpy's
+    legacy generator (real parts, then imaginary parts) over |k|^p on the
+    modes the symmetrisation and the |k| > n/3 cut leave alone; the k = 0
+    mode zero; the kx = 0 plane Hermitian; nothing beyond n/3.  (Exact
+    values are not pinned by a reference vector: bfps.tools imports h5py,
+    which is absent here.)"""
+    n0, n1, n2, p, amp = 12, 16, 20, 1.5, 0.5
+    a = dns.generate_data_3D(n0, n1, n2, p=p, amplitude=amp, rng=np.random.RandomState(3))
+    assert a.shape == (n1, n0, n2 // 2 + 1) and a.dtype == np.complex128
+    rs = np.random.RandomState(3)
+    g = amp * (rs.randn(n1, n0, n2 // 2 + 1) + 1j * rs.randn(n1, n0, n2 // 2 + 1))
+    ky, kz, kx = _signed(n1), _signed(n0), np.arange(n2 // 2 + 1)
+    k = np.sqrt(ky[:, None, None] ** 2 + kz[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    cut = k > min(n0, n1, n2) / 3.
+    assert not a[cut].any() and a[0, 0, 0] == 0
+    # modes outside the kx = 0 plane's mirrored half keep their draw
+    keep = ~cut
+    keep[:, :, 0] = False
+    kk = k.copy()
+    kk[0, 0, 0] = 1
+    assert np.allclose(a[keep], (g / kk ** p)[keep], rtol=1e-15, atol=0)
+    # Hermitian kx = 0 plane (inside the cut)
+    for y in range(n1):
+        for z in range(n0):
+            if not cut[y, z, 0]:
+                assert a[(n1 - y) % n1, (n0 - z) % n0, 0] == np.conj(a[y, z, 0]) or (y in (0, n1 // 2) and z in (0, n0 // 2))
+    # the global generator is used when no RandomState is given
+    np.random.seed(3)
+    assert np.array_equal(dns.generate_data_3D(n0, n1, n2, p=p, amplitude=amp), a)
+
+
+def test_padd_with_zeros_and_vector_field():
+    # (the reference pads cubic grids: it mixes the y and z sizes of
+    # non-cubic ones, bfps/tools.py:196-203)
+    a = np.arange(6 * 6 * 4 * 2).reshape(6, 6, 4, 2).astype(np.complex64) + 1
+    b = dns.padd_with_zeros(a, 10, 10, 12)
+    assert b.shape == (10, 10, 7, 2) and b.dtype == np.complex64
+    # the four (ky, kz) corner blocks of half the source sizes, kx < 3
+    assert np.array_equal(b[:3, :3, :3], a[:3, :3, :3]) and np.array_equal(b[-3:, -3:, :3], a[-3:, -3:, :3])
+    assert np.array_equal(b[:3, -3:, :3], a[:3, -3:, :3]) and np.array_equal(b[-3:, :3, :3], a[-3:, :3, :3])
+    assert b.astype(bool).sum() == 4 * 3 * 3 * 3 * 2
+    # vector field: three seeded components of the half grid, padded
+    q = dns.launch_parameters(32)
+    v = dns.generate_vector_field(q, rseed=11, spectra_slope=2.0, amplitude=0.05)
+    assert v.shape == (32, 32, 17, 3) and v.dtype == np.complex64
+    assert np.array_equal(v, dns.generate_vector_field(q, rseed=11, spectra_slope=2.0, amplitude=0.05))
+    rs = np.random.RandomState(11)
+    comps = [dns.generate_data_3D(16, 16, 16, p=2.0, amplitude=0.05, rng=rs).astype(np.complex64) for _ in range(3)]
+    for c in range(3):
+        assert np.array_equal(v[..., c], dns.padd_with_zeros(comps[c], 32, 32, 32))
+    ky = _signed(32)
+    assert not v[np.abs(ky) >= 8].any() and not v[:, np.abs(ky) >= 8].any() and not v[:, :, 9:].any()
+    assert not np.array_equal(v[..., 0], v[..., 1])
+    u = dns.generate_vector_field(q, rseed=11, precision="double", scalar_generator=dns.generate_data_3D_uniform)
+    assert u.dtype == np.complex128 and np.all(u[0, :8, :5] == 1.0) and u.astype(bool).sum() == 3 * 16 * 5
