@@ -18,7 +18,7 @@ CVORTICITY, CVELOCITY, RVELOCITY, RGRADIENT = 0, 1, 5, 6
 ESTIMATE, MEASURE, PATIENT = 0, 1, 2
 PLAN_RIGOR = {"FFTW_ESTIMATE": ESTIMATE, "FFTW_MEASURE": MEASURE, "FFTW_PATIENT": PATIENT,
               ESTIMATE: ESTIMATE, MEASURE: MEASURE, PATIENT: PATIENT}
-PASS_KINDS = {"z_inv": 0, "y_inv": 1, "z_fwd": 2, "y_fwd": 3}
+PASS_KINDS = {"z_inv": 0, "y_inv": 1, "z_fwd": 2, "y_fwd": 3, "bs": 4, "y3": 5}
 
 # every exported entry point: name -> (restype, argtypes)
 _P = ctypes.c_void_p

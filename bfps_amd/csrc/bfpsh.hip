@@ -778,7 +778,7 @@ namespace {
 std::mutex wisdom_mu;
 std::map<std::string, PassWisdom> wisdom;
 std::map<int, std::string> arch_names;  // device -> architecture (gcnArchName up to ':')
-const char* const kPassKindName[4] = {"z_inv", "y_inv", "z_fwd", "y_fwd"};
+const char* const kPassKindName[BFPSH_PASS_KINDS] = {"z_inv", "y_inv", "z_fwd", "y_fwd", "bs", "y3"};
 }  // namespace
 
 std::string pass_wisdom_key(const bfpsh_ctx* ctx, int kind) {
@@ -799,7 +799,7 @@ std::string pass_wisdom_key(const bfpsh_ctx* ctx, int kind) {
         arch_names[ctx->device] = arch;
     }
     return arch + "/" + (ctx->prec == BFPSH_FP32 ? "f32" : "f64") + "/" + std::to_string(ctx->n) + "/" +
-           kPassKindName[kind & 3];
+           kPassKindName[(kind >= 0 && kind < BFPSH_PASS_KINDS) ? kind : 0];
 }
 
 bool pass_wisdom_lookup(const std::string& key, PassWisdom* w) {
@@ -825,8 +825,8 @@ int bfpsh_set_plan_rigor(bfpsh_ctx* ctx, int rigor) {
 
 int bfpsh_get_pass_shape(bfpsh_ctx* ctx, int kind, int* e, int* b, int* index, int* nshapes) {
     CTX_CHECK();
-    if (kind < 0 || kind > 3) return fail(ctx, BFPSH_EINVAL, "bad pass kind");
-    const PassShapes sh = ops_for(ctx)->pass_shapes();
+    if (kind < 0 || kind >= BFPSH_PASS_KINDS) return fail(ctx, BFPSH_EINVAL, "bad pass kind");
+    const PassShapes sh = ops_for(ctx)->pass_shapes(kind);
     const int v = ctx->pass_var[kind] >= 0 && ctx->pass_var[kind] < sh.n ? ctx->pass_var[kind] : 0;
     if (e) *e = sh.v[v].e;
     if (b) *b = sh.v[v].b;
@@ -837,8 +837,8 @@ int bfpsh_get_pass_shape(bfpsh_ctx* ctx, int kind, int* e, int* b, int* index, i
 
 int bfpsh_set_pass_shape(bfpsh_ctx* ctx, int kind, int index) {
     CTX_CHECK();
-    if (kind < 0 || kind > 3) return fail(ctx, BFPSH_EINVAL, "bad pass kind");
-    const PassShapes sh = ops_for(ctx)->pass_shapes();
+    if (kind < 0 || kind >= BFPSH_PASS_KINDS) return fail(ctx, BFPSH_EINVAL, "bad pass kind");
+    const PassShapes sh = ops_for(ctx)->pass_shapes(kind);
     if (index < 0 || index >= sh.n) return fail(ctx, BFPSH_EINVAL, "pass shape index out of range");
     ctx->pass_var[kind] = index;
     return BFPSH_OK;
@@ -922,7 +922,9 @@ struct bfpsh_ensemble {
     bool captured_ru = false;          // the graph writes the real velocity
     std::vector<char> captured_fixed;  // per class: the graph holds the band-amplitude launches
     static constexpr int NPB = 4;      // neighbour counts batched in one step
-    PartBatch pb[NPB];                 // tracer-loop tables, one per neighbour count
+    PartBatch pb[NPB];                 // tracer-loop groups, one per neighbour count
+    int ngroups = 0;                   // groups the captured graph launches
+    std::vector<uint64_t> captured_layout;  // their launch shapes and tables
 };
 
 static int efail(bfpsh_ensemble* e, int code, const std::string& msg) {
@@ -990,7 +992,23 @@ static bool fixed_changed(const bfpsh_ensemble* e) {
     return false;
 }
 
-static int ensemble_capture(bfpsh_ensemble* e, bool with_ru) {
+// what the captured tracer launches depend on: per group its neighbour
+// count, precision, grid, LDS, table and state addresses
+static std::vector<uint64_t> tracer_layout(const bfpsh_ensemble* e, int ngroups) {
+    std::vector<uint64_t> v;
+    for (int g = 0; g < ngroups; g++) {
+        const PartBatch& b = e->pb[g];
+        v.insert(v.end(), {uint64_t(b.nbr), uint64_t(b.prec), uint64_t(b.blocks), uint64_t(b.nboxes), uint64_t(b.lds),
+                           uint64_t(reinterpret_cast<uintptr_t>(b.d)), uint64_t(reinterpret_cast<uintptr_t>(b.d_dyn))});
+    }
+    return v;
+}
+
+// One graph per step: the field kernels of every class (the stage-1 real
+// velocity of each box kept in its ru when with_ru) and, with the tracers,
+// the loop of each neighbour-count group and the advance of its device
+// Adams-Bashforth state.
+static int ensemble_capture(bfpsh_ensemble* e, bool with_ru, int ngroups) {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     e->exec = nullptr;
@@ -1024,6 +1042,10 @@ static int ensemble_capture(bfpsh_ensemble* e, bool with_ru) {
             break;
         }
     }
+    if (rc == BFPSH_OK && with_ru && ngroups > 0) {
+        rc = bfpsh_part_ens_launch(e->pb, ngroups);
+        if (rc) e->err = e->pb[0].ctx ? e->pb[0].ctx->err : "tracer launch failed";
+    }
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(e->stream, &g);
     for (size_t i = 0; i < e->boxes.size(); i++) e->boxes[i]->prof = prof[i];
@@ -1035,6 +1057,8 @@ static int ensemble_capture(bfpsh_ensemble* e, bool with_ru) {
     e->graph = g;
     EHIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
     e->captured_ru = with_ru;
+    e->ngroups = with_ru ? ngroups : 0;
+    e->captured_layout = tracer_layout(e, e->ngroups);
     return BFPSH_OK;
 }
 
@@ -1120,19 +1144,9 @@ int bfpsh_ensemble_step(bfpsh_ensemble* e, const double* dt, int with_particles)
     bool changed = false;
     int r = ensemble_params(e, dt, &changed);
     if (r) return r;
-    bool any_part = false;
-    for (bfpsh_ctx* c : e->boxes) any_part |= with_particles && c->nsteps > 0 && c->np > 0;
-    // parameter values reach the graph through the device table; a box
-    // switching to or from the fixed-injection forcing changes its launches
-    if (!e->exec || e->captured_ru != any_part || (changed && fixed_changed(e))) {
-        r = ensemble_capture(e, any_part);
-        if (r) return r;
-    }
-    EHIPCHK(hipGraphLaunch(e->exec, e->stream));
-    if (any_part) {
-        // the tracers of all boxes, one launch per neighbour count
-        std::vector<bfpsh_ctx*> pbox;
-        std::vector<double> pdt;
+    std::vector<bfpsh_ctx*> pbox;
+    std::vector<double> pdt;
+    if (with_particles)
         for (size_t i = 0; i < e->boxes.size(); i++) {
             bfpsh_ctx* c = e->boxes[i];
             if (c->nsteps > 0 && c->np > 0) {
@@ -1140,9 +1154,24 @@ int bfpsh_ensemble_step(bfpsh_ensemble* e, const double* dt, int with_particles)
                 pdt.push_back(dt[i]);
             }
         }
-        r = bfpsh_part_loop_batch(pbox, pdt, e->pb, bfpsh_ensemble::NPB);
+    const bool any_part = !pbox.empty();
+    int ngroups = 0;
+    if (any_part) {
+        // tracer tables and device AB state (rewritten only on a change),
+        // and the periodic sorts of the boxes that are due
+        r = bfpsh_part_ens_prepare(pbox, pdt, e->pb, bfpsh_ensemble::NPB, &ngroups);
         if (r) return efail(e, r, pbox[0]->err);
     }
+    // parameter values reach the graph through the device tables; a box
+    // switching to or from the fixed-injection forcing, or a change of the
+    // tracer launch shapes, changes its launches
+    if (!e->exec || e->captured_ru != any_part || (changed && fixed_changed(e)) ||
+        tracer_layout(e, ngroups) != e->captured_layout) {
+        r = ensemble_capture(e, any_part, ngroups);
+        if (r) return r;
+    }
+    EHIPCHK(hipGraphLaunch(e->exec, e->stream));
+    if (any_part) bfpsh_part_ens_advanced(pbox, e->pb, ngroups);
     for (bfpsh_ctx* c : e->boxes) {
         c->ru_valid = false;
         c->iteration++;

@@ -73,7 +73,7 @@ struct bfpsh_ctx {
     int sort_age = 1 << 30;        // steps since the last sort
     // strided-pass tile shapes: index into pass_shapes<T, N>() per pass kind
     // (BFPSH_PASS_*), chosen by the pass autotuner (bfpsh_set_plan_rigor)
-    int pass_var[4] = {0, 0, 0, 0};
+    int pass_var[BFPSH_PASS_KINDS] = {0, 0, 0, 0, 0, 0};
     int plan_rigor = BFPSH_ESTIMATE;
     // profiling
     bool prof = false;
@@ -223,7 +223,7 @@ struct OpsTable {
     int (*rspace_stats)(bfpsh_ctx*, const void*, int, int, const double*, double*, int64_t*);
     int (*compute_vorticity)(bfpsh_ctx*);
     int (*tune_passes)(bfpsh_ctx*, int);
-    PassShapes (*pass_shapes)();
+    PassShapes (*pass_shapes)(int kind);
 };
 
 // Process-wide pass wisdom (bfpsh.hip): the tile shape the autotuner chose
@@ -247,18 +247,30 @@ int create_ctx(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, 
 int bfpsh_part_loop(bfpsh_ctx* ctx, double dt);
 int bfpsh_part_sample(bfpsh_ctx* ctx, const void* field, int nc, double* d_out);
 int bfpsh_part_release(bfpsh_ctx* ctx);
-// tracer loops of many boxes (on one stream) in one launch per neighbour
-// count; pb holds nbatch device tables and their pinned host staging
+// Tracer loops of many boxes (an ensemble, one stream) in one launch per
+// neighbour count.  A PartBatch is one such group: its device table of
+// boxes and their device-side Adams-Bashforth state, with host copies of
+// what the device holds (so that a graph replaying the launches needs no
+// host arguments, and the host rewrites them only on a change).
 struct PartBatch {
-    void* d = nullptr;          // device table
-    size_t cap = 0;
-    void* h[2] = {nullptr, nullptr};  // pinned host staging, used in turn
-    size_t hcap = 0;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    int k = 0;
+    void* d = nullptr;            // device table of boxes
+    void* d_dyn = nullptr;        // device AB state, 4 ints per box
+    size_t cap = 0;               // table bytes allocated
+    std::vector<unsigned char> tab;  // the table the device holds
+    std::vector<int> dyn;            // the AB state the device holds (after the last launch)
+    int nboxes = 0, nbr = 0, prec = 0, maxdeg = 0;
+    uint32_t blocks = 0;          // workgroups of the loop launch
+    size_t lds = 0;               // its dynamic LDS bytes
+    bfpsh_ctx* ctx = nullptr;     // launches go to this box's (the ensemble's) stream
 };
-int bfpsh_part_loop_batch(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch* pb,
-                          int nbatch);
+// Build or refresh the tables of the groups (sorting the boxes that are
+// due); *ngroups receives the number of groups used.
+int bfpsh_part_ens_prepare(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch* pb,
+                           int nbatch, int* ngroups);
+// Launch the loops and the AB-state advance of every group (capturable).
+int bfpsh_part_ens_launch(PartBatch* pb, int ngroups);
+// After a launch: advance the host's view of each box and of the device state.
+void bfpsh_part_ens_advanced(const std::vector<bfpsh_ctx*>& boxes, PartBatch* pb, int ngroups);
 void bfpsh_part_batch_free(PartBatch& pb);
 
 #define BFPSH_FOR_SIZES(X) X(16) X(32) X(48) X(64) X(128) X(256) X(512) X(1024)

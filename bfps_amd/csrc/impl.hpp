@@ -69,6 +69,54 @@ struct Impl {
     }
     static PassShapes shapes() { return SH; }
 
+    // ---------------------------------------------------------- fused passes
+    // tile shapes of the Biot-Savart first pass (MODE 0, the step's) and of
+    // the last forward pass with the update; ctx->pass_var[BFPSH_PASS_BS]
+    // and [BFPSH_PASS_Y3] select one (the velocity and gradient forms of the
+    // first pass and the batched forms keep the built-in shape)
+    static constexpr PassShapes SH_BS = fused_shapes<T, N>(CF::E_BS, CF::B_BS);
+    static constexpr PassShapes SH_Y3 = fused_shapes<T, N>(CF::E_Y3, CF::B_Y3);
+    typedef void (*BsFn)(hipStream_t, const CT*, CT*, const CT*, const Geom&, int, size_t);
+    template <int I>
+    static void bs_launch(hipStream_t st, const CT* src, CT* w, const CT* tw, const Geom& g, int nbox, size_t bstride) {
+        constexpr PassShape p = SH_BS.v[I < SH_BS.n ? I : 0];
+        constexpr int nt = (N / p.e) * p.b;
+        k_inv_z_bs<T, N, 0, true, p.e, p.b><<<dim3(1, N, nbox), nt, 0, st>>>(src, w, tw, g, 0, bstride);
+        if (NXH / p.b > 1)
+            k_inv_z_bs<T, N, 0, false, p.e, p.b><<<dim3(NXH / p.b - 1, N, nbox), nt, 0, st>>>(src, w, tw, g, 1,
+                                                                                              bstride);
+    }
+    template <size_t... I>
+    static constexpr std::array<BsFn, kMaxPassShapes> bs_table(std::index_sequence<I...>) {
+        return {{&bs_launch<int(I)>...}};
+    }
+    static BsFn bs_fn(int v) {
+        static constexpr std::array<BsFn, kMaxPassShapes> t = bs_table(std::make_index_sequence<kMaxPassShapes>{});
+        return t[(v >= 0 && v < SH_BS.n) ? v : 0];
+    }
+    typedef void (*Y3Fn)(hipStream_t, const CT*, const CT*, const CT*, CT*, const CT*, const Geom&, const StepParams&);
+    template <int S, int I>
+    static void y3_launch(hipStream_t st, const CT* w, const CT* v0, const CT* vp, CT* dst, const CT* tw,
+                          const Geom& g, const StepParams& sp) {
+        constexpr PassShape p = SH_Y3.v[I < SH_Y3.n ? I : 0];
+        k_fwd_y_update<T, N, S, false, p.e, p.b><<<dim3(NXH / p.b, N, 1), (N / p.e) * p.b, 0, st>>>(
+            w, v0, vp, dst, tw, g, sp, 0, nullptr);
+    }
+    template <int S, size_t... I>
+    static constexpr std::array<Y3Fn, kMaxPassShapes> y3_table(std::index_sequence<I...>) {
+        return {{&y3_launch<S, int(I)>...}};
+    }
+    static Y3Fn y3_fn(int s, int v) {
+        static constexpr std::array<Y3Fn, kMaxPassShapes> t1 = y3_table<1>(std::make_index_sequence<kMaxPassShapes>{});
+        static constexpr std::array<Y3Fn, kMaxPassShapes> t2 = y3_table<2>(std::make_index_sequence<kMaxPassShapes>{});
+        static constexpr std::array<Y3Fn, kMaxPassShapes> t3 = y3_table<3>(std::make_index_sequence<kMaxPassShapes>{});
+        v = (v >= 0 && v < SH_Y3.n) ? v : 0;
+        return s == 1 ? t1[v] : (s == 2 ? t2[v] : t3[v]);
+    }
+    static PassShapes shapes_of(int kind) {
+        return kind == BFPSH_PASS_BS ? SH_BS : (kind == BFPSH_PASS_Y3 ? SH_Y3 : SH);
+    }
+
     // The pass autotuner: for each pass kind, the shape held by the wisdom
     // when a search of at least this rigor chose it (any wisdom for
     // ESTIMATE), else the fastest of all compiled shapes timed on one
@@ -77,17 +125,22 @@ struct Impl {
     // (bfps/cpp/fftw_interface.hpp:30-35) and reuses its wisdom.
     static int tune_passes(bfpsh_ctx* ctx, int rigor) {
         ctx->plan_rigor = rigor;
+        {
+            int r = ensure_work(ctx, 6);
+            if (r) return r;
+        }
         void* scratch = nullptr;
         hipEvent_t a = nullptr, b = nullptr;
         int rc = BFPSH_OK;
-        for (int kind = 0; kind < 4 && rc == BFPSH_OK; kind++) {
+        for (int kind = 0; kind < BFPSH_PASS_KINDS && rc == BFPSH_OK; kind++) {
             ctx->pass_var[kind] = 0;
-            if (SH.n == 1) continue;
+            const PassShapes sh = shapes_of(kind);
+            if (sh.n == 1) continue;
             const std::string key = pass_wisdom_key(ctx, kind);
             PassWisdom w;
             if (pass_wisdom_lookup(key, &w) && w.rigor >= rigor) {
-                for (int v = 0; v < SH.n; v++)
-                    if (SH.v[v].e == w.e && SH.v[v].b == w.b) ctx->pass_var[kind] = v;
+                for (int v = 0; v < sh.n; v++)
+                    if (sh.v[v].e == w.e && sh.v[v].b == w.b) ctx->pass_var[kind] = v;
                 continue;
             }
             if (rigor == BFPSH_ESTIMATE) continue;
@@ -102,13 +155,29 @@ struct Impl {
             const int reps = rigor >= BFPSH_PATIENT ? 10 : 3;
             double best = 1e300;
             int bestv = 0;
-            for (int v = 0; v < SH.n && rc == BFPSH_OK; v++) {
-                const PassFn f = pass_fn(kind, v);
-                CT* d = static_cast<CT*>(scratch);
+            for (int v = 0; v < sh.n && rc == BFPSH_OK; v++) {
                 const CT* tw = (const CT*)ctx->tw;
-                f(ctx->stream, d, pass_estride(kind), pass_ostride(kind), tw, 1, 0, 1);  // warm-up
+                // one launch of shape v: a plain pass over the scratch plane;
+                // the first pass from cvorticity into the work planes; the
+                // last pass (stage 2) from work planes 0..2 into 3..5
+                auto run = [&]() {
+                    if (kind == BFPSH_PASS_BS) {
+                        bs_fn(v)(ctx->stream, (const CT*)ctx->v0, (CT*)ctx->work, tw, geom(ctx), 1, 0);
+                    } else if (kind == BFPSH_PASS_Y3) {
+                        StepParams sp = ctx->sp;
+                        sp.forcing = BFPSH_FORCING_NONE;
+                        sp.famp_dev = nullptr;
+                        const CT* w = (const CT*)ctx->work;
+                        y3_fn(2, v)(ctx->stream, w, (const CT*)ctx->v0, (const CT*)ctx->v0,
+                                    (CT*)ctx->work + 3 * ctx->ps, tw, geom(ctx), sp);
+                    } else {
+                        pass_fn(kind, v)(ctx->stream, static_cast<CT*>(scratch), pass_estride(kind),
+                                         pass_ostride(kind), tw, 1, 0, 1);
+                    }
+                };
+                run();  // warm-up
                 (void)hipEventRecord(a, ctx->stream);
-                for (int r = 0; r < reps; r++) f(ctx->stream, d, pass_estride(kind), pass_ostride(kind), tw, 1, 0, 1);
+                for (int r = 0; r < reps; r++) run();
                 (void)hipEventRecord(b, ctx->stream);
                 hipError_t e = hipGetLastError();
                 if (e == hipSuccess) e = hipEventSynchronize(b);
@@ -125,8 +194,8 @@ struct Impl {
             }
             if (rc == BFPSH_OK) {
                 ctx->pass_var[kind] = bestv;
-                w.e = SH.v[bestv].e;
-                w.b = SH.v[bestv].b;
+                w.e = sh.v[bestv].e;
+                w.b = sh.v[bestv].b;
                 w.ms = best;
                 w.rigor = rigor;
                 pass_wisdom_store(key, w);
@@ -190,9 +259,11 @@ struct Impl {
                 LAUNCH("band_amplitude", (k_band_amplitude<T, N><<<1, 256, 0, ctx->stream>>>(
                                              src, bt.famp + (s - 1), g, bt.sp.famp, bt.sp.fk0, bt.sp.fk1, 0, nullptr, s)));
         }
-        {
+        if (bt.spb) {
             int r = bs_pass<0>(ctx, src, w, tw, g, "inv_z_bs", nb, bs);
             if (r) return r;
+        } else {
+            LAUNCH("inv_z_bs", bs_fn(ctx->pass_var[BFPSH_PASS_BS])(ctx->stream, src, w, tw, g, 1, 0));
         }
         {
             int r = pass(ctx, BFPSH_PASS_Y_INV, w, 6, "inv_y", nb, bs);
@@ -217,9 +288,7 @@ struct Impl {
             else if (s == 2) LAUNCH(nm, (k_fwd_y_update<T, N, 2, true><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, bs, bt.spb)));
             else LAUNCH(nm, (k_fwd_y_update<T, N, 3, true><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, bs, bt.spb)));
         } else {
-            if (s == 1) LAUNCH(nm, (k_fwd_y_update<T, N, 1, false><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, 0, nullptr)));
-            else if (s == 2) LAUNCH(nm, (k_fwd_y_update<T, N, 2, false><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, 0, nullptr)));
-            else LAUNCH(nm, (k_fwd_y_update<T, N, 3, false><<<tiles3, CF::NT_Y3, 0, ctx->stream>>>(w, v0, vp, dst, tw, g, bt.sp, 0, nullptr)));
+            LAUNCH(nm, y3_fn(s, ctx->pass_var[BFPSH_PASS_Y3])(ctx->stream, w, v0, vp, dst, tw, g, bt.sp));
         }
         return BFPSH_OK;
     }

@@ -10,6 +10,6 @@ const OpsTable* BFPSH_NAME(BFPSH_TN, BFPSH_N)() {
     static const OpsTable t = {&I::step, &I::load_projection, &I::velocity_rspace, &I::gradient_rspace,
                                &I::spectral_velocity, &I::cospectrum, &I::field_ift, &I::field_dft,
                                &I::batch_step_desc, &I::pressure_v, &I::lagrangian_acceleration_v, &I::normalize,
-                               &I::rspace_stats, &I::compute_vorticity, &I::tune_passes, &I::shapes};
+                               &I::rspace_stats, &I::compute_vorticity, &I::tune_passes, &I::shapes_of};
     return &t;
 }

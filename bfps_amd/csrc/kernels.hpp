@@ -153,15 +153,15 @@ template <int MODE> struct BsOut { static constexpr int NF = MODE == 0 ? 6 : (MO
 // MIRROR: the launch covers the kx tile holding kx = 0 and applies the
 // symmetrize substitution there; the other tiles use MIRROR = false
 // (blockIdx.x offset by tile0) and carry no mirror code.
-template <typename T, int N, int MODE, bool MIRROR>
-__global__ void __launch_bounds__((Cfg<T, N>::NT_BS))
+template <typename T, int N, int MODE, bool MIRROR, int E_ = Cfg<T, N>::E_BS, int B_ = Cfg<T, N>::B_BS>
+__global__ void __launch_bounds__(((N / E_) * B_))
 k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __restrict__ tw, Geom g, int tile0,
            size_t bstride) {
     using CF = Cfg<T, N>;
     // box blockIdx.z of a batch of equal boxes bstride elements apart
     w += blockIdx.z * bstride;
     out += blockIdx.z * bstride;
-    constexpr int B = CF::B_BS, NT = CF::NT_BS, E = CF::E_BS, NXH = CF::NXH;
+    constexpr int B = B_, E = E_, NT = (N / E_) * B_, NXH = CF::NXH;
     constexpr size_t PS = plane_size(N), YS = row_stride(N);
     constexpr int NF = BsOut<MODE>::NF;
     using F = RegFft<T, N, B, B, NT, +1>;
@@ -308,6 +308,37 @@ constexpr PassShapes pass_shapes() {
             s.v[s.n++] = PassShape{e, b};
         }
     }
+    return s;
+}
+
+// Tile shapes of the fused first (Biot-Savart + inverse z) and last
+// (forward y + update) passes, which hold three components in registers:
+// the built-in choice first, then for N >= 512 the shapes with E in
+// {8, 16}, 64..1024 threads, at least 64-byte row segments, at most 128 KB
+// of LDS, and three components of E values that take at most 96 registers
+// of a lane (48 in 1024-thread workgroups, whose lanes get 128 registers):
+// larger shapes spill.
+template <typename T, int N>
+constexpr PassShapes fused_shapes(int e0, int b0) {
+    PassShapes s{};
+    s.n = 0;
+    s.v[s.n++] = PassShape{e0, b0};
+    if (N < 512) return s;
+    constexpr int es[2] = {8, 16};
+    constexpr int bs[4] = {4, 8, 16, 32};
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 4; j++) {
+            const int e = es[i], b = bs[j];
+            const long nt = long(N / e) * b;
+            if (N % e != 0 || (N / 2) % b != 0 || nt < 64 || nt > 1024) continue;
+            if (long(N) * b * long(sizeof(C2<T>)) > 131072 || b * int(sizeof(C2<T>)) < 64) continue;
+            const int regs = 3 * e * int(sizeof(C2<T>)) / 4;  // VGPRs of the three components
+            if (regs > (nt > 512 ? 48 : 96)) continue;  // beyond these the compiler spills
+            bool dup = false;
+            for (int k = 0; k < s.n; k++) dup |= s.v[k].e == e && s.v[k].b == b;
+            if (dup || s.n == kMaxPassShapes) continue;
+            s.v[s.n++] = PassShape{e, b};
+        }
     return s;
 }
 
@@ -691,8 +722,8 @@ k_rows_r2c(const T* data, C2<T>* out, const C2<T>* __restrict__ tw) {
 // data: planes 0..2 hold the output of the z forward pass ([y][kz][kx]).
 // v0 = omega(t); vp = previous stage (stage 2: v1, stage 3: v2); dst receives
 // the new stage (stage 3 writes v0 in place and applies the final projection).
-template <typename T, int N, int STAGE, bool BATCH>
-__global__ void __launch_bounds__((Cfg<T, N>::NT_Y3))
+template <typename T, int N, int STAGE, bool BATCH, int E_ = Cfg<T, N>::E_Y3, int B_ = Cfg<T, N>::B_Y3>
+__global__ void __launch_bounds__(((N / E_) * B_))
 k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __restrict__ vp,
                C2<T>* dst, const C2<T>* __restrict__ tw, Geom g, StepParams sp0, size_t bstride,
                const StepParams* __restrict__ spb) {
@@ -707,7 +738,7 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
         dst += blockIdx.z * bstride;
     }
     const StepParams sp = BATCH ? spb[blockIdx.z] : sp0;
-    constexpr int B = CF::B_Y3, NT = CF::NT_Y3, E = CF::E_Y3, NXH = CF::NXH;
+    constexpr int B = B_, E = E_, NT = (N / E_) * B_, NXH = CF::NXH;
     constexpr size_t PS = plane_size(N), YS = row_stride(N);
     using F = RegFft<T, N, B, B, NT, -1>;
     __shared__ C2<T> lds[N * B];

@@ -227,7 +227,11 @@ k_interp_ab(const T* __restrict__ f, double* __restrict__ pos, double* __restric
 
 // The tracer loops of many boxes (an ensemble) in one launch: box b owns
 // workgroups [block0_b, block0_b + ceil(np_b LP / 256)) and its own field,
-// tracers, history state and geometry.
+// tracers, geometry and time step (TracerBox, rewritten by the host only
+// when one of them changes), and its Adams-Bashforth state (TracerDyn),
+// which lives on the device: the loop reads it and k_tracer_dyn_advance
+// moves it to the next step, so that a captured graph replays the loop
+// step after step with no host-side arguments.
 struct TracerBox {
     const void* f;
     double* pos;
@@ -237,13 +241,33 @@ struct TracerBox {
     int64_t np;
     PartGeomD pg;
     double dt;
-    int nsteps, head, order, add_head, deg;
+    int nsteps, deg;
     uint32_t block0;
 };
+struct TracerDyn {
+    int step_idx;  // the order of the step is min(step_idx, nsteps)
+    int head;      // history slot of the newest right-hand side
+    int add_head;  // 1: add into the head slot (it holds this step's partial sum)
+    int pad;
+};
+static_assert(sizeof(TracerDyn) == 4 * sizeof(int), "TracerDyn is mirrored on the host as 4 ints");
+
+// inc_step_idx + shift_rhs_vectors (particles_system.hpp:191-208) of every
+// box of a group, on the device
+__global__ void k_tracer_dyn_advance(const TracerBox* __restrict__ boxes, TracerDyn* __restrict__ dyn, int nboxes) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nboxes) return;
+    TracerDyn d = dyn[i];
+    const int ns = boxes[i].nsteps;
+    d.step_idx += 1;
+    d.head = (d.head + ns - 1) % ns;
+    d.add_head = 0;
+    dyn[i] = d;
+}
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(256)
-k_interp_ab_multi(const TracerBox* __restrict__ boxes, int nboxes) {
+k_interp_ab_multi(const TracerBox* __restrict__ boxes, const TracerDyn* __restrict__ dyn, int nboxes) {
     constexpr int LP = Lanes<NB>::LP;
     extern __shared__ double stab[];
     // the box of this workgroup: the last one starting at or before it
@@ -262,8 +286,9 @@ k_interp_ab_multi(const TracerBox* __restrict__ boxes, int nboxes) {
     const int jx = threadIdx.x % LP;
     if (t >= bx.np) return;  // whole lane groups exit together
     const PartGeomD pg = bx.pg;
-    interp_ab_one<T, NB>((const T*)bx.f, bx.pos, bx.rhs, bx.perm, bx.np, bx.nsteps, bx.head, bx.order, bx.add_head,
-                         bx.dt, stab, deg, pg, t, jx);
+    const TracerDyn d = dyn[lo];
+    interp_ab_one<T, NB>((const T*)bx.f, bx.pos, bx.rhs, bx.perm, bx.np, bx.nsteps, d.head,
+                         min(d.step_idx, bx.nsteps), d.add_head, bx.dt, stab, deg, pg, t, jx);
 }
 
 template <typename T, int NB, int NC>
@@ -297,6 +322,13 @@ k_sample(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
 // and filled by LDS DMA (global_load_lds: no registers, all of a wave's loads
 // in flight at once): while the groups sum component c from one buffer, the
 // next component -- or the next brick's first -- arrives in the other.
+//
+// Interpolation weights: with STORE_W the batch prologue evaluates the
+// 3 (2n+2) weights of each tracer once (Horner in FP64, one weight per
+// thread) into LDS, and every component's sum reads them from there
+// (broadcast reads); otherwise (when the tiles leave no room) the lanes
+// evaluate them per component.  Either way the polynomial degree is a
+// uniform branch, so no lane evaluates the unused high-order terms.
 #ifndef BFPSH_BRICK_NT
 #define BFPSH_BRICK_NT 512
 #endif
@@ -309,17 +341,25 @@ k_sample(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
 template <typename T, int NB>
 struct BrickCfg {
     static constexpr int W = 2 * NB + 2;
-    static constexpr int KX = sizeof(T) == 4 ? 16 : 8;
-    static constexpr int KY = sizeof(T) == 4 ? 16 : 8;
-    // 16 planes deep for FP32 stencils up to 10 wide (n <= 4): the halo per
-    // brick is amortised over twice the tracers
-    static constexpr int KZ = (sizeof(T) == 4 && W <= 10) ? 16 : 8;
-    static constexpr int WX = KX + W - 1, WY = KY + W - 1, WZ = KZ + W - 1;
     static constexpr int DW = int(sizeof(T) / 4);  // dwords per value
     static constexpr int RY = BFPSH_BRICK_RY;      // y rows per LDS read
     static constexpr int LPT = 16 * RY;            // lanes per tracer
     static constexpr int NI = W / RY;              // y rows per lane
     static_assert(W % RY == 0, "stencil rows split evenly over the lane rows");
+    // FP32: 16 x 8 x 8 bricks (16 planes deep for stencils up to 10 wide),
+    // which leave room for the stored weights of a 64-tracer batch beside
+    // two tiles; FP64: 8 x 8 x 8 bricks, weights per component
+    static constexpr int KX = sizeof(T) == 4 ? 16 : 8;
+    static constexpr int KY = 8;
+    static constexpr int KZ = (sizeof(T) == 4 && W <= 10) ? 16 : 8;
+    static constexpr int WX = KX + W - 1, WY = KY + W - 1, WZ = KZ + W - 1;
+    // the tile's x rows are windows of whole 16-byte groups, aligned in
+    // global memory (XA values each), so that one LDS-DMA lane moves a
+    // group: the window starts XS values before the stencil's first column
+    // and is WXT values wide
+    static constexpr int XA = int(16 / sizeof(T));
+    static constexpr int XS = (XA - NB % XA) % XA;
+    static constexpr int WXT = (WX + XS + XA - 1) / XA * XA;
     // y-plane stride (values): with RY >= 2 the rows y and y+1 of a stencil
     // start 16 dwords apart modulo 32 banks, so the two 16-lane rows of a
     // read touch disjoint banks whatever the tracer's x offset
@@ -329,30 +369,40 @@ struct BrickCfg {
         while ((s * DW) % 32 != 16) s++;
         return s;
     }
-    static constexpr int YS = ys_of(WZ * WX);
-    static constexpr int TILE = WY * YS;                    // values, pads included
-    static constexpr int CHUNKS = (TILE * DW + 63) / 64;    // 64-dword DMA chunks (one per wave load)
-    static constexpr size_t TILE_BYTES = size_t(CHUNKS) * 64 * 4;
+    static constexpr int YS = ys_of(WZ * WXT);
+    static_assert(YS % XA == 0, "y planes start on 16-byte groups");
+    static constexpr int TILE = WY * YS;                           // values, pads included
+    static constexpr int CHUNKS = (TILE + 64 * XA - 1) / (64 * XA);  // 1 KB DMA chunks (one per wave load)
+    static constexpr size_t TILE_BYTES = size_t(CHUNKS) * 1024;
     static constexpr int NT = BFPSH_BRICK_NT;
+    static constexpr int KPW = (CHUNKS + NT / 64 - 1) / (NT / 64);  // chunks per wave
     static constexpr int GROUPS = NT / LPT;
     // tracers per batch: a brick's tracers are normally one batch (512^3,
-    // 2^22 tracers: 64 per 16x16x8 brick on average); every batch refills
+    // 2^22 tracers: 32 per 16x8x8 brick on average); every batch refills
     // the tiles of all components
-    static constexpr int PB = sizeof(T) == 4 ? 128 : 64;
-    static_assert(PB <= NT, "one tracer per thread in the batch prologue");
+    static constexpr int PB = 64;
+    static_assert(PB <= NT && NT % PB == 0, "whole groups of threads per tracer in the batch prologue");
+    // stored weights: per tracer 3 axes x WS doubles, tracers WQ doubles
+    // apart (WQ = 2 mod 16: the two tracers of a wave read other banks)
+    static constexpr int WS = 16;
+    static constexpr int WQ = 3 * WS + 2;
+    // the derivative-0 weight table (W polynomials of degree <= max(2n+1, 5))
+    static constexpr int TABD = W * ((W - 1 > 5 ? W - 1 : 5) + 1);
     template <int NC, int MODE> static constexpr size_t res_doubles() {
         return MODE == 0 ? size_t(PB) * NC : (size_t(PB) + 1) / 2;
     }
     // two tiles, then results [PB][NC] (or slots [PB], uint32) + in-cell
-    // positions [PB][3] (doubles) + cells [PB][3] (ints)
-    template <int NC, int MODE> static constexpr size_t lds_bytes() {
-        return 2 * TILE_BYTES + (res_doubles<NC, MODE>() + size_t(PB) * 3) * 8 + size_t(PB) * 3 * 4;
+    // positions [PB][3] (doubles) + cells [PB][3] (ints) [+ weights [PB][WQ]]
+    template <int NC, int MODE, bool SW> static constexpr size_t lds_bytes() {
+        return 2 * TILE_BYTES + (res_doubles<NC, MODE>() + size_t(PB) * 3 + (SW ? size_t(PB) * WQ + TABD : 0)) * 8 +
+               size_t(PB) * 3 * 4;
     }
     static constexpr size_t LDS_MAX = 160 * 1024;
+    static constexpr bool STORE_W = lds_bytes<3, 0, true>() <= LDS_MAX && lds_bytes<9, 1, true>() <= LDS_MAX;
 #ifdef BFPSH_NO_BRICKS
     static constexpr bool ok = false;
 #else
-    static constexpr bool ok = W <= 16 && lds_bytes<3, 0>() <= LDS_MAX && lds_bytes<9, 1>() <= LDS_MAX;
+    static constexpr bool ok = W <= 16 && lds_bytes<3, 0, STORE_W>() <= LDS_MAX && lds_bytes<9, 1, STORE_W>() <= LDS_MAX;
 #endif
 };
 
@@ -360,21 +410,23 @@ struct BrickCfg {
 // blocks; logical blocks run through BG x BG x BG groups of bricks so that
 // the blocks resident on one XCD at a time share their halos through its L2
 // (the brick id itself stays x-fastest, as the bucket keys use it).
-// Measured at 512^3 FP32 (n6 m2): groups of 2^3 bricks 8.36 ms per tracer
-// step against 8.48 ms for 4^3 and 8.63 ms for 8^3.
+// Measured at 512^3 FP32 (n6 m2, 16x16x8 bricks): groups of 2^3 bricks
+// 8.36 ms per tracer step against 8.48 ms for 4^3 and 8.63 ms for 8^3.
+// Brick counts stay below 2^20 (1024^3 in 8^3 bricks), so the walk uses
+// 32-bit arithmetic.
 #ifndef BFPSH_BRICK_GROUP
 #define BFPSH_BRICK_GROUP 2
 #endif
-__device__ __forceinline__ int64_t brick_of(int64_t logical, int nbx, int nby, int nbz) {
-    constexpr int BG = BFPSH_BRICK_GROUP;
+__device__ __forceinline__ uint32_t brick_of(uint32_t logical, uint32_t nbx, uint32_t nby, uint32_t nbz) {
+    constexpr uint32_t BG = BFPSH_BRICK_GROUP;
     if (BG <= 1 || nbx % BG || nby % BG || nbz % BG) return logical;
-    constexpr int64_t G3 = int64_t(BG) * BG * BG;
-    const int64_t g = logical / G3, w = logical % G3;
-    const int ngx = nbx / BG, ngy = nby / BG;
-    const int bx = int(g % ngx) * BG + int(w % BG);
-    const int by = int((g / ngx) % ngy) * BG + int((w / BG) % BG);
-    const int bz = int(g / (int64_t(ngx) * ngy)) * BG + int(w / (BG * BG));
-    return (int64_t(bz) * nby + by) * nbx + bx;
+    constexpr uint32_t G3 = BG * BG * BG;
+    const uint32_t g = logical / G3, w = logical % G3;
+    const uint32_t ngx = nbx / BG, ngy = nby / BG;
+    const uint32_t bx = (g % ngx) * BG + w % BG;
+    const uint32_t by = ((g / ngx) % ngy) * BG + (w / BG) % BG;
+    const uint32_t bz = (g / (ngx * ngy)) * BG + w / (BG * BG);
+    return (bz * nby + by) * nbx + bx;
 }
 
 __global__ void k_brick_keys(const double* __restrict__ pos, int64_t np, PartGeomD pg, int kx, int ky, int kz,
@@ -406,14 +458,33 @@ BFPSH_DEV void bcast_rows(double (&w)[sizeof...(I)], double v, std::integer_sequ
     ((w[I] = dpp_d<0x150 + I * RY>(v)), ...);
 }
 
-// Polynomial weight of one stencil index at in-cell position xr (Horner,
-// lowest-order coefficient first); the lane's table row is in registers.
-template <int DMAX>
-BFPSH_DEV double weight_of(const double (&row)[DMAX + 1], int deg, double xr) {
-    double a = 0.0;
+// Horner of degree D (compile time) on coefficients c[0..D], lowest first
+template <int D, int DM>
+BFPSH_DEV double horner(const double (&c)[DM + 1], double xr) {
+    double a = c[D];
 #pragma unroll
-    for (int k = DMAX; k >= 0; k--)
-        if (k <= deg) a = a * xr + row[k];
+    for (int k = D - 1; k >= 0; k--) a = fma(a, xr, c[k]);
+    return a;
+}
+
+// Polynomial weight of one stencil index at in-cell position xr; the
+// lane's table row is in registers.  Degrees: 3 (m = 1), 5 (m = 2), 2n+1
+// (m = 0, DM); the branch is uniform (deg is a kernel argument).
+template <int DM>
+BFPSH_DEV double weight_of(const double (&row)[DM + 1], int deg, double xr) {
+    if (deg == 3) return horner<3, DM>(row, xr);
+    if constexpr (DM >= 5) {
+        if (deg == 5) return horner<5, DM>(row, xr);
+    }
+    return horner<DM, DM>(row, xr);
+}
+
+// one weight from the table in global memory (runtime degree; the batch
+// prologue evaluates each weight once)
+BFPSH_DEV double weight_rt(const double* __restrict__ tab, int j, int deg, double xr) {
+    const double* c = tab + j * (deg + 1);
+    double a = c[deg];
+    for (int k = deg - 1; k >= 0; k--) a = fma(a, xr, c[k]);
     return a;
 }
 
@@ -423,9 +494,10 @@ BFPSH_DEV double weight_of(const double (&row)[DMAX + 1], int deg, double xr) {
 // so that one XCD's workgroups hold neighbouring bricks at a time).  Work is
 // a sequence of phases (brick, batch of up to PB tracers, component); the
 // tile of phase i + 1 is requested before phase i is summed.  Per batch a
-// prologue loads each tracer's position once (thread q owns tracer q) and
-// leaves its cell offset in the brick and in-cell fractions in LDS.  All
-// control flow that reaches a barrier is uniform over the workgroup.
+// prologue loads each tracer's position once (thread q owns tracer q),
+// leaves its cell offset in the brick and in-cell fractions in LDS, and
+// (STORE_W) evaluates its weights into LDS.  All control flow that reaches
+// a barrier is uniform over the workgroup.
 // Arithmetic as the reference's (particles_field_computer.hpp:142-150):
 // FP64 weights, each FP32 or FP64 field value converted to double and
 // accumulated in double.
@@ -438,18 +510,23 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
     using BC = BrickCfg<T, NB>;
     constexpr int W = BC::W, NT = BC::NT, PB = BC::PB, LPT = BC::LPT, RY = BC::RY, NI = BC::NI;
     constexpr int GROUPS = BC::GROUPS, WX = BC::WX, WY = BC::WY, WZ = BC::WZ, YS = BC::YS, DW = BC::DW;
-    constexpr int KX = BC::KX, KY = BC::KY, KZ = BC::KZ;
-    constexpr int DMAX = W - 1;  // highest polynomial degree of a (n, m) table
+    constexpr int KX = BC::KX, KY = BC::KY, KZ = BC::KZ, WS = BC::WS, WQ = BC::WQ;
+    constexpr int XA = BC::XA, XS = BC::XS, WXT = BC::WXT, KPW = BC::KPW;
+    constexpr bool SW = BC::STORE_W;
+    // highest polynomial degree of an (n, m) table: 2n+1 (m = 0) or 5 (m = 2)
+    constexpr int DM = W - 1 > 5 ? W - 1 : 5;
     extern __shared__ double smem[];
     unsigned char* tiles = reinterpret_cast<unsigned char*>(smem);  // 2 x TILE_BYTES
     double* res = smem + 2 * BC::TILE_BYTES / 8;                    // MODE 0: [PB][NC]
     uint32_t* lslot = reinterpret_cast<uint32_t*>(res);             // MODE 1: [PB]
     double* frac = res + BC::template res_doubles<NC, MODE>();      // [PB][3]
-    int* lcell = reinterpret_cast<int*>(frac + PB * 3);             // [PB][3]
+    double* wgt = frac + PB * 3;                                    // SW: [PB][WQ]
+    double* tabl = wgt + (SW ? PB * WQ : 0);                        // SW: the weight table
+    int* lcell = reinterpret_cast<int*>(tabl + (SW ? BC::TABD : 0)); // [PB][3]
 
     const int n = pg.n;
     const int nbz = n / KZ;
-    const int64_t nbricks = int64_t(nbx) * nby * nbz;
+    const uint32_t nbricks = uint32_t(nbx) * uint32_t(nby) * uint32_t(nbz);
     const int G = gridDim.x;
     const int rank = (G % 8 == 0) ? int(blockIdx.x % 8) * (G / 8) + int(blockIdx.x / 8) : int(blockIdx.x);
     const int tid = threadIdx.x;
@@ -458,23 +535,25 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
     // offsets within a plane fit 32 bits for n <= 1024)
     const size_t ps = 2 * plane_size(n);
     const uint32_t ys = uint32_t(2 * row_stride(n));
+    const uint32_t nxy = uint32_t(nbx) * uint32_t(nby);
 
-    // this lane's row of the weight table (index jx), in registers
-    double trow[DMAX + 1];
-    {
+    // without stored weights: this lane's row of the weight table (index
+    // jx), in registers
+    double trow[DM + 1];
+    if constexpr (!SW) {
         const int jj = jx < W ? jx : 0;
 #pragma unroll
-        for (int k = 0; k <= DMAX; k++) trow[k] = k <= deg ? tab[jj * (deg + 1) + k] : 0.0;
+        for (int k = 0; k <= DM; k++) trow[k] = k <= deg ? tab[jj * (deg + 1) + k] : 0.0;
     }
 
     // next non-empty brick at or after logical round kr (uniform)
-    int64_t kr = 0, brick = 0;
+    uint32_t kr = 0, brick = 0;
     uint32_t s0 = 0, e0 = 0;
-    auto find = [&](int64_t from, int64_t& kout, int64_t& bout, uint32_t& sout, uint32_t& eout) -> bool {
-        for (int64_t k = from;; k++) {
-            const int64_t L = k * G + rank;
+    auto find = [&](uint32_t from, uint32_t& kout, uint32_t& bout, uint32_t& sout, uint32_t& eout) -> bool {
+        for (uint32_t k = from;; k++) {
+            const uint32_t L = k * uint32_t(G) + uint32_t(rank);
             if (L >= nbricks) return false;
-            const int64_t bb = brick_of(L, nbx, nby, nbz);
+            const uint32_t bb = brick_of(L, uint32_t(nbx), uint32_t(nby), uint32_t(nbz));
             const uint32_t a = bstart[bb], e = bend[bb];
             if (a < e) {
                 kout = k;
@@ -485,110 +564,198 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
             }
         }
     };
-    // request the tile of component c of brick bb into buffer b (LDS DMA,
-    // 64 consecutive dwords per wave load; pad slots get a valid dummy)
-    auto fill = [&](int64_t bb, int c, int b) {
-        const int bz = int(bb / (int64_t(nbx) * nby)), by = int((bb / nbx) % nby), bx = int(bb % nbx);
-        const int x0 = bx * KX - NB, y0 = by * KY - NB, z0 = bz * KZ - NB;
+    // request the tile of component c of brick bb into buffer b: LDS DMA of
+    // one 16-byte group (XA values) per lane, 1 KB per wave load.  Lane ln
+    // of this wave's k-th chunk holds the group at element offset loff[k]
+    // from the window's origin (~0u: a pad slot, loaded from the origin);
+    // bricks whose window crosses the box edge wrap each group instead.
+    const int wv = tid / 64, ln = tid % 64;
+    uint32_t loff[KPW];
+#pragma unroll
+    for (int k = 0; k < KPW; k++) {
+        const int ch = wv + k * (NT / 64);
+        const int v0 = (ch * 64 + ln) * XA;
+        const int ly = v0 / YS, r = v0 % YS, lz = r / WXT, lx = r % WXT;
+        loff[k] = (ch < BC::CHUNKS && ly < WY && lz < WZ) ? uint32_t(ly) * ys + uint32_t(lz * n + lx) : ~0u;
+    }
+    auto fill = [&](uint32_t bb, int c, int b) {
+        const int bz = int(bb / nxy), by = int((bb % nxy) / uint32_t(nbx)), bx = int(bb % uint32_t(nbx));
+        const int x0 = bx * KX - NB - XS, y0 = by * KY - NB, z0 = bz * KZ - NB;
         const T* fc = f + c * ps;
         uint32_t* tdw = reinterpret_cast<uint32_t*>(tiles + size_t(b) * BC::TILE_BYTES);
-        const int wv = tid / 64, ln = tid % 64;
-        for (int ch = wv; ch < BC::CHUNKS; ch += NT / 64) {
-            const int d = ch * 64 + ln;
-            const int e = d / DW, half = d % DW;
-            const int ly = e / YS, r = e % YS, lz = r / WX, lx = r % WX;
-            const unsigned char* src = reinterpret_cast<const unsigned char*>(fc);
-            if (ly < WY && lz < WZ) {
-                int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-                gx = gx < 0 ? gx + n : (gx >= n ? gx - n : gx);
-                gy = gy < 0 ? gy + n : (gy >= n ? gy - n : gy);
-                gz = gz < 0 ? gz + n : (gz >= n ? gz - n : gz);
-                src = reinterpret_cast<const unsigned char*>(fc + (uint32_t(gy) * ys + uint32_t(gz) * uint32_t(n) +
-                                                                   uint32_t(gx))) + 4 * half;
+        const bool inside = x0 >= 0 && x0 + WXT <= n && y0 >= 0 && y0 + WY <= n && z0 >= 0 && z0 + WZ <= n;
+        if (inside) {
+            const T* org = fc + (uint32_t(y0) * ys + uint32_t(z0 * n + x0));
+#pragma unroll
+            for (int k = 0; k < KPW; k++) {
+                const int ch = wv + k * (NT / 64);
+                if (k + 1 < KPW || ch < BC::CHUNKS) {  // uniform per wave
+                    const T* src = org + (loff[k] == ~0u ? 0u : loff[k]);
+                    __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src),
+                                                     (__attribute__((address_space(3))) void*)(tdw + ch * 256), 16,
+                                                     0, 0);
+                }
             }
-            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src),
-                                             (__attribute__((address_space(3))) void*)(tdw + ch * 64), 4, 0, 0);
+        } else {
+            for (int ch = wv; ch < BC::CHUNKS; ch += NT / 64) {
+                const int v0 = (ch * 64 + ln) * XA;
+                const int ly = v0 / YS, r = v0 % YS, lz = r / WXT, lx = r % WXT;
+                const T* src = fc;
+                if (ly < WY && lz < WZ) {
+                    int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;  // gx: a group start, never inside a wrap
+                    gx = gx < 0 ? gx + n : (gx >= n ? gx - n : gx);
+                    gy = gy < 0 ? gy + n : (gy >= n ? gy - n : gy);
+                    gz = gz < 0 ? gz + n : (gz >= n ? gz - n : gz);
+                    src = fc + (uint32_t(gy) * ys + uint32_t(gz * n + gx));
+                }
+                __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src),
+                                                 (__attribute__((address_space(3))) void*)(tdw + ch * 256), 16, 0, 0);
+            }
         }
     };
 
+    // batches: the current one (cb) and the one after it (nb), each a brick
+    // and a range [bs, min(bs + PB, e0)) of its sorted tracers
+    struct Bat {
+        uint32_t kr, brick, s0, e0, bs;
+    };
+    auto advance = [&](const Bat& c, Bat& out) -> bool {
+        out = c;
+        out.bs = c.bs + PB;
+        if (out.bs < c.e0) return true;  // more tracers of the same brick
+        if (!find(c.kr + 1, out.kr, out.brick, out.s0, out.e0)) return false;
+        out.bs = out.s0;
+        return true;
+    };
     if (!find(0, kr, brick, s0, e0)) return;  // uniform
-    uint32_t bs = s0;
+    Bat cb{kr, brick, s0, e0, s0}, nb = cb;
+    bool has_next = advance(cb, nb);
+    // each thread stages one tracer of a batch ahead of time: tracer
+    // tid % PB (every one of the NT / PB threads of a tracer evaluates a
+    // share of its weights); the loads for batch b + 1 are issued during
+    // batch b's component phases, so their latency hides behind its sums
+    uint32_t pre_p = 0;
+    double pre_x[3] = {0.0, 0.0, 0.0};
+    {
+        const int q = tid % PB, c0 = int(min(uint32_t(PB), cb.e0 - cb.bs));
+        if (q < c0) {
+            pre_p = perm[cb.bs + q];
+#pragma unroll
+            for (int d = 0; d < 3; d++) pre_x[d] = pos[size_t(pre_p) * 3 + d];
+        }
+    }
+    if constexpr (SW) {
+        for (int i = tid; i < W * (deg + 1); i += NT) tabl[i] = tab[i];  // visible after the first barrier
+    }
     int comp = 0, buf = 0;
-    fill(brick, 0, 0);
+    fill(cb.brick, 0, 0);
     int64_t myp = -1;
     double myx[3] = {0.0, 0.0, 0.0};
     for (;;) {
-        const int cnt = int(min(uint32_t(PB), e0 - bs));
+        const int cnt = int(min(uint32_t(PB), cb.e0 - cb.bs));
         if (comp == 0) {
-            // batch prologue (the previous batch ended with a barrier)
-            const int bz = int(brick / (int64_t(nbx) * nby)), by = int((brick / nbx) % nby),
-                      bx = int(brick % nbx);
-            const int base[3] = {bx * KX, by * KY, bz * KZ};
+            // batch prologue from the staged positions (the previous batch
+            // ended with a barrier): cell offsets in the brick and in-cell
+            // fractions (get_norm_pos_in_cell / pbc_field_layer), and with
+            // STORE_W this thread's share of the tracer's weights
+            const int base[3] = {int(cb.brick % uint32_t(nbx)) * KX, int((cb.brick % nxy) / uint32_t(nbx)) * KY,
+                                 int(cb.brick / nxy) * KZ};
+            const int q = tid % PB, part = tid / PB;
             myp = -1;
-            if (tid < cnt) {
-                myp = perm[bs + tid];
-                if constexpr (MODE == 1) lslot[tid] = uint32_t(myp);
+            if (q < cnt) {
+                double fr[3];
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    const double sp = pos[myp * 3 + d];
-                    myx[d] = sp;
-                    // get_norm_pos_in_cell / pbc_field_layer
+                    const double sp = pre_x[d];
                     const double rep = floor(sp / pg.width[d]);
                     const double inbox = sp - rep * pg.width[d];
                     const double ci = floor(inbox / pg.h[d]);
-                    frac[tid * 3 + d] = (inbox - ci * pg.h[d]) / pg.h[d];
-                    lcell[tid * 3 + d] = cell_of(sp, pg.h[d], n) - base[d];
+                    fr[d] = (inbox - ci * pg.h[d]) / pg.h[d];
+                }
+                if (part == 0) {
+                    myp = pre_p;
+                    if constexpr (MODE == 1) lslot[q] = pre_p;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        myx[d] = pre_x[d];
+                        frac[q * 3 + d] = fr[d];
+                        lcell[q * 3 + d] = cell_of(pre_x[d], pg.h[d], n) - base[d];
+                    }
+                }
+                if constexpr (SW) {
+                    for (int j = part; j < W; j += NT / PB)
+#pragma unroll
+                        for (int a = 0; a < 3; a++) wgt[q * WQ + a * WS + j] = weight_rt(tabl, j, deg, fr[a]);
                 }
             }
         }
         // the phase after this one
-        int64_t nkr = kr, nbrick = brick;
-        uint32_t ns0 = s0, ne0 = e0, nbs = bs;
+        Bat pbn = cb;
         int ncomp = comp + 1;
         bool more = true;
         if (ncomp == NC) {
             ncomp = 0;
-            nbs = bs + PB;
-            if (nbs >= e0) {
-                more = find(kr + 1, nkr, nbrick, ns0, ne0);
-                nbs = ns0;
+            more = has_next;
+            pbn = nb;
+        }
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA and staging loads have landed
+        __syncthreads();                // the tile is whole; the other buffer is free
+        if (has_next) {
+            // stage the next batch: its slots in this batch's first phase,
+            // its positions in the second (the first phase when NC = 1)
+            const int q = tid % PB, ncnt = int(min(uint32_t(PB), nb.e0 - nb.bs));
+            if (q < ncnt) {
+                if (comp == 0) pre_p = perm[nb.bs + q];
+                if (comp == (NC > 1 ? 1 : 0)) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) pre_x[d] = pos[size_t(pre_p) * 3 + d];
+                }
             }
         }
-        __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA and prologue loads have landed
-        __syncthreads();                // the tile is whole; the other buffer is free
-        if (more) fill(nbrick, ncomp, buf ^ 1);
-
+        if (more) fill(pbn.brick, ncomp, buf ^ 1);
         const T* tile = reinterpret_cast<const T*>(tiles + size_t(buf) * BC::TILE_BYTES);
         for (int q = grp; q < cnt; q += GROUPS) {  // whole lane groups iterate together
-            // FP64 weights: lane jx evaluates index jx of each axis (the y
-            // weight of its row ry: index jx + ry, by a row shift); the
-            // group spreads them by DPP broadcasts
-            double wxo = 0.0, wyo = 0.0, wzo = 0.0;
-            if (jx < W) {
-                wxo = weight_of<DMAX>(trow, deg, frac[q * 3 + 0]);
-                wyo = weight_of<DMAX>(trow, deg, frac[q * 3 + 1]);
-                wzo = weight_of<DMAX>(trow, deg, frac[q * 3 + 2]);
+            // FP64 weights of this lane: x index jx, y indices ry + i RY,
+            // every z index
+            double wxo, wyr[NI], wz[W];
+            if constexpr (SW) {
+                const double* wq = wgt + q * WQ;
+                wxo = jx < W ? wq[jx] : 0.0;
+#pragma unroll
+                for (int i = 0; i < NI; i++) wyr[i] = wq[WS + ry + i * RY];
+#pragma unroll
+                for (int j = 0; j < W; j++) wz[j] = wq[2 * WS + j];
+            } else {
+                // lane jx evaluates index jx of each axis (the y weight of
+                // its row ry: index jx + ry, by a row shift); the group
+                // spreads them by DPP broadcasts
+                double wyo = 0.0, wzo = 0.0;
+                wxo = 0.0;
+                if (jx < W) {
+                    wxo = weight_of<DM>(trow, deg, frac[q * 3 + 0]);
+                    wyo = weight_of<DM>(trow, deg, frac[q * 3 + 1]);
+                    wzo = weight_of<DM>(trow, deg, frac[q * 3 + 2]);
+                }
+                double wyl = wyo;
+                if constexpr (RY >= 2) {
+                    const double s1 = dpp_d<0x101>(wyo);
+                    wyl = ry == 1 ? s1 : wyl;
+                }
+                if constexpr (RY >= 4) {
+                    const double s2 = dpp_d<0x102>(wyo), s3 = dpp_d<0x103>(wyo);
+                    wyl = ry == 2 ? s2 : (ry == 3 ? s3 : wyl);
+                }
+                bcast_rows<RY>(wyr, wyl, std::make_integer_sequence<int, NI>{});
+                bcast_rows<1>(wz, wzo, std::make_integer_sequence<int, W>{});
             }
-            double wyl = wyo;
-            if constexpr (RY >= 2) {
-                const double s1 = dpp_d<0x101>(wyo);
-                wyl = ry == 1 ? s1 : wyl;
-            }
-            if constexpr (RY >= 4) {
-                const double s2 = dpp_d<0x102>(wyo), s3 = dpp_d<0x103>(wyo);
-                wyl = ry == 2 ? s2 : (ry == 3 ? s3 : wyl);
-            }
-            double wyr[NI], wz[W];
-            bcast_rows<RY>(wyr, wyl, std::make_integer_sequence<int, NI>{});
-            bcast_rows<1>(wz, wzo, std::make_integer_sequence<int, W>{});
             const int lc0 = lcell[q * 3 + 0], lc1 = lcell[q * 3 + 1], lc2 = lcell[q * 3 + 2];
             // this lane's column: x = lc0 + jx (lanes past the stencil read
             // column W - 1 with weight 0), rows lc1 + ry + i RY, plane lc2 + jz
-            const T* tcol = tile + (lc0 + (jx < W ? jx : W - 1)) + lc2 * WX + (lc1 + ry) * YS;
+            const T* tcol = tile + (XS + lc0 + (jx < W ? jx : W - 1)) + lc2 * WXT + (lc1 + ry) * YS;
             double acc = 0.0;
 #pragma unroll
             for (int jz = 0; jz < W; jz++) {
-                const T* tz = tcol + jz * WX;
+                const T* tz = tcol + jz * WXT;
                 T t[NI];
 #pragma unroll
                 for (int i = 0; i < NI; i++) t[i] = tz[i * RY * YS];
@@ -648,11 +815,10 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
             }
         }
         if (!more) break;
-        kr = nkr;
-        brick = nbrick;
-        s0 = ns0;
-        e0 = ne0;
-        bs = nbs;
+        if (ncomp == 0) {
+            cb = nb;
+            has_next = advance(cb, nb);
+        }
         comp = ncomp;
         buf ^= 1;
     }
@@ -746,7 +912,7 @@ int launch_brick(bfpsh_ctx* ctx, const void* f, double* d_out, double dt, int or
     const int n = ctx->n;
     const int nbx = n / BC::KX, nby = n / BC::KY;
     const int64_t nb = int64_t(nbx) * nby * (n / BC::KZ);
-    const size_t lds = BC::template lds_bytes<NC, MODE>();
+    const size_t lds = BC::template lds_bytes<NC, MODE, BC::STORE_W>();
     auto kern = k_brick<T, NB, NC, MODE>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     // one workgroup per CU (each takes the LDS of a CU), persistent over
@@ -848,36 +1014,17 @@ int sample_t(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
     return fail(ctx, BFPSH_EINVAL, "bad neighbour count");
 }
 
-// The tracer loops of the boxes of one neighbour count in one launch
-// (boxes on one stream, global-gather kernel); per-box sorts as in loop_nb.
-template <typename T, int NB>
-int loop_multi_nb(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch& pb) {
-    constexpr int LP = Lanes<NB>::LP;
+// The tracer tables of the boxes of one neighbour count: rewrite the
+// device table when a box's field, tracers, geometry or time step changed,
+// and the device AB state when the host's view of a box (after a put, a
+// checkpoint read, or a step taken outside the group) differs from what
+// the device holds; due boxes are re-sorted first.
+template <int LP>
+int prepare_group(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch& pb) {
     bfpsh_ctx* ctx = boxes[0];  // the shared stream; errors are reported on this box
     const size_t nb = boxes.size();
-    const size_t bytes = nb * sizeof(TracerBox);
-    // pinned host staging, two buffers in turn, each reused only once the
-    // copy issued from it two batches ago has completed
-    if (pb.hcap < bytes) {
-        for (int k = 0; k < 2; k++) {
-            if (pb.ev[k]) HIPCHK(hipEventSynchronize(pb.ev[k]));
-            if (pb.h[k]) HIPCHK(hipHostFree(pb.h[k]));
-            HIPCHK(hipHostMalloc(&pb.h[k], bytes));
-        }
-        pb.hcap = bytes;
-    }
-    if (pb.cap < bytes) {
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (pb.d) HIPCHK(hipFree(pb.d));
-        HIPCHK(hipMalloc(&pb.d, bytes));
-        pb.cap = bytes;
-    }
-    for (int k = 0; k < 2; k++)
-        if (!pb.ev[k]) HIPCHK(hipEventCreateWithFlags(&pb.ev[k], hipEventDisableTiming));
-    const int k = pb.k;
-    pb.k ^= 1;
-    HIPCHK(hipEventSynchronize(pb.ev[k]));
-    TracerBox* tb = static_cast<TracerBox*>(pb.h[k]);
+    std::vector<TracerBox> tb(nb);
+    std::vector<int> dyn(4 * nb);
     uint32_t blocks = 0;
     int maxdeg = 0;
     for (size_t i = 0; i < nb; i++) {
@@ -887,6 +1034,7 @@ int loop_multi_nb(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double
             if (r) return fail(ctx, r, c->err);
         }
         TracerBox& b = tb[i];
+        std::memset(&b, 0, sizeof(b));
         b.f = c->ru;
         b.pos = c->d_pos;
         b.rhs = c->d_rhs;
@@ -896,44 +1044,95 @@ int loop_multi_nb(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double
         b.pg = pgeom_d(c);
         b.dt = dts[i];
         b.nsteps = c->nsteps;
-        b.head = c->head;
-        b.order = std::min(c->step_idx, c->nsteps);
-        b.add_head = c->add_head ? 1 : 0;
         b.deg = c->deg;
         b.block0 = blocks;
         blocks += uint32_t((c->np * LP + 255) / 256);
         maxdeg = std::max(maxdeg, c->deg);
+        dyn[4 * i + 0] = c->step_idx;
+        dyn[4 * i + 1] = c->head;
+        dyn[4 * i + 2] = c->add_head ? 1 : 0;
+        dyn[4 * i + 3] = 0;
     }
-    HIPCHK(hipMemcpyAsync(pb.d, tb, bytes, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipEventRecord(pb.ev[k], ctx->stream));
-    const size_t lds = size_t(2 * NB + 2) * (maxdeg + 1) * sizeof(double);
-    LAUNCH("interp_ab_multi", (k_interp_ab_multi<T, NB><<<blocks, kThreads, lds, ctx->stream>>>(
-                                  (const TracerBox*)pb.d, int(nb))));
+    const size_t bytes = nb * sizeof(TracerBox);
+    if (pb.cap < bytes) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (void* p : {pb.d, pb.d_dyn})
+            if (p) HIPCHK(hipFree(p));
+        pb.d = pb.d_dyn = nullptr;
+        HIPCHK(hipMalloc(&pb.d, bytes));
+        HIPCHK(hipMalloc(&pb.d_dyn, nb * sizeof(TracerDyn)));
+        pb.cap = bytes;
+        pb.tab.clear();
+        pb.dyn.clear();
+    }
+    const unsigned char* tbb = reinterpret_cast<const unsigned char*>(tb.data());
+    if (pb.tab.size() != bytes || std::memcmp(pb.tab.data(), tbb, bytes) != 0) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(pb.d, tb.data(), bytes, hipMemcpyHostToDevice));
+        pb.tab.assign(tbb, tbb + bytes);
+    }
+    if (pb.dyn != dyn) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(pb.d_dyn, dyn.data(), dyn.size() * sizeof(int), hipMemcpyHostToDevice));
+        pb.dyn = dyn;
+    }
+    pb.nboxes = int(nb);
+    pb.blocks = blocks;
+    pb.maxdeg = maxdeg;
     return BFPSH_OK;
 }
 
-template <typename T>
-int loop_multi_t(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch& pb, int nbr) {
-    switch (nbr) {
-    case 1: return loop_multi_nb<T, 1>(boxes, dts, pb);
-    case 2: return loop_multi_nb<T, 2>(boxes, dts, pb);
-    case 3: return loop_multi_nb<T, 3>(boxes, dts, pb);
-    case 4: return loop_multi_nb<T, 4>(boxes, dts, pb);
-    case 5: return loop_multi_nb<T, 5>(boxes, dts, pb);
-    case 6: return loop_multi_nb<T, 6>(boxes, dts, pb);
-    case 7: return loop_multi_nb<T, 7>(boxes, dts, pb);
-    case 8: return loop_multi_nb<T, 8>(boxes, dts, pb);
-    case 9: return loop_multi_nb<T, 9>(boxes, dts, pb);
-    case 10: return loop_multi_nb<T, 10>(boxes, dts, pb);
+template <typename T, int NB>
+int prepare_nb(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch& pb) {
+    int r = prepare_group<Lanes<NB>::LP>(boxes, dts, pb);
+    if (r) return r;
+    pb.lds = size_t(2 * NB + 2) * (pb.maxdeg + 1) * sizeof(double);
+    return BFPSH_OK;
+}
+
+template <typename T, int NB>
+int launch_nb(bfpsh_ctx* ctx, PartBatch& pb) {
+    const TracerBox* tb = static_cast<const TracerBox*>(pb.d);
+    TracerDyn* dyn = static_cast<TracerDyn*>(pb.d_dyn);
+    LAUNCH("interp_ab_multi", (k_interp_ab_multi<T, NB><<<pb.blocks, kThreads, pb.lds, ctx->stream>>>(
+                                  tb, dyn, pb.nboxes)));
+    LAUNCH("tracer_advance", (k_tracer_dyn_advance<<<(pb.nboxes + 255) / 256, 256, 0, ctx->stream>>>(
+                                 tb, dyn, pb.nboxes)));
+    return BFPSH_OK;
+}
+
+#define BFPSH_NB_SWITCH(CALL)                                  \
+    switch (nbr) {                                             \
+    case 1: return CALL(1); case 2: return CALL(2);            \
+    case 3: return CALL(3); case 4: return CALL(4);            \
+    case 5: return CALL(5); case 6: return CALL(6);            \
+    case 7: return CALL(7); case 8: return CALL(8);            \
+    case 9: return CALL(9); case 10: return CALL(10);          \
     }
+
+template <typename T>
+int prepare_t(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch& pb, int nbr) {
+#define BFPSH_P(K) prepare_nb<T, K>(boxes, dts, pb)
+    BFPSH_NB_SWITCH(BFPSH_P)
+#undef BFPSH_P
     return fail(boxes[0], BFPSH_EINVAL, "bad neighbour count");
 }
 
+template <typename T>
+int launch_t(bfpsh_ctx* ctx, PartBatch& pb, int nbr) {
+#define BFPSH_L(K) launch_nb<T, K>(ctx, pb)
+    BFPSH_NB_SWITCH(BFPSH_L)
+#undef BFPSH_L
+    return fail(ctx, BFPSH_EINVAL, "bad neighbour count");
+}
+#undef BFPSH_NB_SWITCH
+
 }  // namespace
 
-int bfpsh_part_loop_batch(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch* pb,
-                          int nbatch) {
-    // one launch per neighbour count (nbatch tables, one per group)
+int bfpsh_part_ens_prepare(const std::vector<bfpsh_ctx*>& boxes, const std::vector<double>& dts, PartBatch* pb,
+                           int nbatch, int* ngroups) {
+    // one group (one table, one launch) per neighbour count, in order of
+    // first appearance
     std::vector<int> counts;
     for (bfpsh_ctx* c : boxes)
         if (std::find(counts.begin(), counts.end(), c->pn) == counts.end()) counts.push_back(c->pn);
@@ -946,10 +1145,29 @@ int bfpsh_part_loop_batch(const std::vector<bfpsh_ctx*>& boxes, const std::vecto
                 grp.push_back(boxes[i]);
                 gdt.push_back(dts[i]);
             }
-        int r = grp[0]->prec == BFPSH_FP32 ? loop_multi_t<float>(grp, gdt, pb[gi], counts[gi])
-                                           : loop_multi_t<double>(grp, gdt, pb[gi], counts[gi]);
+        pb[gi].nbr = counts[gi];
+        pb[gi].prec = grp[0]->prec;
+        pb[gi].ctx = grp[0];
+        int r = grp[0]->prec == BFPSH_FP32 ? prepare_t<float>(grp, gdt, pb[gi], counts[gi])
+                                           : prepare_t<double>(grp, gdt, pb[gi], counts[gi]);
         if (r) return r;
     }
+    *ngroups = int(counts.size());
+    return BFPSH_OK;
+}
+
+int bfpsh_part_ens_launch(PartBatch* pb, int ngroups) {
+    for (int gi = 0; gi < ngroups; gi++) {
+        bfpsh_ctx* ctx = pb[gi].ctx;
+        int r = pb[gi].prec == BFPSH_FP32 ? launch_t<float>(ctx, pb[gi], pb[gi].nbr)
+                                          : launch_t<double>(ctx, pb[gi], pb[gi].nbr);
+        if (r) return r;
+    }
+    return BFPSH_OK;
+}
+
+void bfpsh_part_ens_advanced(const std::vector<bfpsh_ctx*>& boxes, PartBatch* pb, int ngroups) {
+    // the host's view of every box follows the device's k_tracer_dyn_advance
     for (bfpsh_ctx* c : boxes) {
         c->sort_age += 1;
         c->step_idx += 1;
@@ -957,22 +1175,25 @@ int bfpsh_part_loop_batch(const std::vector<bfpsh_ctx*>& boxes, const std::vecto
         c->head_zero = true;
         c->add_head = false;
     }
-    return BFPSH_OK;
+    for (int gi = 0; gi < ngroups; gi++) {
+        PartBatch& g = pb[gi];
+        const TracerBox* tb = reinterpret_cast<const TracerBox*>(g.tab.data());
+        for (int i = 0; i < g.nboxes && 4 * i + 3 < int(g.dyn.size()); i++) {
+            const int ns = tb[i].nsteps;
+            g.dyn[4 * i + 0] += 1;
+            g.dyn[4 * i + 1] = (g.dyn[4 * i + 1] + ns - 1) % ns;
+            g.dyn[4 * i + 2] = 0;
+        }
+    }
 }
 
 void bfpsh_part_batch_free(PartBatch& pb) {
-    for (int k = 0; k < 2; k++) {
-        if (pb.ev[k]) {
-            (void)hipEventSynchronize(pb.ev[k]);
-            (void)hipEventDestroy(pb.ev[k]);
-        }
-        if (pb.h[k]) (void)hipHostFree(pb.h[k]);
-        pb.ev[k] = nullptr;
-        pb.h[k] = nullptr;
-    }
-    if (pb.d) (void)hipFree(pb.d);
-    pb.d = nullptr;
-    pb.cap = pb.hcap = 0;
+    for (void* p : {pb.d, pb.d_dyn})
+        if (p) (void)hipFree(p);
+    pb.d = pb.d_dyn = nullptr;
+    pb.cap = 0;
+    pb.tab.clear();
+    pb.dyn.clear();
 }
 
 int bfpsh_part_release(bfpsh_ctx* ctx) {

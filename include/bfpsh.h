@@ -73,8 +73,12 @@ int bfpsh_create(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky
  * one scratch plane for MEASURE, 10 for PATIENT) and keep the fastest in
  * the wisdom.  Boxes below 256^3 have one shape. */
 enum { BFPSH_ESTIMATE = 0, BFPSH_MEASURE = 1, BFPSH_PATIENT = 2 };
-/* strided pass kinds: inverse and forward transforms along kz (z) and ky (y) */
-enum { BFPSH_PASS_Z_INV = 0, BFPSH_PASS_Y_INV = 1, BFPSH_PASS_Z_FWD = 2, BFPSH_PASS_Y_FWD = 3 };
+/* strided pass kinds: inverse and forward transforms along kz (z) and ky
+ * (y), the vorticity step's fused first pass (Biot-Savart + inverse z) and
+ * its fused last pass (forward y + dealias, curl, forcing, projection and
+ * the Runge-Kutta update) */
+enum { BFPSH_PASS_Z_INV = 0, BFPSH_PASS_Y_INV = 1, BFPSH_PASS_Z_FWD = 2, BFPSH_PASS_Y_FWD = 3,
+       BFPSH_PASS_BS = 4, BFPSH_PASS_Y3 = 5, BFPSH_PASS_KINDS = 6 };
 
 /* Replaces ~vorticity_equation (bfps/cpp/vorticity_equation.cpp:158-169). */
 int bfpsh_destroy(bfpsh_ctx* ctx);
@@ -255,8 +259,10 @@ int bfpsh_device_bytes(bfpsh_ctx* ctx, size_t* bytes);
 /* Replaces the plan creation with `plan_rigor` in the field constructor
  * (bfps/cpp/field.cpp:105-118): chooses the tile shape of each strided pass
  * kind of this context by `rigor` (BFPSH_ESTIMATE / MEASURE / PATIENT, see
- * above).  Measuring allocates one scratch plane for the time it runs;
- * the context's fields are left untouched. */
+ * above).  Measuring allocates one scratch plane for the time it runs and,
+ * as FFTW_MEASURE overwrites the arrays it plans for, overwrites the
+ * context's work planes (the spectral velocity of compute_velocity); the
+ * vorticity and the Runge-Kutta stages are left untouched. */
 int bfpsh_set_plan_rigor(bfpsh_ctx* ctx, int rigor);
 /* The tile shape in use for pass `kind` (BFPSH_PASS_*): e complex values per
  * thread, b columns per tile; nshapes = shapes compiled for this box size.

@@ -265,3 +265,40 @@ def test_embedded_low_mode_step(B, emb_ref, n, precision):
         assert e_blk < 1e-11 and rest < 1e-13 * scale
     else:
         assert e_blk < 1e-6 and rest < 1e-6 * scale
+
+
+# ------------------------------------------------------------ fused pass shapes
+@pytest.mark.parametrize("n,precision", [(512, "single"), (512, "double"), (1024, "single")])
+def test_every_fused_shape_matches_default(B, n, precision):
+    """Every compiled tile shape of the step's fused first pass (Biot-Savart
+    + inverse z) and last pass (forward y + update) gives the built-in
+    shapes' step within rounding (FP64 1e-13, FP32 1e-6 relative L2), on the
+    low-mode nonlinear field of the embedding test."""
+    from bfps_amd import ic
+    fs = B.vorticity_equation("fs", n, precision=precision, plan_rigor="FFTW_ESTIMATE")
+    sh = fs.pass_shapes()
+    nbs, ny3 = sh["bs"][3], sh["y3"][3]
+    say("case: fused shapes n=%d %s: %d first-pass, %d last-pass shapes" % (n, precision, nbs, ny3))
+    fs.nu, fs.famplitude, fs.fk0, fs.fk1, fs.forcing_type = EMB_NU, 0.5, 2.0, 4.0, "linear"
+    w0 = ic.solenoidal_vorticity(n, seed=77, k0=2.0, kcut=EMB_K, dtype=fs.cdtype)
+    tol = 1e-13 if precision == "double" else 1e-6
+    iy = _low_index(n, 8 * EMB_K)
+    ref = None
+    for v in range(max(nbs, ny3)):
+        fs.set_pass_shape("bs", min(v, nbs - 1))
+        fs.set_pass_shape("y3", min(v, ny3 - 1))
+        fs.io_checkpoint(read=True, data=w0)
+        fs.step(EMB_DT)
+        got = fs.cvorticity
+        blk = got[np.ix_(iy, iy, np.arange(8 * EMB_K + 1))].astype(np.complex128)
+        rest = float(np.abs(got).sum(dtype=np.float64))
+        del got
+        if ref is None:
+            ref, rest0 = blk, rest
+            say("  shapes %s / %s: reference" % (fs.pass_shapes()["bs"][:2], fs.pass_shapes()["y3"][:2]))
+            continue
+        e = rel(blk, ref)
+        say("  shapes %s / %s: block rel %.3e, sum |w| %.6e vs %.6e"
+            % (fs.pass_shapes()["bs"][:2], fs.pass_shapes()["y3"][:2], e, rest, rest0))
+        assert e < tol and abs(rest - rest0) <= 1e3 * tol * rest0
+    fs.close()

@@ -200,9 +200,11 @@ def test_cospectrum(B, oracle):
     assert np.allclose(fs.kk.kshell, ks) and np.array_equal(fs.kk.nshell, ns)
 
 
-# n <= 6 run the LDS-brick kernel; n >= 7 (a tile beyond 80 KB) the
-# global-gather kernel
-@pytest.mark.parametrize("nb,m", [(1, 0), (1, 1), (2, 2), (4, 1), (6, 2), (3, 0), (5, 1), (7, 1), (8, 2), (10, 1)])
+# FP64: n <= 6 run the LDS-brick kernel (n <= 4 with the weights stored per
+# batch, 5 and 6 evaluated per component), n >= 7 the global-gather kernel;
+# n = 1, m = 2 has polynomials of degree 5 over a 4-point stencil
+@pytest.mark.parametrize("nb,m", [(1, 0), (1, 1), (1, 2), (2, 2), (4, 1), (6, 2), (3, 0), (5, 1), (7, 1), (8, 2),
+                                  (10, 1)])
 def test_tracers_vs_oracle(B, oracle, nb, m):
     n = 32
     say("case: tracers n=%d m=%d" % (nb, m))
@@ -272,11 +274,12 @@ def test_errors_are_reported(B):
         B.particles_system(fs, 10, 11, 1)
 
 
-@pytest.mark.parametrize("n,nb,m", [(64, 6, 2), (64, 4, 1), (48, 1, 1), (64, 5, 2), (48, 7, 1), (32, 10, 2)])
+@pytest.mark.parametrize("n,nb,m", [(64, 6, 2), (64, 4, 1), (48, 1, 1), (48, 1, 2), (64, 5, 2), (48, 7, 1),
+                                    (32, 10, 2)])
 def test_tracers_fp32_bricks(B, oracle, n, nb, m):
     """FP32 fields, many tracers per brick and several bricks per axis (n <=
-    6; n >= 7 the global-gather kernel); the interpolated velocities /
-    gradients are FP64 sums of FP32 field values."""
+    7, weights stored per batch; n >= 8 the global-gather kernel); the
+    interpolated velocities / gradients are FP64 sums of FP32 field values."""
     from bfps_amd import ic
     say("case: fp32 tracers n=%d nb=%d m=%d" % (n, nb, m))
     w0 = ic.solenoidal_vorticity(n, seed=21, k0=4.0)
