@@ -54,6 +54,7 @@ SIGNATURES = {
     "bfpsh_field_get": (_I, [_P, _I, _I, _P]),
     "bfpsh_field_copy_from": (_I, [_P, _I, _I]),
     "bfpsh_field_cospectrum": (_I, [_P, _I, _I, _DP]),
+    "bfpsh_field_cospectrum_noalloc": (_I, [_P, _I, _I, _DP]),
     "bfpsh_set_plan_rigor": (_I, [_P, _I]),
     "bfpsh_get_pass_shape": (_I, [_P, _I, _IP, _IP, _IP, _IP]),
     "bfpsh_set_pass_shape": (_I, [_P, _I, _I]),

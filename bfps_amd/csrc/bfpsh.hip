@@ -96,6 +96,35 @@ static void planar_to_host_real(const T* p, T* h, int n, int nc) {
 
 #define PREC_CALL(fn, ...) (ops_for(ctx)->fn(__VA_ARGS__))
 
+// ------------------------------------------------------------ fixed scratch
+// Shell sums of the allocation-free cospectrum (bfpsh_field_cospectrum_noalloc)
+// live in this array, not in a per-call allocation.  Each GPU the library
+// runs on holds its own copy until the process ends; host code reaches a
+// copy only through the address hipGetSymbolAddress returns for its device
+// (resolved once per device when the first context is created there).
+// Every call on a GPU shares that GPU's copy: calls that overlap on one GPU
+// race.
+constexpr int kScratchMaxShells = 1024;
+static __device__ __align__(256) double scratch[kScratchMaxShells * 9];
+constexpr int kScratchMaxDevices = 64;
+static std::mutex scratch_mu;
+static double* scratch_dev[kScratchMaxDevices];  // per device: its copy of `scratch`
+
+// resolve this device's copy (the device is current); 0 on success
+static int scratch_resolve(bfpsh_ctx* ctx, int device) {
+    if (device < 0 || device >= kScratchMaxDevices) return fail(ctx, BFPSH_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (scratch_dev[device]) return BFPSH_OK;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(ctx, BFPSH_EHIP, "no device address for the cospectrum scratch");
+    scratch_dev[device] = static_cast<double*>(p);
+    return BFPSH_OK;
+}
+
+static __global__ void k_fill_zero_d(double* __restrict__ p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
+}
 
 extern "C" {
 
@@ -131,6 +160,10 @@ int create_ctx(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky, 
     ctx->dkz = dkz;
     ctx->es = precision == BFPSH_FP32 ? 4 : 8;
     HIPCHK(hipSetDevice(device));
+    {
+        int r = scratch_resolve(ctx, device);
+        if (r) return r;
+    }
     if (stream) {
         ctx->stream = stream;
         ctx->own_stream = false;
@@ -515,6 +548,36 @@ int bfpsh_field_cospectrum(bfpsh_ctx* ctx, int id_a, int id_b, double* out) {
     return PREC_CALL(cospectrum, ctx, ctx->fields[id_a], ctx->fields[id_b], out);
 }
 
+// The same shell sums with no allocation: partial sums in this device's
+// copy of `scratch` (cleared by a fill kernel first), copied to `out`.
+int bfpsh_field_cospectrum_noalloc(bfpsh_ctx* ctx, int id_a, int id_b, double* out) {
+    CTX_CHECK();
+    if (!out || !field_ok(ctx, id_a) || !field_ok(ctx, id_b) || ctx->field_nc[id_a] != 3 || ctx->field_nc[id_b] != 3)
+        return fail(ctx, BFPSH_EINVAL, "needs two 3-component fields");
+    if (ctx->nshells > kScratchMaxShells)
+        return fail(ctx, BFPSH_EINVAL, "cospectrum of " + std::to_string(ctx->nshells) + " shells exceeds the " +
+                                           std::to_string(kScratchMaxShells) + "-shell fixed scratch");
+    if (ctx->device < 0 || ctx->device >= kScratchMaxDevices || !scratch_dev[ctx->device])
+        return fail(ctx, BFPSH_ESTATE, "cospectrum scratch not resolved on this device");
+    double* d = scratch_dev[ctx->device];
+    const int nv = ctx->nshells * 9;
+    const size_t lds = size_t(nv) * sizeof(double);
+    if (lds > 64 * 1024) return fail(ctx, BFPSH_EINVAL, "too many shells for the cospectrum kernel");
+    k_fill_zero_d<<<(nv + 255) / 256, 256, 0, ctx->stream>>>(d, nv);
+    HIPCHK(hipGetLastError());
+    if (ctx->prec == BFPSH_FP32)
+        k_cospectrum<float><<<1024, 256, lds, ctx->stream>>>((const float2*)ctx->fields[id_a],
+                                                             (const float2*)ctx->fields[id_b], d, geom(ctx), ctx->dk,
+                                                             ctx->nshells);
+    else
+        k_cospectrum<double><<<1024, 256, lds, ctx->stream>>>((const double2*)ctx->fields[id_a],
+                                                              (const double2*)ctx->fields[id_b], d, geom(ctx), ctx->dk,
+                                                              ctx->nshells);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, d, lds, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BFPSH_OK;
+}
 
 int bfpsh_field_normalize(bfpsh_ctx* ctx, int id) {
     CTX_CHECK();

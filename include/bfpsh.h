@@ -164,6 +164,15 @@ int bfpsh_field_copy_from(bfpsh_ctx* ctx, int id, int which);
 /* kspace::cospectrum (bfps/cpp/kspace.cpp:521-598) of two 3-component
  * standalone fields (spectral); out nshells*9 doubles. */
 int bfpsh_field_cospectrum(bfpsh_ctx* ctx, int id_a, int id_b, double* out);
+/* No counterpart in the reference: bfpsh_field_cospectrum without any
+ * allocation, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its shell sums use a fixed device array of at most 1024 shells (73.7 KB,
+ * one per GPU, held until the process ends) instead of a per-call buffer;
+ * more shells are rejected (BFPSH_EINVAL).  Same arguments and output as
+ * bfpsh_field_cospectrum.  Calls that overlap on one GPU share the array
+ * and race. */
+int bfpsh_field_cospectrum_noalloc(bfpsh_ctx* ctx, int id_a, int id_b, double* out);
 /* Replaces field::normalize (bfps/cpp/field.cpp:891-898): every value /=
  * nx*ny*nz, in the field precision. */
 int bfpsh_field_normalize(bfpsh_ctx* ctx, int id);
