@@ -197,13 +197,17 @@ def roofline(prof, n, es, np_, nb, traffic_ok, steps):
            "frac": ach / HBM_PEAK, "traffic": traffic, "avg_ms": ms / cnt, "bytes_per_launch": bytes_per_launch,
            "bytes_note": note, "launches_per_step": cnt / steps}
     if name == "interp_ab":
-        # the tracer kernel is bound by LDS bandwidth: every tracer reads its
-        # (2n+2)^3 stencil of each component from the brick tile in LDS, and
-        # each brick's tile (with its 2n+1 halo) is written once per component
+        # LDS traffic of the tracer kernel: every tracer reads its (2n+2)^3
+        # stencil of each component from the brick tile in LDS, and each
+        # brick's tile (its halo of 2n+1 cells, x rows widened to aligned
+        # 16-byte groups) is written once per component
         w = 2 * nb + 2
-        kx = 16 if es == 4 else 8
-        tile = (kx + w - 1) * (kx + w - 1) * (8 + w - 1)
-        nbricks = n ** 3 // (kx * kx * 8)
+        kx, ky, kz = (16, 8, 16 if w <= 10 else 8) if es == 4 else (8, 8, 8)
+        xa = 16 // es
+        xs = (xa - nb % xa) % xa
+        wxt = (kx + w - 1 + xs + xa - 1) // xa * xa
+        tile = wxt * (ky + w - 1) * (kz + w - 1)
+        nbricks = n ** 3 // (kx * ky * kz)
         lds_bytes = np_ * 3 * w ** 3 * es + nbricks * 3 * tile * es
         out["lds"] = {"bytes_per_launch": lds_bytes, "achieved": lds_bytes / avg_s / 1e9, "peak": LDS_PEAK / 1e9,
                       "unit": "GB/s", "frac": lds_bytes / avg_s / LDS_PEAK}

@@ -134,14 +134,16 @@ class vorticity_equation:
         capi.check(self._h, capi.load().bfpsh_set_plan_rigor(self._h, capi.PLAN_RIGOR[rigor]), "set_plan_rigor")
 
     def pass_shapes(self):
-        """{kind: (values per thread, columns, shape index, shapes compiled)}
-        of the strided passes (kinds z_inv, y_inv, z_fwd, y_fwd)."""
+        """{kind: (values per thread, columns, shape index, shapes compiled,
+        persistent)} of the strided passes (kinds z_inv, y_inv, z_fwd,
+        y_fwd, and the step's fused bs and y3)."""
         out = {}
         for name, k in capi.PASS_KINDS.items():
-            v = [ctypes.c_int() for _ in range(4)]
-            capi.check(self._h, capi.load().bfpsh_get_pass_shape(self._h, k, *[ctypes.byref(x) for x in v]),
+            e, b, pp, idx, n = [ctypes.c_int() for _ in range(5)]
+            capi.check(self._h, capi.load().bfpsh_get_pass_shape(self._h, k, ctypes.byref(e), ctypes.byref(b),
+                                                                 ctypes.byref(pp), ctypes.byref(idx), ctypes.byref(n)),
                        "get_pass_shape")
-            out[name] = tuple(x.value for x in v)
+            out[name] = (e.value, b.value, idx.value, n.value, pp.value)
         return out
 
     def set_pass_shape(self, kind, index):

@@ -56,7 +56,7 @@ SIGNATURES = {
     "bfpsh_field_cospectrum": (_I, [_P, _I, _I, _DP]),
     "bfpsh_field_cospectrum_noalloc": (_I, [_P, _I, _I, _DP]),
     "bfpsh_set_plan_rigor": (_I, [_P, _I]),
-    "bfpsh_get_pass_shape": (_I, [_P, _I, _IP, _IP, _IP, _IP]),
+    "bfpsh_get_pass_shape": (_I, [_P, _I, _IP, _IP, _IP, _IP, _IP]),
     "bfpsh_set_pass_shape": (_I, [_P, _I, _I]),
     "bfpsh_wisdom_import": (_I, [ctypes.c_char_p]),
     "bfpsh_wisdom_export": (_I, [ctypes.c_char_p]),

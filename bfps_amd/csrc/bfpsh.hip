@@ -886,13 +886,14 @@ int bfpsh_set_plan_rigor(bfpsh_ctx* ctx, int rigor) {
     return PREC_CALL(tune_passes, ctx, rigor);
 }
 
-int bfpsh_get_pass_shape(bfpsh_ctx* ctx, int kind, int* e, int* b, int* index, int* nshapes) {
+int bfpsh_get_pass_shape(bfpsh_ctx* ctx, int kind, int* e, int* b, int* persistent, int* index, int* nshapes) {
     CTX_CHECK();
     if (kind < 0 || kind >= BFPSH_PASS_KINDS) return fail(ctx, BFPSH_EINVAL, "bad pass kind");
     const PassShapes sh = ops_for(ctx)->pass_shapes(kind);
     const int v = ctx->pass_var[kind] >= 0 && ctx->pass_var[kind] < sh.n ? ctx->pass_var[kind] : 0;
     if (e) *e = sh.v[v].e;
     if (b) *b = sh.v[v].b;
+    if (persistent) *persistent = sh.v[v].pp;
     if (index) *index = v;
     if (nshapes) *nshapes = sh.n;
     return BFPSH_OK;
@@ -918,8 +919,8 @@ int bfpsh_wisdom_import(const char* path) {
         if (line[0] == '#' || line[0] == '\n') continue;
         char key[256];
         PassWisdom w;
-        if (std::sscanf(line, "%255s %d %d %lf %d", key, &w.e, &w.b, &w.ms, &w.rigor) != 5 || w.e <= 0 || w.b <= 0 ||
-            w.rigor < BFPSH_ESTIMATE || w.rigor > BFPSH_PATIENT) {
+        if (std::sscanf(line, "%255s %d %d %d %lf %d", key, &w.e, &w.b, &w.pp, &w.ms, &w.rigor) != 6 || w.e <= 0 ||
+            w.b <= 0 || (w.pp != 0 && w.pp != 1) || w.rigor < BFPSH_ESTIMATE || w.rigor > BFPSH_PATIENT) {
             rc = BFPSH_EINVAL;
             break;
         }
@@ -943,9 +944,10 @@ int bfpsh_wisdom_export(const char* path) {
     FILE* f = std::fopen(tmp.c_str(), "w");
     if (!f) return BFPSH_EINVAL;
     std::fprintf(f, "# bfps_amd pass wisdom: <arch>/<precision>/<N>/<pass kind> <values per thread> <columns> "
-                    "<ms per plane> <rigor>\n");
+                    "<persistent> <ms> <rigor>\n");
     for (auto& kv : snap)
-        std::fprintf(f, "%s %d %d %.6f %d\n", kv.first.c_str(), kv.second.e, kv.second.b, kv.second.ms, kv.second.rigor);
+        std::fprintf(f, "%s %d %d %d %.6f %d\n", kv.first.c_str(), kv.second.e, kv.second.b, kv.second.pp, kv.second.ms,
+                     kv.second.rigor);
     if (std::fclose(f) != 0 || std::rename(tmp.c_str(), path) != 0) return BFPSH_EINVAL;
     return BFPSH_OK;
 }

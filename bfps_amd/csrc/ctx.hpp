@@ -231,7 +231,7 @@ struct OpsTable {
 // plane and the rigor of the search that chose it.  Keys are
 // "arch/f32|f64/N/kind".
 struct PassWisdom {
-    int e = 0, b = 0;
+    int e = 0, b = 0, pp = 0;
     double ms = 0.0;
     int rigor = BFPSH_MEASURE;  // the plan rigor that chose it
 };

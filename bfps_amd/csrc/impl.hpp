@@ -37,8 +37,24 @@ struct Impl {
     static void pass_launch(hipStream_t st, CT* d, size_t es, size_t os, const CT* tw, int planes, size_t bs,
                             int nbox) {
         constexpr PassShape p = SH.v[I < SH.n ? I : 0];
-        k_pass<T, N, SIGN, p.e, p.b><<<dim3(NXH / p.b, N, planes * nbox), (N / p.e) * p.b, 0, st>>>(
-            d, es, os, tw, planes, bs);
+        constexpr int nt = (N / p.e) * p.b;
+        if constexpr (p.pp) {
+            // as many workgroups as are resident at once (the occupancy of
+            // this kernel on this device, looked up once)
+            static int grid = 0;
+            if (grid == 0) {
+                int dev = 0, ncu = 0, occ = 0;
+                (void)hipGetDevice(&dev);
+                (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+                (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_pass_pp<T, N, SIGN, p.e, p.b>, nt, 0);
+                grid = std::max(1, ncu) * std::max(1, occ);
+            }
+            const int ntx = NXH / p.b, nty = N, ntiles = ntx * nty * planes * nbox;
+            k_pass_pp<T, N, SIGN, p.e, p.b><<<std::min(grid, ntiles), nt, 0, st>>>(d, es, os, tw, planes, bs, ntx,
+                                                                                   nty, ntiles);
+        } else {
+            k_pass<T, N, SIGN, p.e, p.b><<<dim3(NXH / p.b, N, planes * nbox), nt, 0, st>>>(d, es, os, tw, planes, bs);
+        }
     }
     template <int SIGN, size_t... I>
     static constexpr std::array<PassFn, kMaxPassShapes> pass_table(std::index_sequence<I...>) {
@@ -140,7 +156,7 @@ struct Impl {
             PassWisdom w;
             if (pass_wisdom_lookup(key, &w) && w.rigor >= rigor) {
                 for (int v = 0; v < sh.n; v++)
-                    if (sh.v[v].e == w.e && sh.v[v].b == w.b) ctx->pass_var[kind] = v;
+                    if (sh.v[v].e == w.e && sh.v[v].b == w.b && sh.v[v].pp == w.pp) ctx->pass_var[kind] = v;
                 continue;
             }
             if (rigor == BFPSH_ESTIMATE) continue;
@@ -196,6 +212,7 @@ struct Impl {
                 ctx->pass_var[kind] = bestv;
                 w.e = sh.v[bestv].e;
                 w.b = sh.v[bestv].b;
+                w.pp = sh.v[bestv].pp;
                 w.ms = best;
                 w.rigor = rigor;
                 pass_wisdom_store(key, w);

@@ -271,16 +271,70 @@ k_pass(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __
     for (int e = 0; e < E; e++) base[F::out_row(grp, e) * estride] = v[e];
 }
 
-// Tile shapes (E values per thread, B columns) of the plain strided pass
-// for one (precision, N): the built-in choice (Cfg) first, then, for
-// N >= 256 (below that a box is cache-resident and launch-bound), every
-// shape with E in {8, 12, 16, 24, 32} dividing N with a radix plan, B
-// dividing N/2, 64..1024 threads, at least 32-byte row segments and at most
-// 128 KB of LDS -- up to kMaxPassShapes in all.
+// Persistent form of k_pass: gridDim.x workgroups (as many as fit on the
+// chip at once) each walk tiles t = blockIdx.x + k gridDim.x of the
+// ntx x nty x (planes x boxes) tile grid, and load the columns of their
+// next tile while the current one is transformed and stored, so that
+// global-memory latency hides behind the FFT (the barriers of the FFT
+// order LDS only and leave those loads in flight).
+template <typename T, int N, int SIGN, int E, int B>
+__global__ void __launch_bounds__((N / E) * B)
+k_pass_pp(C2<T>* __restrict__ data, size_t estride, size_t ostride, const C2<T>* __restrict__ tw, int planes,
+          size_t bstride, int ntx, int nty, int ntiles) {
+    constexpr int NT = (N / E) * B;
+    using F = RegFft<T, N, B, B, NT, SIGN>;
+    static_assert(F::E == E, "tile shape");
+    constexpr size_t PS = plane_size(N);
+    __shared__ C2<T> lds[N * B];
+    const int tid = threadIdx.x;
+    const int b = tid % B, grp = tid / B;
+    auto base_of = [&](int t) {
+        const int x = t % ntx, r = t / ntx, y = r % nty, zp = r / nty;
+        const int box = zp / planes, plane = zp % planes;
+        return data + box * bstride + plane * PS + size_t(y) * ostride + size_t(x) * B + b;
+    };
+    int t = blockIdx.x;
+    if (t >= ntiles) return;
+    C2<T> cur[E], nxt[E];
+    {
+        const C2<T>* p = base_of(t);
+#pragma unroll
+        for (int e = 0; e < E; e++) cur[e] = p[F::in_row(grp, e) * estride];
+    }
+    for (;;) {
+        const int tn = t + int(gridDim.x);
+        const bool more = tn < ntiles;  // uniform
+        if (more) {
+            const C2<T>* p = base_of(tn);
+#pragma unroll
+            for (int e = 0; e < E; e++) nxt[e] = p[F::in_row(grp, e) * estride];
+        }
+        F::run(cur, lds, tw, grp, b);
+        {
+            C2<T>* p = base_of(t);
+#pragma unroll
+            for (int e = 0; e < E; e++) p[F::out_row(grp, e) * estride] = cur[e];
+        }
+        if (!more) break;
+#pragma unroll
+        for (int e = 0; e < E; e++) cur[e] = nxt[e];
+        t = tn;
+    }
+}
+
+// Tile shapes (E values per thread, B columns, pp: the persistent
+// prefetching form) of the plain strided pass for one (precision, N): the
+// built-in choice (Cfg) first, then, for N >= 256 (below that a box is
+// cache-resident and launch-bound), every shape with E in {8, 12, 16, 24,
+// 32} dividing N with a radix plan, B dividing N/2, 64..1024 threads, at
+// least 32-byte row segments and at most 128 KB of LDS, and the persistent
+// form of each whose two register tiles (4 E registers of FP32, 8 E of
+// FP64) take at most 2/5 of a lane's registers (beyond that it spills) --
+// up to kMaxPassShapes in all.
 struct PassShape {
-    int e, b;
+    int e, b, pp;
 };
-constexpr int kMaxPassShapes = 8;
+constexpr int kMaxPassShapes = 16;
 struct PassShapes {
     int n;
     PassShape v[kMaxPassShapes];
@@ -290,24 +344,28 @@ constexpr PassShapes pass_shapes() {
     using CF = Cfg<T, N>;
     PassShapes s{};
     s.n = 0;
-    s.v[s.n++] = PassShape{CF::E_PASS, CF::B};
+    s.v[s.n++] = PassShape{CF::E_PASS, CF::B, 0};
     if (N < 256) return s;
     constexpr int es[5] = {8, 12, 16, 24, 32};
     constexpr int bs[6] = {2, 4, 8, 16, 32, 64};
-    for (int i = 0; i < 5; i++) {
-        const int e = es[i];
-        if (N % e != 0 || make_plan_e(N, e).ns <= 0) continue;
-        for (int j = 0; j < 6; j++) {
-            const int b = bs[j];
-            const long nt = long(N / e) * b;
-            if ((N / 2) % b != 0 || nt < 64 || nt > 1024) continue;
-            if (long(N) * b * long(sizeof(C2<T>)) > 131072 || b * int(sizeof(C2<T>)) < 32) continue;
-            bool dup = false;
-            for (int k = 0; k < s.n; k++) dup |= s.v[k].e == e && s.v[k].b == b;
-            if (dup || s.n == kMaxPassShapes) continue;
-            s.v[s.n++] = PassShape{e, b};
+    for (int pp = 0; pp < 2; pp++)
+        for (int i = 0; i < 5; i++) {
+            const int e = es[i];
+            if (N % e != 0 || make_plan_e(N, e).ns <= 0) continue;
+            for (int j = 0; j < 6; j++) {
+                const int b = bs[j];
+                const long nt = long(N / e) * b;
+                if ((N / 2) % b != 0 || nt < 64 || nt > 1024) continue;
+                // persistent form: its two register tiles at most 2/5 of the
+                // registers a lane gets (256, or 128 in 1024-thread workgroups)
+                if (pp && 5 * (2 * e * int(sizeof(C2<T>)) / 4) > 2 * (nt > 512 ? 128 : 256)) continue;
+                if (long(N) * b * long(sizeof(C2<T>)) > 131072 || b * int(sizeof(C2<T>)) < 32) continue;
+                bool dup = false;
+                for (int k = 0; k < s.n; k++) dup |= s.v[k].e == e && s.v[k].b == b && s.v[k].pp == pp;
+                if (dup || s.n == kMaxPassShapes) continue;
+                s.v[s.n++] = PassShape{e, b, pp};
+            }
         }
-    }
     return s;
 }
 
@@ -322,7 +380,7 @@ template <typename T, int N>
 constexpr PassShapes fused_shapes(int e0, int b0) {
     PassShapes s{};
     s.n = 0;
-    s.v[s.n++] = PassShape{e0, b0};
+    s.v[s.n++] = PassShape{e0, b0, 0};
     if (N < 512) return s;
     constexpr int es[2] = {8, 16};
     constexpr int bs[4] = {4, 8, 16, 32};
@@ -337,7 +395,7 @@ constexpr PassShapes fused_shapes(int e0, int b0) {
             bool dup = false;
             for (int k = 0; k < s.n; k++) dup |= s.v[k].e == e && s.v[k].b == b;
             if (dup || s.n == kMaxPassShapes) continue;
-            s.v[s.n++] = PassShape{e, b};
+            s.v[s.n++] = PassShape{e, b, 0};
         }
     return s;
 }

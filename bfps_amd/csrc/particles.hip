@@ -406,18 +406,32 @@ struct BrickCfg {
 #endif
 };
 
-// Visiting order of the bricks: each XCD walks a contiguous range of logical
-// blocks; logical blocks run through BG x BG x BG groups of bricks so that
-// the blocks resident on one XCD at a time share their halos through its L2
-// (the brick id itself stays x-fastest, as the bucket keys use it).
-// Measured at 512^3 FP32 (n6 m2, 16x16x8 bricks): groups of 2^3 bricks
-// 8.36 ms per tracer step against 8.48 ms for 4^3 and 8.63 ms for 8^3.
+// Visiting order of the bricks.  Workgroup g takes logical bricks
+// k G + rank(g), k = 0, 1, ..., with rank XCD-aware (blocks are dealt
+// round-robin to the 8 XCDs, so the G/8 ranks of one XCD are consecutive).
+// With G = 256 (one workgroup per CU of an MI355X) the 32 bricks one XCD
+// takes in round k form a superblock of 4 x 4 x 2 bricks, and successive
+// rounds of an XCD walk a column of superblocks along z: the halos the 32
+// workgroups of an XCD share at a time (and the planes shared with the
+// next superblock of the column) are read from HBM once and then from
+// that XCD's L2.  Other grids fall back to groups of 2^3 bricks.
 // Brick counts stay below 2^20 (1024^3 in 8^3 bricks), so the walk uses
 // 32-bit arithmetic.
 #ifndef BFPSH_BRICK_GROUP
 #define BFPSH_BRICK_GROUP 2
 #endif
-__device__ __forceinline__ uint32_t brick_of(uint32_t logical, uint32_t nbx, uint32_t nby, uint32_t nbz) {
+__device__ __forceinline__ uint32_t brick_of(uint32_t logical, uint32_t nbx, uint32_t nby, uint32_t nbz,
+                                             uint32_t G) {
+    constexpr uint32_t SX = 4, SY = 4, SZ = 2, SB = SX * SY * SZ;
+    if (G == 8 * SB && nbx % SX == 0 && nby % SY == 0 && nbz % SZ == 0 && ((nbx / SX) * (nby / SY)) % 8 == 0) {
+        const uint32_t nsx = nbx / SX, nsz = nbz / SZ;
+        const uint32_t k = logical / G, x = (logical % G) / SB, j = logical % SB;  // round, XCD, slot
+        const uint32_t col = (k / nsz) * 8 + x, lev = k % nsz;
+        const uint32_t bx = (col % nsx) * SX + j % SX;
+        const uint32_t by = (col / nsx) * SY + (j / SX) % SY;
+        const uint32_t bz = lev * SZ + j / (SX * SY);
+        return (bz * nby + by) * nbx + bx;
+    }
     constexpr uint32_t BG = BFPSH_BRICK_GROUP;
     if (BG <= 1 || nbx % BG || nby % BG || nbz % BG) return logical;
     constexpr uint32_t G3 = BG * BG * BG;
@@ -553,7 +567,7 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
         for (uint32_t k = from;; k++) {
             const uint32_t L = k * uint32_t(G) + uint32_t(rank);
             if (L >= nbricks) return false;
-            const uint32_t bb = brick_of(L, uint32_t(nbx), uint32_t(nby), uint32_t(nbz));
+            const uint32_t bb = brick_of(L, uint32_t(nbx), uint32_t(nby), uint32_t(nbz), uint32_t(G));
             const uint32_t a = bstart[bb], e = bend[bb];
             if (a < e) {
                 kout = k;

@@ -274,16 +274,18 @@ int bfpsh_device_bytes(bfpsh_ctx* ctx, size_t* bytes);
  * vorticity and the Runge-Kutta stages are left untouched. */
 int bfpsh_set_plan_rigor(bfpsh_ctx* ctx, int rigor);
 /* The tile shape in use for pass `kind` (BFPSH_PASS_*): e complex values per
- * thread, b columns per tile; nshapes = shapes compiled for this box size.
- * No reference counterpart (FFTW plans are opaque). */
-int bfpsh_get_pass_shape(bfpsh_ctx* ctx, int kind, int* e, int* b, int* index, int* nshapes);
+ * thread, b columns per tile, persistent = 1 for the form whose workgroups
+ * stay resident and load their next tile while transforming the current
+ * one; nshapes = shapes compiled for this box size.  No reference
+ * counterpart (FFTW plans are opaque). */
+int bfpsh_get_pass_shape(bfpsh_ctx* ctx, int kind, int* e, int* b, int* persistent, int* index, int* nshapes);
 /* Forces shape `index` (0 <= index < nshapes; 0 is the built-in one) for
  * pass `kind`; for tests of every shape.  No reference counterpart. */
 int bfpsh_set_pass_shape(bfpsh_ctx* ctx, int kind, int index);
 /* Replace fftwf_import_wisdom_from_filename / fftwf_export_wisdom_to_filename
  * (bfps/cpp/full_code/main_code.hpp:101-104,145-149) and fftw_forget_wisdom:
  * the process-wide pass wisdom as a text file, one line per (architecture,
- * precision, N, pass kind): "<arch>/<f32|f64>/<N>/<kind> <e> <b> <ms> <rigor>".
+ * precision, N, pass kind): "<arch>/<f32|f64>/<N>/<kind> <e> <b> <persistent> <ms> <rigor>".
  * Importing a missing file is not an error (as the reference's start of a
  * new simulation); lines that do not parse are rejected. */
 int bfpsh_wisdom_import(const char* path);

@@ -231,14 +231,14 @@ def test_every_pass_shape_vs_oracle_256(B, oracle, precision):
     tol = 1e-13 if precision == "double" else 1e-6
     for idx in range(nsh):
         _set_all(fs, idx)
-        e, b = fs.pass_shapes()["y_inv"][:2]
+        e, b, _, _, pp = fs.pass_shapes()["y_inv"]
         f.put(r, real_space=True)
         f.dft()
         c = f.get()
         e_f = rel(c, ref)
         f.ift()
         e_b = rel(f.get(), back)
-        say("  shape %d (E=%d, B=%d): dft rel %.3e, ift(dft) rel %.3e" % (idx, e, b, e_f, e_b))
+        say("  shape %d (E=%d, B=%d, persistent %d): dft rel %.3e, ift(dft) rel %.3e" % (idx, e, b, pp, e_f, e_b))
         assert e_f < tol and e_b < tol
     f.close()
     fs.close()
@@ -260,7 +260,7 @@ def test_every_pass_shape_matches_default(B, n, precision):
     c0 = r0 = None
     for idx in range(nsh):
         _set_all(fs, idx)
-        e, b = fs.pass_shapes()["y_inv"][:2]
+        e, b, _, _, pp = fs.pass_shapes()["y_inv"]
         f.put(r, real_space=True)
         f.dft()
         c = f.get()
@@ -268,10 +268,10 @@ def test_every_pass_shape_matches_default(B, n, precision):
         rr = f.get()
         if idx == 0:
             c0, r0 = c, rr
-            say("  shape 0 (E=%d, B=%d): reference" % (e, b))
+            say("  shape 0 (E=%d, B=%d, persistent %d): reference" % (e, b, pp))
             continue
         e_f, e_b = rel(c, c0), rel(rr, r0)
-        say("  shape %d (E=%d, B=%d): dft rel %.3e, ift rel %.3e" % (idx, e, b, e_f, e_b))
+        say("  shape %d (E=%d, B=%d, persistent %d): dft rel %.3e, ift rel %.3e" % (idx, e, b, pp, e_f, e_b))
         assert e_f < tol and e_b < tol
         del c, rr
     f.close()
@@ -286,8 +286,8 @@ def test_plan_rigor_and_wisdom(B, tmp_path):
     fs = B.vorticity_equation("t", 256, precision="single", plan_rigor="FFTW_MEASURE")
     got = fs.pass_shapes()
     say("case: plan rigor at 256^3: %s" % got)
-    for k, (e, b, idx, nsh) in got.items():
-        assert 0 <= idx < nsh and e > 0 and b > 0
+    for k, (e, b, idx, nsh, pp) in got.items():
+        assert 0 <= idx < nsh and e > 0 and b > 0 and pp in (0, 1)
     path = str(tmp_path / "w.txt")
     B.wisdom_export(path)
     lines = [l for l in open(path) if not l.startswith("#")]

@@ -16,9 +16,9 @@ def B():
     bfps_amd.wisdom_forget()
 
 
-LINES = ["gfx950/f32/1024/y_inv 16 16 2.460000 1",
-         "gfx950/f32/1024/z_fwd 32 8 1.950000 2",
-         "gfx950/f64/512/z_inv 8 4 0.812500 0"]
+LINES = ["gfx950/f32/1024/y_inv 16 16 0 2.460000 1",
+         "gfx950/f32/1024/z_fwd 32 8 1 1.950000 2",
+         "gfx950/f64/512/z_inv 8 4 0 0.812500 0"]
 
 
 def read_entries(path):
@@ -34,10 +34,10 @@ def test_round_trip(B, tmp_path):
     assert read_entries(out) == sorted(LINES)
     # a later import replaces entries of the same key and keeps the others
     upd = tmp_path / "upd.txt"
-    upd.write_text("gfx950/f32/1024/y_inv 8 8 2.000000 2\n")
+    upd.write_text("gfx950/f32/1024/y_inv 8 8 1 2.000000 2\n")
     B.wisdom_import(str(upd))
     B.wisdom_export(str(out))
-    want = sorted(["gfx950/f32/1024/y_inv 8 8 2.000000 2"] + LINES[1:])
+    want = sorted(["gfx950/f32/1024/y_inv 8 8 1 2.000000 2"] + LINES[1:])
     assert read_entries(out) == want
 
 
@@ -53,9 +53,10 @@ def test_forget_and_missing_file(B, tmp_path):
     B.wisdom_import(str(tmp_path / "none.txt"))
 
 
-@pytest.mark.parametrize("bad", ["gfx950/f32/1024/y_inv 16 16\n",          # too few fields
-                                 "gfx950/f32/1024/y_inv 0 16 1.0 1\n",     # no shape
-                                 "gfx950/f32/1024/y_inv 16 16 1.0 7\n"])   # unknown rigor
+@pytest.mark.parametrize("bad", ["gfx950/f32/1024/y_inv 16 16 0 1.0\n",     # too few fields
+                                 "gfx950/f32/1024/y_inv 0 16 0 1.0 1\n",    # no shape
+                                 "gfx950/f32/1024/y_inv 16 16 2 1.0 1\n",   # persistent neither 0 nor 1
+                                 "gfx950/f32/1024/y_inv 16 16 0 1.0 7\n"])  # unknown rigor
 def test_bad_lines_rejected(B, tmp_path, bad):
     src = tmp_path / "bad.txt"
     src.write_text(LINES[1] + "\n" + bad)
