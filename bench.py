@@ -70,13 +70,24 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_512_fp32.txt")
 
 
 def merge_profile(prof):
-    """Fold the kx-tile-0 launch of the Biot-Savart pass into "inv_z_bs"."""
+    """Fold the kx-tile-0 launch of the Biot-Savart pass into "inv_z_bs";
+    the tracer velocity loop is "interp_ab" whichever kernel form ran (the
+    column walk is profiled as "interp_ab_col")."""
     out = dict(prof)
     if "inv_z_bs_t0" in out and "inv_z_bs" in out:
         t0_ms, _ = out.pop("inv_z_bs_t0")
         ms, cnt = out["inv_z_bs"]
         out["inv_z_bs"] = (ms + t0_ms, cnt)
+    if "interp_ab_col" in out:
+        ms, cnt = out.pop("interp_ab_col")
+        a, b = out.get("interp_ab", (0.0, 0))
+        out["interp_ab"] = (ms + a, cnt + b)
     return out
+
+
+def tracer_form(prof):
+    """Kernel form of the tracer velocity loop in a raw profile."""
+    return "column walk" if "interp_ab_col" in prof else ("bricks" if "interp_ab" in prof else None)
 
 
 def measured_traffic(kernel):
@@ -141,7 +152,9 @@ def run_precision(B, ic, wl, precision, steps, warmup, gradient=True):
     if gradient and ps is not None:
         stats = stats_leg(B, fs, ps, n, 4 if precision == "single" else 8)
     res = {"ms_per_step": ms / steps, "wall_ms_per_step": 1e3 * wall / steps, "profile": prof,
-           "grad_sample_s": grad_s, "device_bytes": fs.device_bytes(), "stats": stats}
+           "grad_sample_s": grad_s, "device_bytes": fs.device_bytes(), "stats": stats,
+           "pass_shapes": {k: {"values_per_thread": v[0], "columns": v[1], "persistent": bool(v[4])}
+                           for k, v in fs.pass_shapes().items()}}
     fs.close()
     return res
 
@@ -178,6 +191,7 @@ def roofline(prof, n, es, np_, nb, traffic_ok, steps):
     """Roofline of the kernel with the largest share of the step: achieved =
     algorithmic bytes per launch / average launch duration (HIP events on
     the context stream)."""
+    form = tracer_form(prof)
     prof = merge_profile(prof)
     name, (ms, cnt) = max(prof.items(), key=lambda kv: kv[1][0])
     avg_s = ms / cnt / 1e3
@@ -198,17 +212,24 @@ def roofline(prof, n, es, np_, nb, traffic_ok, steps):
            "bytes_note": note, "launches_per_step": cnt / steps}
     if name == "interp_ab":
         # LDS traffic of the tracer kernel: every tracer reads its (2n+2)^3
-        # stencil of each component from the brick tile in LDS, and each
-        # brick's tile (its halo of 2n+1 cells, x rows widened to aligned
-        # 16-byte groups) is written once per component
+        # stencil of each component from LDS; the windows are written once
+        # per component: column walk, each y plane of a 32 x 16 (x, z)
+        # column with its halo (x rows widened to aligned 16-byte groups)
+        # once per walk; bricks, each brick's tile with its halo of 2n+1
+        # cells on all three axes
         w = 2 * nb + 2
-        kx, ky, kz = (16, 8, 16 if w <= 10 else 8) if es == 4 else (8, 8, 8)
         xa = 16 // es
         xs = (xa - nb % xa) % xa
-        wxt = (kx + w - 1 + xs + xa - 1) // xa * xa
-        tile = wxt * (ky + w - 1) * (kz + w - 1)
-        nbricks = n ** 3 // (kx * ky * kz)
-        lds_bytes = np_ * 3 * w ** 3 * es + nbricks * 3 * tile * es
+        if form == "column walk":
+            kx, kz = 32, 16
+            wxt = (kx + w - 1 + xs + xa - 1) // xa * xa
+            windows = 3 * (n // kx) * (n // kz) * (n + w - 1) * wxt * (kz + w - 1)
+        else:
+            kx, ky, kz = (16, 8, 16 if w <= 10 else 8) if es == 4 else (8, 8, 8)
+            wxt = (kx + w - 1 + xs + xa - 1) // xa * xa
+            windows = 3 * (n ** 3 // (kx * ky * kz)) * wxt * (ky + w - 1) * (kz + w - 1)
+        lds_bytes = np_ * 3 * w ** 3 * es + windows * es
+        out["form"] = form
         out["lds"] = {"bytes_per_launch": lds_bytes, "achieved": lds_bytes / avg_s / 1e9, "peak": LDS_PEAK / 1e9,
                       "unit": "GB/s", "frac": lds_bytes / avg_s / LDS_PEAK}
     return out
@@ -539,6 +560,7 @@ def main():
         "kernel_ms": {k: v[0] / v[1] for k, v in merge_profile(r32["profile"]).items()},
         "wall_ms_per_step": r32["wall_ms_per_step"],
         "device_gb": r32["device_bytes"] / 1e9,
+        "pass_shapes": r32["pass_shapes"],
     }
     kp = merge_profile(r32["profile"])
     if np_ > 0 and "interp_ab" in kp:
@@ -546,7 +568,7 @@ def main():
         # field) and stencil-point x component products per second
         t_ia = kp["interp_ab"][0] / kp["interp_ab"][1] / 1e3
         w = 2 * wl["nb"] + 2
-        out["tracer_kernel"] = {"ms": t_ia * 1e3, "interps_per_s": np_ / t_ia,
+        out["tracer_kernel"] = {"form": tracer_form(r32["profile"]), "ms": t_ia * 1e3, "interps_per_s": np_ / t_ia,
                                 "stencil_point_components_per_s": np_ * w ** 3 * 3 / t_ia}
     if not args.no_fp64:
         s64 = max(2, args.steps // 2)

@@ -420,6 +420,10 @@ struct BrickCfg {
 #ifndef BFPSH_BRICK_GROUP
 #define BFPSH_BRICK_GROUP 2
 #endif
+// planes of LDS reads in flight ahead of the stencil sums
+#ifndef BFPSH_BRICK_PREFETCH
+#define BFPSH_BRICK_PREFETCH 2
+#endif
 __device__ __forceinline__ uint32_t brick_of(uint32_t logical, uint32_t nbx, uint32_t nby, uint32_t nbz,
                                              uint32_t G) {
     constexpr uint32_t SX = 4, SY = 4, SZ = 2, SB = SX * SY * SZ;
@@ -491,6 +495,29 @@ BFPSH_DEV double weight_of(const double (&row)[DM + 1], int deg, double xr) {
         if (deg == 5) return horner<5, DM>(row, xr);
     }
     return horner<DM, DM>(row, xr);
+}
+
+// Adams-Bashforth move of tracer p from its position x by the new right-hand
+// side r0 (already in the head slot of the history ring) and the order - 1
+// older ones (particles_adams_bashforth.hpp:18-133)
+BFPSH_DEV void ab_move(double* __restrict__ pos, const double* __restrict__ rhs, int64_t p, const double (&r0)[3],
+                       const double (&x)[3], int64_t np, int nsteps, int head, int order, double dt) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double r[6];
+        r[0] = r0[d];
+        for (int i = 1; i < order; i++) r[i] = rhs[size_t((head + i) % nsteps) * np * 3 + p * 3 + d];
+        double incr;
+        switch (order) {
+        case 1: incr = dt * r[0]; break;
+        case 2: incr = dt * (3. * r[0] - r[1]) / 2.; break;
+        case 3: incr = dt * (23. * r[0] - 16. * r[1] + 5. * r[2]) / 12.; break;
+        case 4: incr = dt * (55. * r[0] - 59. * r[1] + 37. * r[2] - 9. * r[3]) / 24.; break;
+        case 5: incr = dt * (1901. * r[0] - 2774. * r[1] + 2616. * r[2] - 1274. * r[3] + 251. * r[4]) / 720.; break;
+        default: incr = dt * (4277. * r[0] - 7923. * r[1] + 9982. * r[2] - 7298. * r[3] + 2877. * r[4] - 475. * r[5]) / 1440.; break;
+        }
+        pos[p * 3 + d] = x[d] + incr;
+    }
 }
 
 // one weight from the table in global memory (runtime degree; the batch
@@ -767,12 +794,23 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
             // column W - 1 with weight 0), rows lc1 + ry + i RY, plane lc2 + jz
             const T* tcol = tile + (XS + lc0 + (jx < W ? jx : W - 1)) + lc2 * WXT + (lc1 + ry) * YS;
             double acc = 0.0;
+            // software pipeline: the LDS reads of plane jz + PD are issued
+            // before plane jz's conversions and FMAs (two waves per SIMD
+            // cannot hide the LDS latency by themselves); the loop is fully
+            // unrolled, so tb lives in registers, PD + 1 planes at a time
+            constexpr int PD = BFPSH_BRICK_PREFETCH;
+            T tb[W][NI];
+#pragma unroll
+            for (int jz = 0; jz < PD && jz < W; jz++)
+#pragma unroll
+                for (int i = 0; i < NI; i++) tb[jz][i] = tcol[jz * WXT + i * RY * YS];
 #pragma unroll
             for (int jz = 0; jz < W; jz++) {
-                const T* tz = tcol + jz * WXT;
-                T t[NI];
+                if (jz + PD < W) {
 #pragma unroll
-                for (int i = 0; i < NI; i++) t[i] = tz[i * RY * YS];
+                    for (int i = 0; i < NI; i++) tb[jz + PD][i] = tcol[(jz + PD) * WXT + i * RY * YS];
+                }
+                const T* t = tb[jz];
                 double c0 = 0.0, c1 = 0.0;
 #pragma unroll
                 for (int i = 0; i < NI; i += 2) {
@@ -809,22 +847,7 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
                     h0[0] = r0[0];
                     h0[1] = r0[1];
                     h0[2] = r0[2];
-#pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        double r[6];
-                        r[0] = r0[d];
-                        for (int i = 1; i < order; i++) r[i] = rhs[size_t((head + i) % nsteps) * np * 3 + p * 3 + d];
-                        double incr;
-                        switch (order) {
-                        case 1: incr = dt * r[0]; break;
-                        case 2: incr = dt * (3. * r[0] - r[1]) / 2.; break;
-                        case 3: incr = dt * (23. * r[0] - 16. * r[1] + 5. * r[2]) / 12.; break;
-                        case 4: incr = dt * (55. * r[0] - 59. * r[1] + 37. * r[2] - 9. * r[3]) / 24.; break;
-                        case 5: incr = dt * (1901. * r[0] - 2774. * r[1] + 2616. * r[2] - 1274. * r[3] + 251. * r[4]) / 720.; break;
-                        default: incr = dt * (4277. * r[0] - 7923. * r[1] + 9982. * r[2] - 7298. * r[3] + 2877. * r[4] - 475. * r[5]) / 1440.; break;
-                        }
-                        pos[p * 3 + d] = myx[d] + incr;
-                    }
+                    ab_move(pos, rhs, p, r0, myx, np, nsteps, head, order, dt);
                 }
             }
         }
@@ -836,6 +859,382 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
         comp = ncomp;
         buf ^= 1;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Column walk: the tracer kernel for boxes of 64^3 and more.  A workgroup
+// owns a column of KX x KZ cells (x, z) and walks it along y in steps of KY
+// cells, one velocity component per walk.  The y planes of the stencil
+// window (KY + W - 1 of them) sit in an LDS ring of R = 2 KY + W - 1 slots,
+// so every plane of the column is moved from HBM once per walk while the
+// KY planes of the next step arrive by LDS DMA during the current step's
+// sums.  Compared with independent bricks (whose tiles carry the W - 1 halo
+// planes of all three axes) this removes the y halo from the traffic:
+// 512^3, n = 6: 5.6 KB per plane slot, (n + 13) planes per walk and column,
+// 4.5 GB per launch against 22 GB for 16 x 8 x 8 bricks.
+// Columns are split into y segments when there are fewer than two columns
+// per CU; consecutive workgroups of an XCD take z-neighbour columns, whose
+// planes overlap in z (the shared rows are read from that XCD's L2).
+// Each slot is a z x x window [WZ][ZS] of the plane, ZS chosen so that the
+// two 16-lane row phases of a tracer (NI rows apart) read disjoint banks.
+#ifndef BFPSH_COL_NT
+#define BFPSH_COL_NT 1024
+#endif
+template <typename T, int NB>
+struct ColCfg {
+    static constexpr int W = 2 * NB + 2;
+    static constexpr int DW = int(sizeof(T) / 4);
+    static constexpr int RZ = 2;         // 16-lane row phases per tracer
+    static constexpr int LPT = 16 * RZ;  // lanes per tracer
+    static constexpr int NI = W / RZ;    // z rows per lane: phase rz takes rows rz NI .. rz NI + NI - 1
+    static constexpr int KX = 32, KZ = 16;
+    static constexpr int WX = KX + W - 1, WZ = KZ + W - 1;
+    static constexpr int XA = int(16 / sizeof(T));
+    static constexpr int XS = (XA - NB % XA) % XA;
+    static constexpr int WXT = (WX + XS + XA - 1) / XA * XA;
+    static constexpr int zs_of(int v) {
+        for (int s = v; s < v + 64 * XA; s += XA)
+            if ((NI * s * DW) % 32 == 16) return s;
+        return v;
+    }
+    static constexpr int ZS = zs_of(WXT);
+    static constexpr int SLOT = WZ * ZS;                       // values per slot, pads included
+    static constexpr int CH = (SLOT + 64 * XA - 1) / (64 * XA);  // 1 KB DMA chunks per slot
+    static constexpr int SLOT_BYTES = CH * 1024;
+    // 1024 threads: one workgroup holds a CU's LDS, and 16 waves (4 per
+    // SIMD) hide the LDS latency of the sums better than 8
+    static constexpr int NT = BFPSH_COL_NT, GROUPS = NT / LPT, PB = 64, WS = 16, WQ = 3 * WS + 2;
+    static constexpr int TABD = W * ((W - 1 > 5 ? W - 1 : 5) + 1);
+    // y steps per work item (n / KY for an unsplit column)
+    static constexpr int MAXS = 256;
+    // weights [PB][WQ], table, results [PB], in-cell positions [PB][3]
+    // (doubles), cells [PB][3] (ints), step ranges [2][MAXS] (uint32)
+    static constexpr size_t EXTRA = (size_t(PB) * WQ + TABD + PB + size_t(PB) * 3) * 8 + size_t(PB) * 3 * 4 +
+                                    size_t(2) * MAXS * 4;
+    static constexpr size_t LDS_MAX = 160 * 1024;
+    static constexpr size_t lds_of(int ky) { return size_t(2 * ky + W - 1) * SLOT_BYTES + EXTRA; }
+    static constexpr int KY = lds_of(8) <= LDS_MAX ? 8 : (lds_of(4) <= LDS_MAX ? 4 : 0);
+    static constexpr int R = 2 * KY + W - 1;
+    static constexpr size_t LDS = lds_of(KY);
+#ifdef BFPSH_NO_COLUMNS
+    static constexpr bool ok = false;
+#else
+    static constexpr bool ok = sizeof(T) == 4 && KY >= 4 && W <= 16 && CH <= NT / 64 && W % RZ == 0;
+#endif
+};
+
+// sort key of a tracer: (column, y step), columns z-fastest
+__global__ void k_col_keys(const double* __restrict__ pos, int64_t np, PartGeomD pg, int kx, int ky, int kz,
+                           int ncz, int nys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int cx = cell_of(pos[p * 3 + 0], pg.h[0], pg.n) / kx;
+    const int cy = cell_of(pos[p * 3 + 1], pg.h[1], pg.n) / ky;
+    const int cz = cell_of(pos[p * 3 + 2], pg.h[2], pg.n) / kz;
+    keys[p] = uint32_t((cx * ncz + cz) * nys + cy);
+    vals[p] = uint32_t(p);
+}
+
+// Velocity at every tracer + Adams-Bashforth into the history ring (the
+// MODE 0 work of k_brick, same arithmetic: FP64 weights, each FP32 field
+// value converted to double and accumulated in double,
+// particles_field_computer.hpp:142-150).  Work item wi = seg ncols + col
+// (col = cx ncz + cz) for wi = rank, rank + G, ...; per work item three
+// walks (components 0, 1, 2) over the segment's y steps [s0, s1), whose
+// tracer ranges are read into LDS once per work item.  Each component of
+// the right-hand side goes to the head slot of the history ring as it is
+// computed; k_ab_moves then moves every tracer (a coalesced pass over the
+// history, kept out of the walks so that no workgroup waits on its
+// loads).  Per batch,
+// wave 0 holds one tracer per lane (slot and position loaded one and two
+// batches ahead) and leaves its cell offsets and in-cell fractions in LDS;
+// then every thread evaluates a share of the weights.  All control flow
+// that reaches a barrier is uniform over the workgroup.
+template <typename T, int NB>
+__global__ void __launch_bounds__((ColCfg<T, NB>::NT))
+k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __restrict__ rhs,
+         const uint32_t* __restrict__ perm, const uint32_t* __restrict__ kstart, const uint32_t* __restrict__ kend,
+         int ncz, int nys, int ncols, int sps, int nwork, int64_t np, int head, int add_head,
+         const double* __restrict__ tab, int deg, PartGeomD pg) {
+    using CC = ColCfg<T, NB>;
+    constexpr int W = CC::W, NT = CC::NT, PB = CC::PB, LPT = CC::LPT, RZ = CC::RZ, NI = CC::NI;
+    constexpr int GROUPS = CC::GROUPS, WZ = CC::WZ, ZS = CC::ZS, XA = CC::XA, XS = CC::XS, WXT = CC::WXT;
+    constexpr int KX = CC::KX, KY = CC::KY, KZ = CC::KZ, WS = CC::WS, WQ = CC::WQ, R = CC::R, CH = CC::CH;
+    constexpr int MAXS = CC::MAXS;
+    constexpr int SLOTV = CC::SLOT_BYTES / int(sizeof(T));
+    // highest polynomial degree of an (n, m) table: 2n+1 (m = 0) or 5 (m = 2)
+    constexpr int DM = W - 1 > 5 ? W - 1 : 5;
+    extern __shared__ double smem[];
+    unsigned char* ring = reinterpret_cast<unsigned char*>(smem);  // R x SLOT_BYTES
+    double* wgt = smem + size_t(R) * CC::SLOT_BYTES / 8;            // [PB][WQ]
+    double* tabl = wgt + PB * WQ;                                   // the weight table
+    double* res = tabl + CC::TABD;                                  // [PB]
+    double* frac = res + PB;                                        // [PB][3]
+    int* lcell = reinterpret_cast<int*>(frac + PB * 3);             // [PB][3]
+    uint32_t* rng = reinterpret_cast<uint32_t*>(lcell + PB * 3);    // [2][MAXS]: step ranges of the work item
+
+    const int n = pg.n;
+    const int G = gridDim.x;
+    const int rank = (G % 8 == 0) ? int(blockIdx.x % 8) * (G / 8) + int(blockIdx.x / 8) : int(blockIdx.x);
+    const int tid = threadIdx.x;
+    const int lane = tid % LPT, grp = tid / LPT, jx = lane & 15, rz = lane >> 4;
+    const int wv = tid / 64, ln = tid % 64;
+    // real planes 2 plane_size(n) reals apart, y rows ys reals apart (the
+    // offsets within a plane fit 32 bits for n <= 1024)
+    const size_t ps = 2 * plane_size(n);
+    const uint32_t ys = uint32_t(2 * row_stride(n));
+    for (int i = tid; i < W * (deg + 1); i += NT) tabl[i] = tab[i];  // visible after the first barrier
+
+    // work item -> column origin and y steps
+    struct Item {
+        int cx0, cz0, s0, s1;
+    };
+    auto item = [&](int wi) -> Item {
+        const int col = wi % ncols, seg = wi / ncols;
+        const int s0 = seg * sps;
+        return Item{(col / ncz) * KX, (col % ncz) * KZ, s0, min(s0 + sps, nys)};
+    };
+    auto key_of = [&](int wi, int s) -> uint32_t { return uint32_t(wi % ncols) * uint32_t(nys) + uint32_t(s); };
+    // tracer range of step s of work item wi: from the LDS table when wi is
+    // the current work item cur (uniform)
+    auto range = [&](int wi, int cur, int s0, int s, uint32_t& a, uint32_t& e) {
+        if (wi == cur) {
+            a = rng[s - s0];
+            e = rng[MAXS + s - s0];
+        } else {
+            const uint32_t k = key_of(wi, s);
+            a = kstart[k];
+            e = kend[k];
+        }
+    };
+
+    // batch cursor over (work item, walk, step, batch start); uniform
+    struct Cur {
+        int wi, c, s, s0, s1;
+        uint32_t bs, e;
+    };
+    // first non-empty step at or after cu.s in walks c.. of work items wi..
+    auto settle = [&](Cur& cu, int cur) -> bool {
+        for (;;) {
+            if (cu.wi >= nwork) return false;
+            for (; cu.s < cu.s1; cu.s++) {
+                uint32_t a, e;
+                range(cu.wi, cur, cu.s0, cu.s, a, e);
+                if (a < e) {
+                    cu.bs = a;
+                    cu.e = e;
+                    return true;
+                }
+            }
+            if (++cu.c == 3) {
+                cu.c = 0;
+                cu.wi += G;
+                if (cu.wi >= nwork) return false;
+                const Item it = item(cu.wi);
+                cu.s0 = it.s0;
+                cu.s1 = it.s1;
+            }
+            cu.s = cu.s0;
+        }
+    };
+    auto next = [&](Cur& cu, int cur) -> bool {
+        cu.bs += PB;
+        if (cu.bs < cu.e) return true;
+        cu.s++;
+        return settle(cu, cur);
+    };
+
+    // LDS DMA: wave wv < CH moves chunk wv of each plane slot; lane ln holds
+    // the 16-byte group at offset loff from the plane's column origin
+    // (~0u: a pad slot, loaded from the origin).  x and z wrap per group
+    // (x groups never straddle the box edge: n and the window start are
+    // multiples of XA).
+    uint32_t loff = ~0u;
+    auto set_column = [&](const Item& it) {
+        const int v0 = (wv * 64 + ln) * XA;
+        const int lz = v0 / ZS, lx = v0 % ZS;
+        loff = ~0u;
+        if (wv < CH && lz < WZ && lx < WXT) {
+            int gx = it.cx0 - NB - XS + lx, gz = it.cz0 - NB + lz;
+            gx = gx < 0 ? gx + n : (gx >= n ? gx - n : gx);
+            gz = gz < 0 ? gz + n : (gz >= n ? gz - n : gz);
+            loff = uint32_t(gz * n + gx);
+        }
+    };
+    // plane P of the walk (global y = s0 KY - NB + P, wrapped) into slot P % R
+    auto load_plane = [&](const T* fc, int ybase, int P) {
+        if (wv < CH) {  // uniform per wave
+            int gy = ybase + P;
+            gy = gy < 0 ? gy + n : (gy >= n ? gy - n : gy);
+            const T* src = fc + uint32_t(gy) * ys + (loff == ~0u ? 0u : loff);
+            unsigned char* dst = ring + size_t(P % R) * CC::SLOT_BYTES + size_t(wv) * 1024;
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src),
+                                             (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+        }
+    };
+
+    // staging (wave 0): lane q holds tracer q of the next batch (slot p1,
+    // position x1) and the slot p2 of the batch after it
+    Cur n1{rank, 0, 0, 0, 0, 0, 0};
+    if (n1.wi < nwork) {
+        const Item it = item(n1.wi);
+        n1.s = n1.s0 = it.s0;
+        n1.s1 = it.s1;
+    }
+    bool h1 = settle(n1, -1);
+    Cur n2 = n1;
+    bool h2 = h1 && next(n2, -1);
+    uint32_t p1 = 0, p2 = 0;
+    double x1[3] = {0.0, 0.0, 0.0};
+    if (tid < PB) {
+        if (h1 && n1.bs + tid < n1.e) {
+            p1 = perm[n1.bs + tid];
+#pragma unroll
+            for (int d = 0; d < 3; d++) x1[d] = pos[size_t(p1) * 3 + d];
+        }
+        if (h2 && n2.bs + tid < n2.e) p2 = perm[n2.bs + tid];
+    }
+
+    for (int wi = rank; wi < nwork; wi += G) {
+        const Item it = item(wi);
+        set_column(it);
+        const int ybase = it.s0 * KY - NB;
+        for (int i = tid; i < it.s1 - it.s0; i += NT) {
+            const uint32_t k = key_of(wi, it.s0 + i);
+            rng[i] = kstart[k];
+            rng[MAXS + i] = kend[k];
+        }
+        for (int c = 0; c < 3; c++) {
+            const T* fc = f + c * ps;
+            __syncthreads();  // the previous walk's sums are done with the ring; the range table is whole
+            for (int P = 0; P < KY + W - 1; P++) load_plane(fc, ybase, P);
+            for (int s = it.s0; s < it.s1; s++) {
+                __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA and staging loads have landed
+                __syncthreads();                // step s's window is whole; step s - 1's first slots are free
+                if (s + 1 < it.s1) {
+                    const int P0 = (s + 1 - it.s0) * KY + W - 1;
+                    for (int k = 0; k < KY; k++) load_plane(fc, ybase, P0 + k);
+                }
+                const uint32_t a = rng[s - it.s0], e = rng[MAXS + s - it.s0];
+                for (uint32_t bs = a; bs < e; bs += PB) {
+                    // this batch is n1 (uniform): take its staged tracers,
+                    // then stage the next two
+                    const int cnt = int(min(uint32_t(PB), e - bs));
+                    const uint32_t myp = p1;
+                    const double myx[3] = {x1[0], x1[1], x1[2]};
+                    n1 = n2;
+                    h1 = h2;
+                    p1 = p2;
+                    h2 = h2 && next(n2, wi);
+                    if (tid < PB) {
+                        if (h1 && n1.bs + tid < n1.e) {
+#pragma unroll
+                            for (int d = 0; d < 3; d++) x1[d] = pos[size_t(p1) * 3 + d];
+                        }
+                        if (h2 && n2.bs + tid < n2.e) p2 = perm[n2.bs + tid];
+                        // cell offsets in the column window and in-cell
+                        // fractions (get_norm_pos_in_cell / pbc_field_layer)
+                        if (tid < cnt) {
+                            const int base[3] = {it.cx0, it.s0 * KY, it.cz0};
+#pragma unroll
+                            for (int d = 0; d < 3; d++) {
+                                const double sp = myx[d];
+                                const double rep = floor(sp / pg.width[d]);
+                                const double inbox = sp - rep * pg.width[d];
+                                const double ci = floor(inbox / pg.h[d]);
+                                frac[tid * 3 + d] = (inbox - ci * pg.h[d]) / pg.h[d];
+                                lcell[tid * 3 + d] = cell_of(sp, pg.h[d], n) - base[d];
+                            }
+                        }
+                    }
+                    __syncthreads();  // cells and fractions of the batch
+                    {
+                        // thread (q, j = part) evaluates weight j of tracer
+                        // q on the three axes from its table row in registers
+                        static_assert(NT / PB >= W, "one weight index per thread");
+                        const int q = tid % PB, j = tid / PB;
+                        if (q < cnt && j < W) {
+                            double trow[DM + 1];
+#pragma unroll
+                            for (int k = 0; k <= DM; k++) trow[k] = k <= deg ? tabl[j * (deg + 1) + k] : 0.0;
+#pragma unroll
+                            for (int ax = 0; ax < 3; ax++)
+                                wgt[q * WQ + ax * WS + j] = weight_of<DM>(trow, deg, frac[q * 3 + ax]);
+                        }
+                    }
+                    __syncthreads();  // weights of the batch
+                    const T* rt = reinterpret_cast<const T*>(ring);
+                    for (int qb = grp; qb < cnt; qb += GROUPS) {  // whole lane groups iterate together
+                        const double* wq = wgt + qb * WQ;
+                        const double wxo = jx < W ? wq[jx] : 0.0;
+                        double wzr[NI];
+#pragma unroll
+                        for (int i = 0; i < NI; i++) wzr[i] = wq[2 * WS + rz * NI + i];
+                        const int lc0 = lcell[qb * 3 + 0], lc1 = lcell[qb * 3 + 1], lc2 = lcell[qb * 3 + 2];
+                        // this lane's column x = lc0 + jx (lanes past the
+                        // stencil read column W - 1 with weight 0), z rows
+                        // lc2 + rz NI + i (consecutive rows: one base
+                        // address serves ds_read2 pairs), y planes lc1 + jy
+                        // in ring slots
+                        const T* tcol = rt + (XS + lc0 + (jx < W ? jx : W - 1)) + (lc2 + rz * NI) * ZS;
+                        // plane jy sits in slot (lc1 + jy) mod R: from pa
+                        // before the ring's end, from pb = pa - R slots after
+                        const int sl0 = lc1 % R;
+                        const T* pa = tcol + sl0 * SLOTV;
+                        const T* pb = pa - R * SLOTV;
+                        const int wrap = R - sl0;
+                        double acc = 0.0;
+                        // software pipeline: plane jy + PD's LDS reads are
+                        // issued before plane jy's conversions and FMAs
+                        constexpr int PD = BFPSH_BRICK_PREFETCH;
+                        // (the plane's weight is read with its values: LDS
+                        // reads return in order, so waiting for plane jy
+                        // never waits for the planes behind it)
+                        T tb[W][NI];
+                        double wyv[W];
+                        auto rd = [&](int jy) {
+                            const T* tz = (jy < wrap ? pa : pb) + jy * SLOTV;
+                            wyv[jy] = wq[WS + jy];
+#pragma unroll
+                            for (int i = 0; i < NI; i++) tb[jy][i] = tz[i * ZS];
+                        };
+#pragma unroll
+                        for (int jy = 0; jy < PD && jy < W; jy++) rd(jy);
+#pragma unroll
+                        for (int jy = 0; jy < W; jy++) {
+                            if (jy + PD < W) rd(jy + PD);
+                            double cz = 0.0;
+#pragma unroll
+                            for (int i = 0; i < NI; i++) cz = fma(wzr[i], double(tb[jy][i]), cz);
+                            acc = fma(wyv[jy], cz, acc);
+                        }
+                        acc = row16_sum(acc * wxo);
+#pragma unroll
+                        for (int o = 16; o < LPT; o *= 2) acc += __shfl_xor(acc, o, LPT);
+                        if (lane == 0) res[qb] = acc;
+                    }
+                    __syncthreads();  // the batch's results are complete; its LDS slots may be reused
+                    if (tid < cnt) {
+                        double* h0 = rhs + size_t(head) * np * 3 + int64_t(myp) * 3 + c;
+                        *h0 = res[tid] + (add_head ? *h0 : 0.0);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Adams-Bashforth move of every tracer from the right-hand side in the head
+// slot of the history ring (the second half of the column walk's velocity
+// loop)
+__global__ void k_ab_moves(double* __restrict__ pos, const double* __restrict__ rhs, int64_t np, int nsteps,
+                           int head, int order, double dt) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const double* h0 = rhs + size_t(head) * np * 3 + p * 3;
+    const double r0[3] = {h0[0], h0[1], h0[2]};
+    const double x[3] = {pos[p * 3 + 0], pos[p * 3 + 1], pos[p * 3 + 2]};
+    ab_move(pos, rhs, p, r0, x, np, nsteps, head, order, dt);
 }
 
 constexpr int kThreads = 256;
@@ -941,10 +1340,87 @@ int launch_brick(bfpsh_ctx* ctx, const void* f, double* d_out, double dt, int or
     return BFPSH_OK;
 }
 
+// column walk geometry of a box: columns, y steps, steps per segment, work
+// items (segments split the columns when there are fewer than two per CU)
+struct ColGeom {
+    int ncx, ncz, nys, ncols, sps, nwork;
+};
+template <typename T, int NB>
+bool col_geom(const bfpsh_ctx* ctx, ColGeom& g) {
+    using CC = ColCfg<T, NB>;
+    static_assert(CC::ok, "column walk geometry of a configuration that has none");
+    const int n = ctx->n;
+    if (ctx->force_coop || ctx->tracer_kernel != 0 || n < 64 || n % CC::KX || n % CC::KZ || n % CC::KY) return false;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) return false;
+    g.ncx = n / CC::KX;
+    g.ncz = n / CC::KZ;
+    g.nys = n / CC::KY;
+    g.ncols = g.ncx * g.ncz;
+    const int want = (2 * ncu + g.ncols - 1) / g.ncols;  // segments per column
+    const int nseg = std::max(1, std::min(g.nys, want));
+    g.sps = (g.nys + nseg - 1) / nseg;
+    g.nwork = g.ncols * ((g.nys + g.sps - 1) / g.sps);
+    return g.sps <= CC::MAXS;
+}
+
+// bucket the tracers by (column, y step): sorted permutation in d_perm + np,
+// per-key ranges in d_bstart[0..nk), d_bstart[nk..2nk)
+template <typename T, int NB>
+int sort_cols(bfpsh_ctx* ctx, const ColGeom& g) {
+    using CC = ColCfg<T, NB>;
+    int r = ensure_sort(ctx);
+    if (r) return r;
+    const int64_t nk = int64_t(g.ncols) * g.nys;
+    r = ensure_bricks<T, NB>(ctx, nk);
+    if (r) return r;
+    const int64_t np = ctx->np;
+    int nbits = 1;
+    while ((int64_t(1) << nbits) < nk) nbits++;
+    HIPCHK(hipMemsetAsync(ctx->d_bstart, 0, 2 * nk * sizeof(uint32_t), ctx->stream));
+    LAUNCH("part_keys", (k_col_keys<<<unsigned((np + 255) / 256), 256, 0, ctx->stream>>>(
+                            ctx->d_pos, np, pgeom_d(ctx), CC::KX, CC::KY, CC::KZ, g.ncz, g.nys, ctx->d_keys,
+                            ctx->d_perm)));
+    size_t tmp = ctx->sort_tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_tmp, tmp, ctx->d_keys, ctx->d_keys + np, ctx->d_perm,
+                                              ctx->d_perm + np, int(np), 0, nbits, ctx->stream));
+    LAUNCH("part_bounds", (k_brick_bounds<<<unsigned((np + 255) / 256), 256, 0, ctx->stream>>>(
+                              ctx->d_keys + np, np, ctx->d_bstart, ctx->d_bstart + nk)));
+    ctx->sort_age = 0;
+    return BFPSH_OK;
+}
+
+template <typename T, int NB>
+int launch_col(bfpsh_ctx* ctx, const ColGeom& g, double dt, int order) {
+    using CC = ColCfg<T, NB>;
+    const int64_t nk = int64_t(g.ncols) * g.nys;
+    auto kern = k_column<T, NB>;
+    HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(CC::LDS)));
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    // one workgroup per CU, persistent over its work items
+    const unsigned grid = unsigned(std::max(1, std::min(g.nwork, ncu)));
+    LAUNCH("interp_ab_col", (kern<<<grid, CC::NT, CC::LDS, ctx->stream>>>(
+                            (const T*)ctx->ru, ctx->d_pos, ctx->d_rhs, ctx->d_perm + ctx->np, ctx->d_bstart,
+                            ctx->d_bstart + nk, g.ncz, g.nys, g.ncols, g.sps, g.nwork, ctx->np, ctx->head,
+                            ctx->add_head ? 1 : 0, ctx->d_tab, ctx->deg, pgeom_d(ctx))));
+    LAUNCH("ab_moves", (k_ab_moves<<<unsigned((ctx->np + 255) / 256), 256, 0, ctx->stream>>>(
+                           ctx->d_pos, ctx->d_rhs, ctx->np, ctx->nsteps, ctx->head, order, dt)));
+    return BFPSH_OK;
+}
+
 template <typename T, int NB>
 int loop_nb(bfpsh_ctx* ctx, double dt, int order) {
+    if constexpr (ColCfg<T, NB>::ok) {
+        ColGeom g;
+        if (col_geom<T, NB>(ctx, g)) {
+            int r = sort_cols<T, NB>(ctx, g);
+            if (r) return r;
+            return launch_col<T, NB>(ctx, g, dt, order);
+        }
+    }
     if constexpr (BrickCfg<T, NB>::ok) {
-        if (ctx->force_coop) goto coop;
+        if (ctx->force_coop || ctx->tracer_kernel == 2) goto coop;
         int r = sort_bricks<T, NB>(ctx);
         if (r) return r;
         return launch_brick<T, NB, 3, 0>(ctx, ctx->ru, nullptr, dt, order, "interp_ab");
@@ -964,12 +1440,14 @@ coop:
 template <typename T, int NB>
 int sample_nb(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
     if constexpr (BrickCfg<T, NB>::ok) {
-        int r = sort_bricks<T, NB>(ctx);
-        if (r) return r;
-        if constexpr (sizeof(T) == 8)
-            if (nc == 1) return launch_brick<T, NB, 1, 1>(ctx, f, d_out, 0.0, 0, "sample1");
-        if (nc == 3) return launch_brick<T, NB, 3, 1>(ctx, f, d_out, 0.0, 0, "sample");
-        return launch_brick<T, NB, 9, 1>(ctx, f, d_out, 0.0, 0, "sample9");
+        if (ctx->tracer_kernel != 2) {
+            int r = sort_bricks<T, NB>(ctx);
+            if (r) return r;
+            if constexpr (sizeof(T) == 8)
+                if (nc == 1) return launch_brick<T, NB, 1, 1>(ctx, f, d_out, 0.0, 0, "sample1");
+            if (nc == 3) return launch_brick<T, NB, 3, 1>(ctx, f, d_out, 0.0, 0, "sample");
+            return launch_brick<T, NB, 9, 1>(ctx, f, d_out, 0.0, 0, "sample9");
+        }
     }
     {
         int r = sort_tracers(ctx);

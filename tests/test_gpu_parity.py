@@ -278,7 +278,8 @@ def test_errors_are_reported(B):
                                     (32, 10, 2)])
 def test_tracers_fp32_bricks(B, oracle, n, nb, m):
     """FP32 fields, many tracers per brick and several bricks per axis (n <=
-    7, weights stored per batch; n >= 8 the global-gather kernel); the
+    7, weights stored per batch; n >= 8 the global-gather kernel; the
+    velocity loop of the 64^3 boxes with n <= 6 is the column walk); the
     interpolated velocities / gradients are FP64 sums of FP32 field values."""
     from bfps_amd import ic
     say("case: fp32 tracers n=%d nb=%d m=%d" % (n, nb, m))
@@ -316,6 +317,78 @@ def test_tracers_fp32_bricks(B, oracle, n, nb, m):
     sg, og = ps.sample("gradient"), op.sample("gradient")
     say("  gradient sample rel err %.3e" % (np.abs(sg - og).max() / np.abs(og).max()))
     assert np.abs(sg - og).max() < 1e-5 * np.abs(og).max()
+
+
+def _fp32_tracer_run(B, n, nb, m, pos, kernel, monkeypatch, seed=21, steps=3, w0=None):
+    """FP32 box with tracers advanced `steps` NSVEparticles steps by the
+    velocity-loop kernel `kernel` ("auto": the column walk for these boxes,
+    "brick": independent bricks); returns (positions, history, step index)."""
+    from bfps_amd import ic
+    if kernel == "auto":
+        monkeypatch.delenv("BFPSH_TRACER_KERNEL", raising=False)
+    else:
+        monkeypatch.setenv("BFPSH_TRACER_KERNEL", kernel)
+    fs = B.vorticity_equation("t", n, precision="single")
+    fs.nu, fs.forcing_type = 0.01, "linear"
+    fs.io_checkpoint(read=True, data=ic.solenoidal_vorticity(n, seed=seed, k0=4.0) if w0 is None else w0)
+    ps = B.particles_system(fs, len(pos), nb, m, 4, step_idx=1)
+    ps.put(pos)
+    monkeypatch.delenv("BFPSH_TRACER_KERNEL", raising=False)
+    dt = 0.5 / n
+    fs.profile(True)
+    for _ in range(steps):
+        B.NSVEparticles_step(fs, dt)
+    prof = fs.profile_report()
+    # the column walk is profiled as interp_ab_col, bricks as interp_ab
+    assert ("interp_ab_col" in prof) == (kernel == "auto") and ("interp_ab" in prof) == (kernel == "brick")
+    gp, grhs, _, gsi = ps.get()
+    fs.close()
+    return gp, grhs, gsi
+
+
+@pytest.mark.parametrize("n,nb,m", [(128, 6, 2), (128, 4, 1), (128, 1, 1), (128, 2, 2), (128, 3, 0), (128, 5, 2),
+                                    (256, 6, 2), (256, 4, 1)])
+def test_tracers_fp32_columns(B, oracle, monkeypatch, n, nb, m):
+    """FP32 boxes of 64^3 and more advance their tracers by the column walk
+    (y-stepping LDS ring of planes; n <= 6, and n <= 4 with 8-cell steps);
+    against the oracle (as the brick kernel: FP64 sums of FP32 field values)
+    and against the brick kernel on the same case (same sums in another
+    order: within 1e-12 relative).  Tracers include negative and
+    out-of-box coordinates, a dense cluster in one column (several batches
+    per y step) and empty columns (the cluster's share)."""
+    from bfps_amd import ic
+    say("case: fp32 column walk n=%d nb=%d m=%d" % (n, nb, m))
+    npart = 30000
+    pos = ic.uniform_positions(npart, seed=5 + nb)
+    h = 2 * np.pi / n
+    rng = np.random.default_rng(nb)
+    pos[:20] -= 2 * np.pi
+    pos[20:40] += 4 * np.pi
+    # 3000 tracers in a 4 x 8 x 4-cell block: up to ~100 per y step
+    pos[1000:4000] = np.array([5.5, 40.0, 9.5]) * h + rng.random((3000, 3)) * np.array([4, 8, 4]) * h
+    # exactly on cell faces
+    pos[4000:4100] = np.round(pos[4000:4100] / h) * h
+    gp, grhs, gsi = _fp32_tracer_run(B, n, nb, m, pos, "auto", monkeypatch)
+    bp, brhs, bsi = _fp32_tracer_run(B, n, nb, m, pos, "brick", monkeypatch)
+    scale = max(1.0, np.abs(brhs).max())
+    say("  column vs brick: pos %.3e rhs %.3e" % (np.abs(gp - bp).max(), np.abs(grhs - brhs).max() / scale))
+    assert gsi == bsi == 4
+    assert np.abs(grhs - brhs).max() <= 1e-12 * scale
+    assert np.abs(gp - bp).max() <= 1e-12 * max(1.0, np.abs(bp).max())
+    if n <= 128:
+        o = oracle.Solver(n, precision="single")
+        o.set_physics(0.01, "linear", 1.0, 2.0, 4.0)
+        o.set_cfield("vorticity", ic.solenoidal_vorticity(n, seed=21, k0=4.0))
+        o.io_projection()
+        op = oracle.Particles(o, npart, nb, m, 4, step_idx=1)
+        op.set_state(pos)
+        for _ in range(3):
+            oracle.nsve_particles_step(o, op, 0.5 / n)
+        opos, orhs, osi = op.get_state()
+        say("  column vs oracle: pos %.3e rhs %.3e" % (np.abs(gp - opos).max(), np.abs(grhs - orhs).max()))
+        assert osi == gsi
+        assert np.abs(grhs - orhs).max() < 1e-5 * max(1.0, np.abs(orhs).max())
+        assert np.abs(gp - opos).max() < 1e-7
 
 
 @pytest.mark.parametrize("precision,k,inband", [("double", (1, 2, 0), True), ("double", (2, -1, 3), True),
