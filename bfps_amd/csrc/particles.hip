@@ -497,6 +497,25 @@ BFPSH_DEV double weight_of(const double (&row)[DM + 1], int deg, double xr) {
     return horner<DM, DM>(row, xr);
 }
 
+// weight j of one tracer on the three axes (in-cell positions fx, fy, fz)
+// from the degree-D coefficients c of polynomial j, into w[0], w[WS], w[2 WS]
+template <int D, int WS>
+BFPSH_DEV void weights3(const double* __restrict__ c, double fx, double fy, double fz, double* __restrict__ w) {
+    double k[D + 1];
+#pragma unroll
+    for (int i = 0; i <= D; i++) k[i] = c[i];
+    double a = k[D], b = k[D], e = k[D];
+#pragma unroll
+    for (int i = D - 1; i >= 0; i--) {
+        a = fma(a, fx, k[i]);
+        b = fma(b, fy, k[i]);
+        e = fma(e, fz, k[i]);
+    }
+    w[0] = a;
+    w[WS] = b;
+    w[2 * WS] = e;
+}
+
 // Adams-Bashforth move of tracer p from its position x by the new right-hand
 // side r0 (already in the head slot of the history ring) and the order - 1
 // older ones (particles_adams_bashforth.hpp:18-133)
@@ -935,24 +954,23 @@ __global__ void k_col_keys(const double* __restrict__ pos, int64_t np, PartGeomD
     vals[p] = uint32_t(p);
 }
 
-// Velocity at every tracer + Adams-Bashforth into the history ring (the
-// MODE 0 work of k_brick, same arithmetic: FP64 weights, each FP32 field
-// value converted to double and accumulated in double,
-// particles_field_computer.hpp:142-150).  Work item wi = seg ncols + col
-// (col = cx ncz + cz) for wi = rank, rank + G, ...; per work item three
-// walks (components 0, 1, 2) over the segment's y steps [s0, s1), whose
-// tracer ranges are read into LDS once per work item.  Each component of
-// the right-hand side goes to the head slot of the history ring as it is
-// computed; k_ab_moves then moves every tracer (a coalesced pass over the
-// history, kept out of the walks so that no workgroup waits on its
-// loads).  Per batch,
+// The NC components of a field at every tracer (k_brick's work, same
+// arithmetic: FP64 weights, each FP32 field value converted to double and
+// accumulated in double, particles_field_computer.hpp:142-150).  MODE 0:
+// the velocity into the head slot of the history ring (k_ab_moves then
+// moves every tracer: a coalesced pass over the history, kept out of the
+// walks so that no workgroup waits on its loads); MODE 1: out[p][NC].
+// Work item wi = seg ncols + col (col = cx ncz + cz) for wi = rank,
+// rank + G, ...; per work item NC walks (one per component) over the
+// segment's y steps [s0, s1), whose tracer ranges are read into LDS once
+// per work item.  Per batch,
 // wave 0 holds one tracer per lane (slot and position loaded one and two
 // batches ahead) and leaves its cell offsets and in-cell fractions in LDS;
 // then every thread evaluates a share of the weights.  All control flow
 // that reaches a barrier is uniform over the workgroup.
-template <typename T, int NB>
+template <typename T, int NB, int NC, int MODE>
 __global__ void __launch_bounds__((ColCfg<T, NB>::NT))
-k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __restrict__ rhs,
+k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __restrict__ out,
          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ kstart, const uint32_t* __restrict__ kend,
          int ncz, int nys, int ncols, int sps, int nwork, int64_t np, int head, int add_head,
          const double* __restrict__ tab, int deg, PartGeomD pg) {
@@ -1026,7 +1044,7 @@ k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
                     return true;
                 }
             }
-            if (++cu.c == 3) {
+            if (++cu.c == NC) {
                 cu.c = 0;
                 cu.wi += G;
                 if (cu.wi >= nwork) return false;
@@ -1104,7 +1122,7 @@ k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
             rng[i] = kstart[k];
             rng[MAXS + i] = kend[k];
         }
-        for (int c = 0; c < 3; c++) {
+        for (int c = 0; c < NC; c++) {
             const T* fc = f + c * ps;
             __syncthreads();  // the previous walk's sums are done with the ring; the range table is whole
             for (int P = 0; P < KY + W - 1; P++) load_plane(fc, ybase, P);
@@ -1154,12 +1172,21 @@ k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
                         static_assert(NT / PB >= W, "one weight index per thread");
                         const int q = tid % PB, j = tid / PB;
                         if (q < cnt && j < W) {
-                            double trow[DM + 1];
+                            const double* c = tabl + j * (deg + 1);
+                            double* wo = wgt + q * WQ + j;
+                            const double fx = frac[q * 3 + 0], fy = frac[q * 3 + 1], fz = frac[q * 3 + 2];
+                            if (deg == 5) {
+                                weights3<5, WS>(c, fx, fy, fz, wo);
+                            } else if (deg == 3) {
+                                weights3<3, WS>(c, fx, fy, fz, wo);
+                            } else {
+                                double trow[DM + 1];
 #pragma unroll
-                            for (int k = 0; k <= DM; k++) trow[k] = k <= deg ? tabl[j * (deg + 1) + k] : 0.0;
-#pragma unroll
-                            for (int ax = 0; ax < 3; ax++)
-                                wgt[q * WQ + ax * WS + j] = weight_of<DM>(trow, deg, frac[q * 3 + ax]);
+                                for (int k = 0; k <= DM; k++) trow[k] = k <= deg ? c[k] : 0.0;
+                                wo[0] = weight_of<DM>(trow, deg, fx);
+                                wo[WS] = weight_of<DM>(trow, deg, fy);
+                                wo[2 * WS] = weight_of<DM>(trow, deg, fz);
+                            }
                         }
                     }
                     __syncthreads();  // weights of the batch
@@ -1215,8 +1242,13 @@ k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
                     }
                     __syncthreads();  // the batch's results are complete; its LDS slots may be reused
                     if (tid < cnt) {
-                        double* h0 = rhs + size_t(head) * np * 3 + int64_t(myp) * 3 + c;
-                        *h0 = res[tid] + (add_head ? *h0 : 0.0);
+                        if constexpr (MODE == 0) {
+                            static_assert(NC == 3, "the velocity loop interpolates 3 components");
+                            double* h0 = out + size_t(head) * np * 3 + int64_t(myp) * 3 + c;
+                            *h0 = res[tid] + (add_head ? *h0 : 0.0);
+                        } else {
+                            out[int64_t(myp) * NC + c] = res[tid];
+                        }
                     }
                 }
             }
@@ -1390,22 +1422,20 @@ int sort_cols(bfpsh_ctx* ctx, const ColGeom& g) {
     return BFPSH_OK;
 }
 
-template <typename T, int NB>
-int launch_col(bfpsh_ctx* ctx, const ColGeom& g, double dt, int order) {
+template <typename T, int NB, int NC, int MODE>
+int launch_col(bfpsh_ctx* ctx, const ColGeom& g, const void* f, double* d_out, const char* name) {
     using CC = ColCfg<T, NB>;
     const int64_t nk = int64_t(g.ncols) * g.nys;
-    auto kern = k_column<T, NB>;
+    auto kern = k_column<T, NB, NC, MODE>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(CC::LDS)));
     int ncu = 0;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     // one workgroup per CU, persistent over its work items
     const unsigned grid = unsigned(std::max(1, std::min(g.nwork, ncu)));
-    LAUNCH("interp_ab_col", (kern<<<grid, CC::NT, CC::LDS, ctx->stream>>>(
-                            (const T*)ctx->ru, ctx->d_pos, ctx->d_rhs, ctx->d_perm + ctx->np, ctx->d_bstart,
-                            ctx->d_bstart + nk, g.ncz, g.nys, g.ncols, g.sps, g.nwork, ctx->np, ctx->head,
-                            ctx->add_head ? 1 : 0, ctx->d_tab, ctx->deg, pgeom_d(ctx))));
-    LAUNCH("ab_moves", (k_ab_moves<<<unsigned((ctx->np + 255) / 256), 256, 0, ctx->stream>>>(
-                           ctx->d_pos, ctx->d_rhs, ctx->np, ctx->nsteps, ctx->head, order, dt)));
+    LAUNCH(name, (kern<<<grid, CC::NT, CC::LDS, ctx->stream>>>(
+                     (const T*)f, ctx->d_pos, d_out, ctx->d_perm + ctx->np, ctx->d_bstart, ctx->d_bstart + nk, g.ncz,
+                     g.nys, g.ncols, g.sps, g.nwork, ctx->np, ctx->head, ctx->add_head ? 1 : 0, ctx->d_tab, ctx->deg,
+                     pgeom_d(ctx))));
     return BFPSH_OK;
 }
 
@@ -1416,7 +1446,11 @@ int loop_nb(bfpsh_ctx* ctx, double dt, int order) {
         if (col_geom<T, NB>(ctx, g)) {
             int r = sort_cols<T, NB>(ctx, g);
             if (r) return r;
-            return launch_col<T, NB>(ctx, g, dt, order);
+            r = launch_col<T, NB, 3, 0>(ctx, g, ctx->ru, ctx->d_rhs, "interp_ab_col");
+            if (r) return r;
+            LAUNCH("ab_moves", (k_ab_moves<<<unsigned((ctx->np + 255) / 256), 256, 0, ctx->stream>>>(
+                                   ctx->d_pos, ctx->d_rhs, ctx->np, ctx->nsteps, ctx->head, order, dt)));
+            return BFPSH_OK;
         }
     }
     if constexpr (BrickCfg<T, NB>::ok) {
@@ -1439,6 +1473,15 @@ coop:
 
 template <typename T, int NB>
 int sample_nb(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
+    if constexpr (ColCfg<T, NB>::ok) {
+        ColGeom g;
+        if ((nc == 3 || nc == 9) && col_geom<T, NB>(ctx, g)) {
+            int r = sort_cols<T, NB>(ctx, g);
+            if (r) return r;
+            if (nc == 3) return launch_col<T, NB, 3, 1>(ctx, g, f, d_out, "sample_col");
+            return launch_col<T, NB, 9, 1>(ctx, g, f, d_out, "sample9_col");
+        }
+    }
     if constexpr (BrickCfg<T, NB>::ok) {
         if (ctx->tracer_kernel != 2) {
             int r = sort_bricks<T, NB>(ctx);

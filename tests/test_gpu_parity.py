@@ -338,22 +338,29 @@ def _fp32_tracer_run(B, n, nb, m, pos, kernel, monkeypatch, seed=21, steps=3, w0
     fs.profile(True)
     for _ in range(steps):
         B.NSVEparticles_step(fs, dt)
+    fs.velocity_rspace()
+    sv = ps.sample("velocity")
+    fs.gradient_rspace()
+    sg = ps.sample("gradient")
     prof = fs.profile_report()
-    # the column walk is profiled as interp_ab_col, bricks as interp_ab
-    assert ("interp_ab_col" in prof) == (kernel == "auto") and ("interp_ab" in prof) == (kernel == "brick")
+    # the column walk is profiled as *_col, bricks without the suffix
+    col = kernel == "auto"
+    assert ("interp_ab_col" in prof) == col and ("interp_ab" in prof) == (not col)
+    assert ("sample_col" in prof) == col and ("sample9_col" in prof) == col
     gp, grhs, _, gsi = ps.get()
     fs.close()
-    return gp, grhs, gsi
+    return gp, grhs, gsi, sv, sg
 
 
 @pytest.mark.parametrize("n,nb,m", [(128, 6, 2), (128, 4, 1), (128, 1, 1), (128, 2, 2), (128, 3, 0), (128, 5, 2),
                                     (256, 6, 2), (256, 4, 1)])
 def test_tracers_fp32_columns(B, oracle, monkeypatch, n, nb, m):
-    """FP32 boxes of 64^3 and more advance their tracers by the column walk
-    (y-stepping LDS ring of planes; n <= 6, and n <= 4 with 8-cell steps);
-    against the oracle (as the brick kernel: FP64 sums of FP32 field values)
-    and against the brick kernel on the same case (same sums in another
-    order: within 1e-12 relative).  Tracers include negative and
+    """FP32 boxes of 64^3 and more advance and sample their tracers by the
+    column walk (y-stepping LDS ring of planes; n <= 6, and n <= 4 with
+    8-cell steps); against the oracle (as the brick kernel: FP64 sums of
+    FP32 field values) and against the brick kernel on the same case
+    (trajectories, velocity and gradient samples: the same sums in another
+    order, within 1e-12 relative).  Tracers include negative and
     out-of-box coordinates, a dense cluster in one column (several batches
     per y step) and empty columns (the cluster's share)."""
     from bfps_amd import ic
@@ -368,13 +375,17 @@ def test_tracers_fp32_columns(B, oracle, monkeypatch, n, nb, m):
     pos[1000:4000] = np.array([5.5, 40.0, 9.5]) * h + rng.random((3000, 3)) * np.array([4, 8, 4]) * h
     # exactly on cell faces
     pos[4000:4100] = np.round(pos[4000:4100] / h) * h
-    gp, grhs, gsi = _fp32_tracer_run(B, n, nb, m, pos, "auto", monkeypatch)
-    bp, brhs, bsi = _fp32_tracer_run(B, n, nb, m, pos, "brick", monkeypatch)
+    gp, grhs, gsi, gv, gg = _fp32_tracer_run(B, n, nb, m, pos, "auto", monkeypatch)
+    bp, brhs, bsi, bv, bg = _fp32_tracer_run(B, n, nb, m, pos, "brick", monkeypatch)
     scale = max(1.0, np.abs(brhs).max())
-    say("  column vs brick: pos %.3e rhs %.3e" % (np.abs(gp - bp).max(), np.abs(grhs - brhs).max() / scale))
+    say("  column vs brick: pos %.3e rhs %.3e velocity %.3e gradient %.3e"
+        % (np.abs(gp - bp).max(), np.abs(grhs - brhs).max() / scale, np.abs(gv - bv).max() / np.abs(bv).max(),
+           np.abs(gg - bg).max() / np.abs(bg).max()))
     assert gsi == bsi == 4
     assert np.abs(grhs - brhs).max() <= 1e-12 * scale
     assert np.abs(gp - bp).max() <= 1e-12 * max(1.0, np.abs(bp).max())
+    assert np.abs(gv - bv).max() <= 1e-12 * np.abs(bv).max()
+    assert np.abs(gg - bg).max() <= 1e-12 * np.abs(bg).max()
     if n <= 128:
         o = oracle.Solver(n, precision="single")
         o.set_physics(0.01, "linear", 1.0, 2.0, 4.0)
