@@ -68,6 +68,7 @@ struct bfpsh_ctx {
     uint32_t* d_perm = nullptr;    // 2 x np
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
+    double* d_crec = nullptr;      // column walk: per tracer in-cell fractions (3) + packed cells (np x 4)
     uint32_t* d_bstart = nullptr;  // per-brick [start, end) (2 x nbricks)
     int64_t nbricks = 0;
     int sort_every = 1;            // re-sort period in tracer steps

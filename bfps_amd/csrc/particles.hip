@@ -926,10 +926,9 @@ struct ColCfg {
     static constexpr int TABD = W * ((W - 1 > 5 ? W - 1 : 5) + 1);
     // y steps per work item (n / KY for an unsplit column)
     static constexpr int MAXS = 256;
-    // weights [PB][WQ], table, results [PB], in-cell positions [PB][3]
-    // (doubles), cells [PB][3] (ints), step ranges [2][MAXS] (uint32)
-    static constexpr size_t EXTRA = (size_t(PB) * WQ + TABD + PB + size_t(PB) * 3) * 8 + size_t(PB) * 3 * 4 +
-                                    size_t(2) * MAXS * 4;
+    // weights [PB][WQ], table, results [PB] (doubles), cells [PB][3] (ints),
+    // step ranges [2][MAXS] (uint32)
+    static constexpr size_t EXTRA = (size_t(PB) * WQ + TABD + PB) * 8 + size_t(PB) * 3 * 4 + size_t(2) * MAXS * 4;
     static constexpr size_t LDS_MAX = 160 * 1024;
     static constexpr size_t lds_of(int ky) { return size_t(2 * ky + W - 1) * SLOT_BYTES + EXTRA; }
     static constexpr int KY = lds_of(8) <= LDS_MAX ? 8 : (lds_of(4) <= LDS_MAX ? 4 : 0);
@@ -942,16 +941,33 @@ struct ColCfg {
 #endif
 };
 
-// sort key of a tracer: (column, y step), columns z-fastest
+// sort key of a tracer, (column, y step) with columns z-fastest, and its
+// record for the walk: in-cell fractions (get_norm_pos_in_cell /
+// pbc_field_layer, as k_brick's prologue) and the cell, 10 bits per axis
 __global__ void k_col_keys(const double* __restrict__ pos, int64_t np, PartGeomD pg, int kx, int ky, int kz,
-                           int ncz, int nys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                           int ncz, int nys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                           double* __restrict__ rec) {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= np) return;
-    const int cx = cell_of(pos[p * 3 + 0], pg.h[0], pg.n) / kx;
-    const int cy = cell_of(pos[p * 3 + 1], pg.h[1], pg.n) / ky;
-    const int cz = cell_of(pos[p * 3 + 2], pg.h[2], pg.n) / kz;
-    keys[p] = uint32_t((cx * ncz + cz) * nys + cy);
+    int cell[3];
+    double fr[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double sp = pos[p * 3 + d];
+        const double rep = floor(sp / pg.width[d]);
+        const double inbox = sp - rep * pg.width[d];
+        const double ci = floor(inbox / pg.h[d]);
+        fr[d] = (inbox - ci * pg.h[d]) / pg.h[d];
+        cell[d] = cell_of(sp, pg.h[d], pg.n);
+    }
+    keys[p] = uint32_t(((cell[0] / kx) * ncz + cell[2] / kz) * nys + cell[1] / ky);
     vals[p] = uint32_t(p);
+    double4 r;
+    r.x = fr[0];
+    r.y = fr[1];
+    r.z = fr[2];
+    r.w = __longlong_as_double((long long)(uint32_t(cell[0]) | (uint32_t(cell[1]) << 10) | (uint32_t(cell[2]) << 20)));
+    reinterpret_cast<double4*>(rec)[p] = r;
 }
 
 // The NC components of a field at every tracer (k_brick's work, same
@@ -963,14 +979,15 @@ __global__ void k_col_keys(const double* __restrict__ pos, int64_t np, PartGeomD
 // Work item wi = seg ncols + col (col = cx ncz + cz) for wi = rank,
 // rank + G, ...; per work item NC walks (one per component) over the
 // segment's y steps [s0, s1), whose tracer ranges are read into LDS once
-// per work item.  Per batch,
-// wave 0 holds one tracer per lane (slot and position loaded one and two
-// batches ahead) and leaves its cell offsets and in-cell fractions in LDS;
-// then every thread evaluates a share of the weights.  All control flow
-// that reaches a barrier is uniform over the workgroup.
+// per work item.  Per batch, thread t holds tracer t mod 64: its slot and
+// its record (in-cell fractions and cell, written by k_col_keys) are
+// loaded one and two batches ahead, so a batch starts with no global
+// latency; each thread evaluates one weight index of its tracer on the
+// three axes and the first wave leaves the cell offsets in LDS.  All
+// control flow that reaches a barrier is uniform over the workgroup.
 template <typename T, int NB, int NC, int MODE>
 __global__ void __launch_bounds__((ColCfg<T, NB>::NT))
-k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __restrict__ out,
+k_column(const T* __restrict__ f, const double* __restrict__ crec, double* __restrict__ out,
          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ kstart, const uint32_t* __restrict__ kend,
          int ncz, int nys, int ncols, int sps, int nwork, int64_t np, int head, int add_head,
          const double* __restrict__ tab, int deg, PartGeomD pg) {
@@ -987,8 +1004,7 @@ k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
     double* wgt = smem + size_t(R) * CC::SLOT_BYTES / 8;            // [PB][WQ]
     double* tabl = wgt + PB * WQ;                                   // the weight table
     double* res = tabl + CC::TABD;                                  // [PB]
-    double* frac = res + PB;                                        // [PB][3]
-    int* lcell = reinterpret_cast<int*>(frac + PB * 3);             // [PB][3]
+    int* lcell = reinterpret_cast<int*>(res + PB);                  // [PB][3]
     uint32_t* rng = reinterpret_cast<uint32_t*>(lcell + PB * 3);    // [2][MAXS]: step ranges of the work item
 
     const int n = pg.n;
@@ -1091,8 +1107,9 @@ k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
         }
     };
 
-    // staging (wave 0): lane q holds tracer q of the next batch (slot p1,
-    // position x1) and the slot p2 of the batch after it
+    // staging: thread t holds tracer t mod PB of the next batch (slot p1,
+    // record x1) and the slot p2 of the batch after it
+    const int qq = tid % PB;
     Cur n1{rank, 0, 0, 0, 0, 0, 0};
     if (n1.wi < nwork) {
         const Item it = item(n1.wi);
@@ -1103,15 +1120,13 @@ k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
     Cur n2 = n1;
     bool h2 = h1 && next(n2, -1);
     uint32_t p1 = 0, p2 = 0;
-    double x1[3] = {0.0, 0.0, 0.0};
-    if (tid < PB) {
-        if (h1 && n1.bs + tid < n1.e) {
-            p1 = perm[n1.bs + tid];
-#pragma unroll
-            for (int d = 0; d < 3; d++) x1[d] = pos[size_t(p1) * 3 + d];
-        }
-        if (h2 && n2.bs + tid < n2.e) p2 = perm[n2.bs + tid];
+    double4 x1 = make_double4(0.0, 0.0, 0.0, 0.0);
+    const double4* rec4 = reinterpret_cast<const double4*>(crec);
+    if (h1 && n1.bs + qq < n1.e) {
+        p1 = perm[n1.bs + qq];
+        x1 = rec4[p1];
     }
+    if (h2 && n2.bs + qq < n2.e) p2 = perm[n2.bs + qq];
 
     for (int wi = rank; wi < nwork; wi += G) {
         const Item it = item(wi);
@@ -1139,42 +1154,30 @@ k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
                     // then stage the next two
                     const int cnt = int(min(uint32_t(PB), e - bs));
                     const uint32_t myp = p1;
-                    const double myx[3] = {x1[0], x1[1], x1[2]};
+                    const double4 myr = x1;
                     n1 = n2;
                     h1 = h2;
                     p1 = p2;
                     h2 = h2 && next(n2, wi);
-                    if (tid < PB) {
-                        if (h1 && n1.bs + tid < n1.e) {
-#pragma unroll
-                            for (int d = 0; d < 3; d++) x1[d] = pos[size_t(p1) * 3 + d];
-                        }
-                        if (h2 && n2.bs + tid < n2.e) p2 = perm[n2.bs + tid];
-                        // cell offsets in the column window and in-cell
-                        // fractions (get_norm_pos_in_cell / pbc_field_layer)
-                        if (tid < cnt) {
-                            const int base[3] = {it.cx0, it.s0 * KY, it.cz0};
-#pragma unroll
-                            for (int d = 0; d < 3; d++) {
-                                const double sp = myx[d];
-                                const double rep = floor(sp / pg.width[d]);
-                                const double inbox = sp - rep * pg.width[d];
-                                const double ci = floor(inbox / pg.h[d]);
-                                frac[tid * 3 + d] = (inbox - ci * pg.h[d]) / pg.h[d];
-                                lcell[tid * 3 + d] = cell_of(sp, pg.h[d], n) - base[d];
-                            }
-                        }
-                    }
-                    __syncthreads();  // cells and fractions of the batch
+                    if (h1 && n1.bs + qq < n1.e) x1 = rec4[p1];
+                    if (h2 && n2.bs + qq < n2.e) p2 = perm[n2.bs + qq];
                     {
-                        // thread (q, j = part) evaluates weight j of tracer
-                        // q on the three axes from its table row in registers
+                        // thread (q, j) evaluates weight j of tracer q on
+                        // the three axes from its table row in registers;
+                        // the first wave leaves the cell offsets in the
+                        // column window
                         static_assert(NT / PB >= W, "one weight index per thread");
-                        const int q = tid % PB, j = tid / PB;
+                        const int q = qq, j = tid / PB;
+                        if (q < cnt && j == 0) {
+                            const uint32_t pc = uint32_t(__double_as_longlong(myr.w));
+                            lcell[q * 3 + 0] = int(pc & 1023u) - it.cx0;
+                            lcell[q * 3 + 1] = int((pc >> 10) & 1023u) - it.s0 * KY;
+                            lcell[q * 3 + 2] = int((pc >> 20) & 1023u) - it.cz0;
+                        }
                         if (q < cnt && j < W) {
                             const double* c = tabl + j * (deg + 1);
                             double* wo = wgt + q * WQ + j;
-                            const double fx = frac[q * 3 + 0], fy = frac[q * 3 + 1], fz = frac[q * 3 + 2];
+                            const double fx = myr.x, fy = myr.y, fz = myr.z;
                             if (deg == 5) {
                                 weights3<5, WS>(c, fx, fy, fz, wo);
                             } else if (deg == 3) {
@@ -1189,7 +1192,7 @@ k_column(const T* __restrict__ f, const double* __restrict__ pos, double* __rest
                             }
                         }
                     }
-                    __syncthreads();  // weights of the batch
+                    __syncthreads();  // weights and cells of the batch
                     const T* rt = reinterpret_cast<const T*>(ring);
                     for (int qb = grp; qb < cnt; qb += GROUPS) {  // whole lane groups iterate together
                         const double* wq = wgt + qb * WQ;
@@ -1382,7 +1385,9 @@ bool col_geom(const bfpsh_ctx* ctx, ColGeom& g) {
     using CC = ColCfg<T, NB>;
     static_assert(CC::ok, "column walk geometry of a configuration that has none");
     const int n = ctx->n;
-    if (ctx->force_coop || ctx->tracer_kernel != 0 || n < 64 || n % CC::KX || n % CC::KZ || n % CC::KY) return false;
+    // (cells are packed 10 bits per axis)
+    if (ctx->force_coop || ctx->tracer_kernel != 0 || n < 64 || n > 1024 || n % CC::KX || n % CC::KZ || n % CC::KY)
+        return false;
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) return false;
     g.ncx = n / CC::KX;
@@ -1407,12 +1412,16 @@ int sort_cols(bfpsh_ctx* ctx, const ColGeom& g) {
     r = ensure_bricks<T, NB>(ctx, nk);
     if (r) return r;
     const int64_t np = ctx->np;
+    if (!ctx->d_crec) {
+        HIPCHK(hipMalloc((void**)&ctx->d_crec, size_t(np) * 4 * sizeof(double)));
+        ctx->bytes += size_t(np) * 4 * sizeof(double);
+    }
     int nbits = 1;
     while ((int64_t(1) << nbits) < nk) nbits++;
     HIPCHK(hipMemsetAsync(ctx->d_bstart, 0, 2 * nk * sizeof(uint32_t), ctx->stream));
     LAUNCH("part_keys", (k_col_keys<<<unsigned((np + 255) / 256), 256, 0, ctx->stream>>>(
                             ctx->d_pos, np, pgeom_d(ctx), CC::KX, CC::KY, CC::KZ, g.ncz, g.nys, ctx->d_keys,
-                            ctx->d_perm)));
+                            ctx->d_perm, ctx->d_crec)));
     size_t tmp = ctx->sort_tmp_bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_tmp, tmp, ctx->d_keys, ctx->d_keys + np, ctx->d_perm,
                                               ctx->d_perm + np, int(np), 0, nbits, ctx->stream));
@@ -1433,7 +1442,7 @@ int launch_col(bfpsh_ctx* ctx, const ColGeom& g, const void* f, double* d_out, c
     // one workgroup per CU, persistent over its work items
     const unsigned grid = unsigned(std::max(1, std::min(g.nwork, ncu)));
     LAUNCH(name, (kern<<<grid, CC::NT, CC::LDS, ctx->stream>>>(
-                     (const T*)f, ctx->d_pos, d_out, ctx->d_perm + ctx->np, ctx->d_bstart, ctx->d_bstart + nk, g.ncz,
+                     (const T*)f, ctx->d_crec, d_out, ctx->d_perm + ctx->np, ctx->d_bstart, ctx->d_bstart + nk, g.ncz,
                      g.nys, g.ncols, g.sps, g.nwork, ctx->np, ctx->head, ctx->add_head ? 1 : 0, ctx->d_tab, ctx->deg,
                      pgeom_d(ctx))));
     return BFPSH_OK;
@@ -1732,8 +1741,9 @@ void bfpsh_part_batch_free(PartBatch& pb) {
 }
 
 int bfpsh_part_release(bfpsh_ctx* ctx) {
-    for (void* p : {(void*)ctx->d_keys, (void*)ctx->d_perm, ctx->d_sort_tmp, (void*)ctx->d_bstart})
+    for (void* p : {(void*)ctx->d_keys, (void*)ctx->d_perm, ctx->d_sort_tmp, (void*)ctx->d_bstart, (void*)ctx->d_crec})
         if (p) (void)hipFree(p);
+    ctx->d_crec = nullptr;
     ctx->d_bstart = nullptr;
     ctx->nbricks = 0;
     ctx->d_keys = nullptr;
