@@ -66,7 +66,7 @@ KERNEL_VOLUMES = {
     "fwd_y_update2": 12,   # read N, v0, v1, write v2
     "fwd_y_update3": 12,   # read N, v0, v2, write v0
 }
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_512_fp32.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_%d_fp32.txt")
 
 
 def merge_profile(prof):
@@ -90,12 +90,13 @@ def tracer_form(prof):
     return "column walk" if "interp_ab_col" in prof else ("bricks" if "interp_ab" in prof else None)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, n):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (FETCH_SIZE x calibration + WRITE_SIZE), or None."""
-    if not os.path.exists(TRAFFIC_FILE):
+    path = TRAFFIC_FILE % n
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         f = line.split()
         if len(f) == 2 and f[0] == kernel:
             return float(f[1])
@@ -206,7 +207,7 @@ def roofline(prof, n, es, np_, nb, traffic_ok, steps):
     else:
         return {"kernel": name, "avg_ms": ms / cnt}
     ach = bytes_per_launch / avg_s
-    traffic = measured_traffic(name) if traffic_ok else None
+    traffic = measured_traffic(name, n) if traffic_ok else None
     out = {"bound": "hbm", "kernel": name, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
            "frac": ach / HBM_PEAK, "traffic": traffic, "avg_ms": ms / cnt, "bytes_per_launch": bytes_per_launch,
            "bytes_note": note, "launches_per_step": cnt / steps}
@@ -524,7 +525,7 @@ def main():
     from bfps_amd import ic
 
     wl = dict(WORKLOADS[args.workload])
-    default_case = args.workload == "512" and args.np is None
+    default_case = args.np is None  # the tracer counts the committed PMC summaries were measured with
     if args.np is not None:
         wl["np"] = args.np
     n, np_ = wl["n"], wl["np"]

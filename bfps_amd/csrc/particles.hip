@@ -892,8 +892,10 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
 // 512^3, n = 6: 5.6 KB per plane slot, (n + 13) planes per walk and column,
 // 4.5 GB per launch against 22 GB for 16 x 8 x 8 bricks.
 // Columns are split into y segments when there are fewer than two columns
-// per CU; consecutive workgroups of an XCD take z-neighbour columns, whose
-// planes overlap in z (the shared rows are read from that XCD's L2).
+// per CU.  Consecutive workgroups of an XCD take x-neighbour columns (x
+// fastest) and walk y at about the same pace: a window row (KX + 2n + 1
+// values) spans three 128-byte lines, its own and one of each x neighbour,
+// so those lines are read from HBM once and then from that XCD's L2.
 // Each slot is a z x x window [WZ][ZS] of the plane, ZS chosen so that the
 // two 16-lane row phases of a tracer (NI rows apart) read disjoint banks.
 #ifndef BFPSH_COL_NT
@@ -945,7 +947,7 @@ struct ColCfg {
 // record for the walk: in-cell fractions (get_norm_pos_in_cell /
 // pbc_field_layer, as k_brick's prologue) and the cell, 10 bits per axis
 __global__ void k_col_keys(const double* __restrict__ pos, int64_t np, PartGeomD pg, int kx, int ky, int kz,
-                           int ncz, int nys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                           int ncx, int nys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                            double* __restrict__ rec) {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= np) return;
@@ -960,7 +962,7 @@ __global__ void k_col_keys(const double* __restrict__ pos, int64_t np, PartGeomD
         fr[d] = (inbox - ci * pg.h[d]) / pg.h[d];
         cell[d] = cell_of(sp, pg.h[d], pg.n);
     }
-    keys[p] = uint32_t(((cell[0] / kx) * ncz + cell[2] / kz) * nys + cell[1] / ky);
+    keys[p] = uint32_t(((cell[2] / kz) * ncx + cell[0] / kx) * nys + cell[1] / ky);
     vals[p] = uint32_t(p);
     double4 r;
     r.x = fr[0];
@@ -976,7 +978,7 @@ __global__ void k_col_keys(const double* __restrict__ pos, int64_t np, PartGeomD
 // the velocity into the head slot of the history ring (k_ab_moves then
 // moves every tracer: a coalesced pass over the history, kept out of the
 // walks so that no workgroup waits on its loads); MODE 1: out[p][NC].
-// Work item wi = seg ncols + col (col = cx ncz + cz) for wi = rank,
+// Work item wi = seg ncols + col (col = cz ncx + cx) for wi = rank,
 // rank + G, ...; per work item NC walks (one per component) over the
 // segment's y steps [s0, s1), whose tracer ranges are read into LDS once
 // per work item.  Per batch, thread t holds tracer t mod 64: its slot and
@@ -989,7 +991,7 @@ template <typename T, int NB, int NC, int MODE>
 __global__ void __launch_bounds__((ColCfg<T, NB>::NT))
 k_column(const T* __restrict__ f, const double* __restrict__ crec, double* __restrict__ out,
          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ kstart, const uint32_t* __restrict__ kend,
-         int ncz, int nys, int ncols, int sps, int nwork, int64_t np, int head, int add_head,
+         int ncx, int nys, int ncols, int sps, int nwork, int64_t np, int head, int add_head,
          const double* __restrict__ tab, int deg, PartGeomD pg) {
     using CC = ColCfg<T, NB>;
     constexpr int W = CC::W, NT = CC::NT, PB = CC::PB, LPT = CC::LPT, RZ = CC::RZ, NI = CC::NI;
@@ -1026,7 +1028,7 @@ k_column(const T* __restrict__ f, const double* __restrict__ crec, double* __res
     auto item = [&](int wi) -> Item {
         const int col = wi % ncols, seg = wi / ncols;
         const int s0 = seg * sps;
-        return Item{(col / ncz) * KX, (col % ncz) * KZ, s0, min(s0 + sps, nys)};
+        return Item{(col % ncx) * KX, (col / ncx) * KZ, s0, min(s0 + sps, nys)};
     };
     auto key_of = [&](int wi, int s) -> uint32_t { return uint32_t(wi % ncols) * uint32_t(nys) + uint32_t(s); };
     // tracer range of step s of work item wi: from the LDS table when wi is
@@ -1420,7 +1422,7 @@ int sort_cols(bfpsh_ctx* ctx, const ColGeom& g) {
     while ((int64_t(1) << nbits) < nk) nbits++;
     HIPCHK(hipMemsetAsync(ctx->d_bstart, 0, 2 * nk * sizeof(uint32_t), ctx->stream));
     LAUNCH("part_keys", (k_col_keys<<<unsigned((np + 255) / 256), 256, 0, ctx->stream>>>(
-                            ctx->d_pos, np, pgeom_d(ctx), CC::KX, CC::KY, CC::KZ, g.ncz, g.nys, ctx->d_keys,
+                            ctx->d_pos, np, pgeom_d(ctx), CC::KX, CC::KY, CC::KZ, g.ncx, g.nys, ctx->d_keys,
                             ctx->d_perm, ctx->d_crec)));
     size_t tmp = ctx->sort_tmp_bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->d_sort_tmp, tmp, ctx->d_keys, ctx->d_keys + np, ctx->d_perm,
@@ -1442,7 +1444,7 @@ int launch_col(bfpsh_ctx* ctx, const ColGeom& g, const void* f, double* d_out, c
     // one workgroup per CU, persistent over its work items
     const unsigned grid = unsigned(std::max(1, std::min(g.nwork, ncu)));
     LAUNCH(name, (kern<<<grid, CC::NT, CC::LDS, ctx->stream>>>(
-                     (const T*)f, ctx->d_crec, d_out, ctx->d_perm + ctx->np, ctx->d_bstart, ctx->d_bstart + nk, g.ncz,
+                     (const T*)f, ctx->d_crec, d_out, ctx->d_perm + ctx->np, ctx->d_bstart, ctx->d_bstart + nk, g.ncx,
                      g.nys, g.ncols, g.sps, g.nwork, ctx->np, ctx->head, ctx->add_head ? 1 : 0, ctx->d_tab, ctx->deg,
                      pgeom_d(ctx))));
     return BFPSH_OK;
