@@ -705,7 +705,10 @@ k_brick(const T* __restrict__ f, double* __restrict__ pos, double* __restrict__ 
         }
     }
     if constexpr (SW) {
-        for (int i = tid; i < W * (deg + 1); i += NT) tabl[i] = tab[i];  // visible after the first barrier
+        // the first batch's prologue evaluates weights from the table before
+        // the loop's first barrier: the copy must be whole before it starts
+        for (int i = tid; i < W * (deg + 1); i += NT) tabl[i] = tab[i];
+        __syncthreads();
     }
     int comp = 0, buf = 0;
     fill(cb.brick, 0, 0);
@@ -939,7 +942,7 @@ struct ColCfg {
 #ifdef BFPSH_NO_COLUMNS
     static constexpr bool ok = false;
 #else
-    static constexpr bool ok = sizeof(T) == 4 && KY >= 4 && W <= 16 && CH <= NT / 64 && W % RZ == 0;
+    static constexpr bool ok = KY >= 4 && W <= 16 && CH <= NT / 64 && W % RZ == 0;
 #endif
 };
 
@@ -1019,7 +1022,8 @@ k_column(const T* __restrict__ f, const double* __restrict__ crec, double* __res
     // offsets within a plane fit 32 bits for n <= 1024)
     const size_t ps = 2 * plane_size(n);
     const uint32_t ys = uint32_t(2 * row_stride(n));
-    for (int i = tid; i < W * (deg + 1); i += NT) tabl[i] = tab[i];  // visible after the first barrier
+    // the weight table: first read after the first walk's opening barrier
+    for (int i = tid; i < W * (deg + 1); i += NT) tabl[i] = tab[i];
 
     // work item -> column origin and y steps
     struct Item {

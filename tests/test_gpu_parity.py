@@ -319,18 +319,19 @@ def test_tracers_fp32_bricks(B, oracle, n, nb, m):
     assert np.abs(sg - og).max() < 1e-5 * np.abs(og).max()
 
 
-def _fp32_tracer_run(B, n, nb, m, pos, kernel, monkeypatch, seed=21, steps=3, w0=None):
-    """FP32 box with tracers advanced `steps` NSVEparticles steps by the
+def _fp32_tracer_run(B, n, nb, m, pos, kernel, monkeypatch, seed=21, steps=3, precision="single"):
+    """A box with tracers advanced `steps` NSVEparticles steps by the
     velocity-loop kernel `kernel` ("auto": the column walk for these boxes,
-    "brick": independent bricks); returns (positions, history, step index)."""
+    "brick": independent bricks), then sampled (velocity, gradient);
+    returns (positions, history, step index, velocity, gradient)."""
     from bfps_amd import ic
     if kernel == "auto":
         monkeypatch.delenv("BFPSH_TRACER_KERNEL", raising=False)
     else:
         monkeypatch.setenv("BFPSH_TRACER_KERNEL", kernel)
-    fs = B.vorticity_equation("t", n, precision="single")
+    fs = B.vorticity_equation("t", n, precision=precision)
     fs.nu, fs.forcing_type = 0.01, "linear"
-    fs.io_checkpoint(read=True, data=ic.solenoidal_vorticity(n, seed=seed, k0=4.0) if w0 is None else w0)
+    fs.io_checkpoint(read=True, data=ic.solenoidal_vorticity(n, seed=seed, k0=4.0, dtype=fs.cdtype))
     ps = B.particles_system(fs, len(pos), nb, m, 4, step_idx=1)
     ps.put(pos)
     monkeypatch.delenv("BFPSH_TRACER_KERNEL", raising=False)
@@ -400,6 +401,71 @@ def test_tracers_fp32_columns(B, oracle, monkeypatch, n, nb, m):
         assert osi == gsi
         assert np.abs(grhs - orhs).max() < 1e-5 * max(1.0, np.abs(orhs).max())
         assert np.abs(gp - opos).max() < 1e-7
+
+
+@pytest.mark.parametrize("precision,n,nb,m", [("double", 48, 4, 1), ("single", 48, 4, 1), ("double", 32, 6, 2)])
+def test_brick_first_batches_vs_gather(B, monkeypatch, precision, n, nb, m):
+    """Regression test for the brick kernel's weight table: every workgroup's
+    first batch evaluates its weights from the table copied into LDS, so the
+    copy must be complete before that batch's prologue.  Boxes with fewer
+    bricks than workgroups make every batch a first batch; 20 steps with the
+    field passes (other kernels' LDS contents) in between; the brick kernel's
+    trajectories must match the global-gather kernel's."""
+    from bfps_amd import ic
+    say("case: first batches %s n=%d nb=%d m=%d" % (precision, n, nb, m))
+    pos = ic.uniform_positions(30000, seed=77 + nb)
+    out = {}
+    for kern in ("brick", "gather"):
+        monkeypatch.setenv("BFPSH_TRACER_KERNEL", kern)
+        fs = B.vorticity_equation("t", n, precision=precision)
+        fs.nu, fs.forcing_type = 0.01, "linear"
+        fs.io_checkpoint(read=True, data=ic.solenoidal_vorticity(n, seed=3, k0=3.0, dtype=fs.cdtype))
+        ps = B.particles_system(fs, len(pos), nb, m, 4, step_idx=1)
+        ps.put(pos)
+        monkeypatch.delenv("BFPSH_TRACER_KERNEL", raising=False)
+        for _ in range(20):
+            B.NSVEparticles_step(fs, 0.5 / n)
+        out[kern] = ps.get()
+        fs.close()
+    d = np.abs(out["brick"][0] - out["gather"][0]).max()
+    r = np.abs(out["brick"][1] - out["gather"][1]).max() / np.abs(out["gather"][1]).max()
+    say("  brick vs gather: pos %.3e rhs %.3e" % (d, r))
+    tol = 1e-12 if precision == "double" else 1e-9
+    assert d < tol and r < tol
+
+
+@pytest.mark.parametrize("n,nb,m", [(64, 1, 1), (128, 2, 2), (128, 3, 0), (64, 3, 1)])
+def test_tracers_fp64_columns(B, oracle, monkeypatch, n, nb, m):
+    """FP64 boxes of 64^3 and more with n <= 3 run the column walk too (FP64
+    planes in the LDS ring): against the brick kernel on the same case and
+    against the FP64 oracle, at the FP64 tracer tolerance (1e-12)."""
+    from bfps_amd import ic
+    say("case: fp64 column walk n=%d nb=%d m=%d" % (n, nb, m))
+    npart = 20000
+    pos = ic.uniform_positions(npart, seed=40 + nb)
+    pos[:20] -= 2 * np.pi
+    gp, grhs, gsi, gv, gg = _fp32_tracer_run(B, n, nb, m, pos, "auto", monkeypatch, precision="double")
+    bp, brhs, bsi, bv, bg = _fp32_tracer_run(B, n, nb, m, pos, "brick", monkeypatch, precision="double")
+    say("  column vs brick: pos %.3e rhs %.3e velocity %.3e gradient %.3e"
+        % (np.abs(gp - bp).max(), np.abs(grhs - brhs).max(), np.abs(gv - bv).max(), np.abs(gg - bg).max()))
+    assert gsi == bsi == 4
+    assert np.abs(grhs - brhs).max() <= 1e-13 * max(1.0, np.abs(brhs).max())
+    assert np.abs(gp - bp).max() <= 1e-13 * max(1.0, np.abs(bp).max())
+    assert np.abs(gv - bv).max() <= 1e-13 * np.abs(bv).max()
+    assert np.abs(gg - bg).max() <= 1e-13 * np.abs(bg).max()
+    o = oracle.Solver(n, precision="double")
+    o.set_physics(0.01, "linear", 1.0, 2.0, 4.0)
+    o.set_cfield("vorticity", ic.solenoidal_vorticity(n, seed=21, k0=4.0, dtype=np.complex128))
+    o.io_projection()
+    op = oracle.Particles(o, npart, nb, m, 4, step_idx=1)
+    op.set_state(pos)
+    for _ in range(3):
+        oracle.nsve_particles_step(o, op, 0.5 / n)
+    opos, orhs, osi = op.get_state()
+    say("  column vs oracle: pos %.3e rhs %.3e" % (np.abs(gp - opos).max(), np.abs(grhs - orhs).max()))
+    assert osi == gsi
+    assert np.abs(gp - opos).max() < 1e-12
+    assert np.abs(grhs - orhs).max() < 1e-12 * max(1.0, np.abs(orhs).max())
 
 
 @pytest.mark.parametrize("precision,k,inband", [("double", (1, 2, 0), True), ("double", (2, -1, 3), True),
