@@ -72,22 +72,28 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_%d_fp32.txt")
 def merge_profile(prof):
     """Fold the kx-tile-0 launch of the Biot-Savart pass into "inv_z_bs";
     the tracer velocity loop is "interp_ab" whichever kernel form ran (the
-    column walk is profiled as "interp_ab_col")."""
+    column walk and the global gather are profiled as "interp_ab_col" and
+    "interp_ab_gather")."""
     out = dict(prof)
     if "inv_z_bs_t0" in out and "inv_z_bs" in out:
         t0_ms, _ = out.pop("inv_z_bs_t0")
         ms, cnt = out["inv_z_bs"]
         out["inv_z_bs"] = (ms + t0_ms, cnt)
-    if "interp_ab_col" in out:
-        ms, cnt = out.pop("interp_ab_col")
-        a, b = out.get("interp_ab", (0.0, 0))
-        out["interp_ab"] = (ms + a, cnt + b)
+    for form in ("interp_ab_col", "interp_ab_gather"):
+        if form in out:
+            ms, cnt = out.pop(form)
+            a, b = out.get("interp_ab", (0.0, 0))
+            out["interp_ab"] = (ms + a, cnt + b)
     return out
 
 
 def tracer_form(prof):
     """Kernel form of the tracer velocity loop in a raw profile."""
-    return "column walk" if "interp_ab_col" in prof else ("bricks" if "interp_ab" in prof else None)
+    if "interp_ab_col" in prof:
+        return "column walk"
+    if "interp_ab_gather" in prof:
+        return "global gather"
+    return "bricks" if "interp_ab" in prof else None
 
 
 def measured_traffic(kernel, n):

@@ -654,7 +654,12 @@ int bfpsh_particles_create(bfpsh_ctx* ctx, int64_t np, int n, int m, int nsteps,
         // kernel choice for the velocity loop: automatic, or forced to one
         // form (the tests run each form on the same case)
         const char* tk = std::getenv("BFPSH_TRACER_KERNEL");
-        ctx->tracer_kernel = !tk ? 0 : (!std::strcmp(tk, "brick") ? 1 : (!std::strcmp(tk, "gather") ? 2 : 0));
+        ctx->tracer_kernel = !tk ? 0
+                             : !std::strcmp(tk, "brick")  ? 1
+                             : !std::strcmp(tk, "gather") ? 2
+                             : !std::strcmp(tk, "column") ? 3
+                                                          : 0;
+        ctx->tracer_form = 0;
     }
     ctx->nsteps = nsteps;
     ctx->step_idx = step_idx;

@@ -31,7 +31,8 @@ struct bfpsh_ctx {
     bool own_stream = true;        // false: the stream of an ensemble
     void* slab = nullptr;          // ensemble member: v0|v1|v2|work(6)|ru live here
     bool force_coop = false;       // tracers: global-gather kernel, no per-step sort
-    int tracer_kernel = 0;         // velocity loop: 0 automatic, 1 bricks only, 2 global gather only
+    int tracer_kernel = 0;         // velocity loop and samples: 0 automatic, forced 1 bricks, 2 global gather, 3 column walk
+    int tracer_form = 0;           // the form chosen for this tracer set (0: not yet)
     // device buffers (element size es = sizeof(rnumber))
     size_t es = 4;
     size_t ps = 0;                 // complex elements per plane, row pads included (plane_size)

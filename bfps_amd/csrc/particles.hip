@@ -1392,7 +1392,8 @@ bool col_geom(const bfpsh_ctx* ctx, ColGeom& g) {
     static_assert(CC::ok, "column walk geometry of a configuration that has none");
     const int n = ctx->n;
     // (cells are packed 10 bits per axis)
-    if (ctx->force_coop || ctx->tracer_kernel != 0 || n < 64 || n > 1024 || n % CC::KX || n % CC::KZ || n % CC::KY)
+    if (ctx->force_coop || (ctx->tracer_kernel != 0 && ctx->tracer_kernel != 3) || n < 64 || n > 1024 ||
+        n % CC::KX || n % CC::KZ || n % CC::KY)
         return false;
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) return false;
@@ -1454,43 +1455,30 @@ int launch_col(bfpsh_ctx* ctx, const ColGeom& g, const void* f, double* d_out, c
     return BFPSH_OK;
 }
 
-template <typename T, int NB>
-int loop_nb(bfpsh_ctx* ctx, double dt, int order) {
-    if constexpr (ColCfg<T, NB>::ok) {
-        ColGeom g;
-        if (col_geom<T, NB>(ctx, g)) {
-            int r = sort_cols<T, NB>(ctx, g);
-            if (r) return r;
-            r = launch_col<T, NB, 3, 0>(ctx, g, ctx->ru, ctx->d_rhs, "interp_ab_col");
-            if (r) return r;
-            LAUNCH("ab_moves", (k_ab_moves<<<unsigned((ctx->np + 255) / 256), 256, 0, ctx->stream>>>(
-                                   ctx->d_pos, ctx->d_rhs, ctx->np, ctx->nsteps, ctx->head, order, dt)));
-            return BFPSH_OK;
-        }
-    }
-    if constexpr (BrickCfg<T, NB>::ok) {
-        if (ctx->force_coop || ctx->tracer_kernel == 2) goto coop;
-        int r = sort_bricks<T, NB>(ctx);
-        if (r) return r;
-        return launch_brick<T, NB, 3, 0>(ctx, ctx->ru, nullptr, dt, order, "interp_ab");
-    }
-coop:
-    if (ctx->sort_age >= ctx->sort_every) {
-        int r = sort_tracers(ctx);
-        if (r) return r;
-    }
-    const size_t lds = size_t(2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
-    LAUNCH("interp_ab", (k_interp_ab<T, NB><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
-                            (const T*)ctx->ru, ctx->d_pos, ctx->d_rhs, ctx->d_perm + ctx->np, ctx->np, ctx->nsteps,
-                            ctx->head, order, ctx->add_head ? 1 : 0, dt, ctx->d_tab, ctx->deg, pgeom_d(ctx))));
-    return BFPSH_OK;
-}
+// Kernel forms of the velocity loop and the samples (BFPSH_TRACER_KERNEL
+// forces one: brick, gather or column).
+enum { FORM_AUTO = 0, FORM_BRICK = 1, FORM_GATHER = 2, FORM_COLUMN = 3 };
 
 template <typename T, int NB>
-int sample_nb(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
+bool form_ok(const bfpsh_ctx* ctx, int form) {
+    if (form == FORM_COLUMN) {
+        if constexpr (ColCfg<T, NB>::ok) {
+            ColGeom g;
+            return col_geom<T, NB>(ctx, g);
+        }
+        return false;
+    }
+    if (form == FORM_BRICK) return BrickCfg<T, NB>::ok && !ctx->force_coop;
+    return true;
+}
+
+// nc components of f at every tracer by kernel form `form` (applicable)
+template <typename T, int NB>
+int sample_form(bfpsh_ctx* ctx, int form, const void* f, int nc, double* d_out) {
     if constexpr (ColCfg<T, NB>::ok) {
-        ColGeom g;
-        if ((nc == 3 || nc == 9) && col_geom<T, NB>(ctx, g)) {
+        if (form == FORM_COLUMN && (nc == 3 || nc == 9)) {
+            ColGeom g;
+            col_geom<T, NB>(ctx, g);
             int r = sort_cols<T, NB>(ctx, g);
             if (r) return r;
             if (nc == 3) return launch_col<T, NB, 3, 1>(ctx, g, f, d_out, "sample_col");
@@ -1498,7 +1486,7 @@ int sample_nb(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
         }
     }
     if constexpr (BrickCfg<T, NB>::ok) {
-        if (ctx->tracer_kernel != 2) {
+        if (form != FORM_GATHER && !ctx->force_coop) {
             int r = sort_bricks<T, NB>(ctx);
             if (r) return r;
             if constexpr (sizeof(T) == 8)
@@ -1514,20 +1502,123 @@ int sample_nb(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
     const size_t lds = size_t(2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
     if constexpr (sizeof(T) == 8)
         if (nc == 1) {
-            LAUNCH("sample1", (k_sample<T, NB, 1><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
-                                  (const T*)f, ctx->d_pos, d_out, ctx->d_perm + ctx->np, ctx->np, ctx->d_tab, ctx->deg,
-                                  pgeom_d(ctx))));
+            LAUNCH("sample1_gather", (k_sample<T, NB, 1><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds,
+                                                          ctx->stream>>>((const T*)f, ctx->d_pos, d_out,
+                                                                         ctx->d_perm + ctx->np, ctx->np, ctx->d_tab,
+                                                                         ctx->deg, pgeom_d(ctx))));
             return BFPSH_OK;
         }
     if (nc == 3)
-        LAUNCH("sample", (k_sample<T, NB, 3><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
-                             (const T*)f, ctx->d_pos, d_out, ctx->d_perm + ctx->np, ctx->np, ctx->d_tab, ctx->deg,
-                             pgeom_d(ctx))));
+        LAUNCH("sample_gather", (k_sample<T, NB, 3><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
+                                    (const T*)f, ctx->d_pos, d_out, ctx->d_perm + ctx->np, ctx->np, ctx->d_tab, ctx->deg,
+                                    pgeom_d(ctx))));
     else
-        LAUNCH("sample9", (k_sample<T, NB, 9><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
-                              (const T*)f, ctx->d_pos, d_out, ctx->d_perm + ctx->np, ctx->np, ctx->d_tab, ctx->deg,
-                              pgeom_d(ctx))));
+        LAUNCH("sample9_gather", (k_sample<T, NB, 9><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds,
+                                                      ctx->stream>>>((const T*)f, ctx->d_pos, d_out,
+                                                                     ctx->d_perm + ctx->np, ctx->np, ctx->d_tab,
+                                                                     ctx->deg, pgeom_d(ctx))));
     return BFPSH_OK;
+}
+
+// The kernel form of a box's velocity loop and samples: the forced one if it
+// applies; else, once per tracer set, the fastest applicable form timed on
+// a 3-component sample of f (plan rigor MEASURE or PATIENT, as the pass
+// autotuner), or the first applicable of column walk, bricks, global
+// gather (ESTIMATE).
+template <typename T, int NB>
+int tracer_form(bfpsh_ctx* ctx, const void* f, int* form) {
+    if (ctx->tracer_kernel != FORM_AUTO && form_ok<T, NB>(ctx, ctx->tracer_kernel)) {
+        *form = ctx->tracer_kernel;
+        return BFPSH_OK;
+    }
+    if (ctx->tracer_form != FORM_AUTO) {
+        *form = ctx->tracer_form;
+        return BFPSH_OK;
+    }
+    int cand[3], ncand = 0;
+    for (int c : {int(FORM_COLUMN), int(FORM_BRICK), int(FORM_GATHER)})
+        if (form_ok<T, NB>(ctx, c)) cand[ncand++] = c;
+    int best = cand[0];
+    if (ncand > 1 && ctx->plan_rigor != BFPSH_ESTIMATE && ctx->tracer_kernel == FORM_AUTO) {
+        double* tmp = nullptr;
+        hipEvent_t a = nullptr, b = nullptr;
+        HIPCHK(hipMalloc((void**)&tmp, size_t(ctx->np) * 3 * sizeof(double)));
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        const bool prof = ctx->prof;
+        ctx->prof = false;  // the trials stay out of the kernel profile
+        double tbest = 1e300;
+        int r = BFPSH_OK;
+        for (int i = 0; i < ncand && r == BFPSH_OK; i++) {
+            r = sample_form<T, NB>(ctx, cand[i], f, 3, tmp);  // warm-up
+            if (r == BFPSH_OK && hipEventRecord(a, ctx->stream) != hipSuccess) r = BFPSH_EHIP;
+            if (r == BFPSH_OK) r = sample_form<T, NB>(ctx, cand[i], f, 3, tmp);
+            float ms = 0.f;
+            if (r == BFPSH_OK && (hipEventRecord(b, ctx->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                                  hipEventElapsedTime(&ms, a, b) != hipSuccess))
+                r = BFPSH_EHIP;
+            if (r == BFPSH_OK && ms < tbest) {
+                tbest = ms;
+                best = cand[i];
+            }
+        }
+        ctx->prof = prof;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        (void)hipFree(tmp);
+        if (r) return r == BFPSH_EHIP ? fail(ctx, r, "tracer kernel choice: HIP error") : r;
+        ctx->sort_age = 1 << 30;  // the trials' sorts leave another order behind
+    }
+    ctx->tracer_form = best;
+    *form = best;
+    return BFPSH_OK;
+}
+
+template <typename T, int NB>
+int loop_nb(bfpsh_ctx* ctx, double dt, int order) {
+    int form = FORM_GATHER;
+    {
+        int r = tracer_form<T, NB>(ctx, ctx->ru, &form);
+        if (r) return r;
+    }
+    if constexpr (ColCfg<T, NB>::ok) {
+        if (form == FORM_COLUMN) {
+            ColGeom g;
+            col_geom<T, NB>(ctx, g);
+            int r = sort_cols<T, NB>(ctx, g);
+            if (r) return r;
+            r = launch_col<T, NB, 3, 0>(ctx, g, ctx->ru, ctx->d_rhs, "interp_ab_col");
+            if (r) return r;
+            LAUNCH("ab_moves", (k_ab_moves<<<unsigned((ctx->np + 255) / 256), 256, 0, ctx->stream>>>(
+                                   ctx->d_pos, ctx->d_rhs, ctx->np, ctx->nsteps, ctx->head, order, dt)));
+            return BFPSH_OK;
+        }
+    }
+    if constexpr (BrickCfg<T, NB>::ok) {
+        if (form == FORM_BRICK) {
+            int r = sort_bricks<T, NB>(ctx);
+            if (r) return r;
+            return launch_brick<T, NB, 3, 0>(ctx, ctx->ru, nullptr, dt, order, "interp_ab");
+        }
+    }
+    if (ctx->sort_age >= ctx->sort_every) {
+        int r = sort_tracers(ctx);
+        if (r) return r;
+    }
+    const size_t lds = size_t(2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
+    LAUNCH("interp_ab_gather", (k_interp_ab<T, NB><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
+                                   (const T*)ctx->ru, ctx->d_pos, ctx->d_rhs, ctx->d_perm + ctx->np, ctx->np,
+                                   ctx->nsteps, ctx->head, order, ctx->add_head ? 1 : 0, dt, ctx->d_tab, ctx->deg,
+                                   pgeom_d(ctx))));
+    return BFPSH_OK;
+}
+
+template <typename T, int NB>
+int sample_nb(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
+    int form = FORM_GATHER;
+    int r = tracer_form<T, NB>(ctx, f, &form);
+    if (r) return r;
+    return sample_form<T, NB>(ctx, form, f, nc, d_out);
 }
 
 template <typename T>

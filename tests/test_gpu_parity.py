@@ -321,14 +321,11 @@ def test_tracers_fp32_bricks(B, oracle, n, nb, m):
 
 def _fp32_tracer_run(B, n, nb, m, pos, kernel, monkeypatch, seed=21, steps=3, precision="single"):
     """A box with tracers advanced `steps` NSVEparticles steps by the
-    velocity-loop kernel `kernel` ("auto": the column walk for these boxes,
-    "brick": independent bricks), then sampled (velocity, gradient);
+    velocity-loop kernel `kernel` (forced: "column" walk or "brick"s), then
+    sampled (velocity, gradient);
     returns (positions, history, step index, velocity, gradient)."""
     from bfps_amd import ic
-    if kernel == "auto":
-        monkeypatch.delenv("BFPSH_TRACER_KERNEL", raising=False)
-    else:
-        monkeypatch.setenv("BFPSH_TRACER_KERNEL", kernel)
+    monkeypatch.setenv("BFPSH_TRACER_KERNEL", kernel)
     fs = B.vorticity_equation("t", n, precision=precision)
     fs.nu, fs.forcing_type = 0.01, "linear"
     fs.io_checkpoint(read=True, data=ic.solenoidal_vorticity(n, seed=seed, k0=4.0, dtype=fs.cdtype))
@@ -345,7 +342,7 @@ def _fp32_tracer_run(B, n, nb, m, pos, kernel, monkeypatch, seed=21, steps=3, pr
     sg = ps.sample("gradient")
     prof = fs.profile_report()
     # the column walk is profiled as *_col, bricks without the suffix
-    col = kernel == "auto"
+    col = kernel == "column"
     assert ("interp_ab_col" in prof) == col and ("interp_ab" in prof) == (not col)
     assert ("sample_col" in prof) == col and ("sample9_col" in prof) == col
     gp, grhs, _, gsi = ps.get()
@@ -376,7 +373,7 @@ def test_tracers_fp32_columns(B, oracle, monkeypatch, n, nb, m):
     pos[1000:4000] = np.array([5.5, 40.0, 9.5]) * h + rng.random((3000, 3)) * np.array([4, 8, 4]) * h
     # exactly on cell faces
     pos[4000:4100] = np.round(pos[4000:4100] / h) * h
-    gp, grhs, gsi, gv, gg = _fp32_tracer_run(B, n, nb, m, pos, "auto", monkeypatch)
+    gp, grhs, gsi, gv, gg = _fp32_tracer_run(B, n, nb, m, pos, "column", monkeypatch)
     bp, brhs, bsi, bv, bg = _fp32_tracer_run(B, n, nb, m, pos, "brick", monkeypatch)
     scale = max(1.0, np.abs(brhs).max())
     say("  column vs brick: pos %.3e rhs %.3e velocity %.3e gradient %.3e"
@@ -444,7 +441,7 @@ def test_tracers_fp64_columns(B, oracle, monkeypatch, n, nb, m):
     npart = 20000
     pos = ic.uniform_positions(npart, seed=40 + nb)
     pos[:20] -= 2 * np.pi
-    gp, grhs, gsi, gv, gg = _fp32_tracer_run(B, n, nb, m, pos, "auto", monkeypatch, precision="double")
+    gp, grhs, gsi, gv, gg = _fp32_tracer_run(B, n, nb, m, pos, "column", monkeypatch, precision="double")
     bp, brhs, bsi, bv, bg = _fp32_tracer_run(B, n, nb, m, pos, "brick", monkeypatch, precision="double")
     say("  column vs brick: pos %.3e rhs %.3e velocity %.3e gradient %.3e"
         % (np.abs(gp - bp).max(), np.abs(grhs - brhs).max(), np.abs(gv - bv).max(), np.abs(gg - bg).max()))
