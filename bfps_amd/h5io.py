@@ -19,8 +19,10 @@ the HDF5 file format those files use:
   of floats that holds complex numbers), fill value, data layout v3
   (contiguous or compact) and modification-time messages; header
   continuations when reading.  Strings are written fixed-length (a Python
-  str as ASCII bytes); variable-length strings and chunked (extendable)
-  datasets are not read.
+  str as ASCII bytes).  Reading also takes what h5py writes into the
+  reference launcher's run file: variable-length strings (global heap
+  collections) and chunked, unfiltered datasets (data layout v3 class 2,
+  v1 B-tree of chunks), such as its extendable statistics series.
 
 The writer lays files out as the C library does for these objects (the
 same message set, flags and encodings, dataspaces with maximum dimensions,
@@ -47,6 +49,12 @@ INTERNAL_K = 16   # B-tree node holds up to 2 INTERNAL_K children
 
 class H5FormatError(Exception):
     pass
+
+
+# variable-length strings: read as Python bytes (an object array), stored as
+# (length, global heap collection address, object index) = 16 bytes
+VLEN_STR = np.dtype(object)
+VLEN_BYTES = 16
 
 
 # ---------------------------------------------------------------- datatypes
@@ -102,6 +110,11 @@ def _decode_datatype(b, off):
         return np.dtype("<f%d" % size), 8 + 12
     if cls == 3:  # fixed-length string (any padding, ASCII or UTF-8)
         return np.dtype("S%d" % size), 8
+    if cls == 9:  # variable-length: strings only (h5py's str), read from the global heap
+        if bits[0] & 0x0F != 1:
+            raise H5FormatError("variable-length sequences")
+        blen = _decode_datatype(b, off + 8)[1]
+        return VLEN_STR, 8 + blen
     if cls == 6:  # compound: {r, i} floats -> complex
         nmemb = bits[0] | (bits[1] << 8)
         p = off + 8
@@ -144,9 +157,10 @@ class _Group(dict):
 
 
 class _Dataset:
-    def __init__(self, dtype, shape, address=None, data=None, compact=None):
+    def __init__(self, dtype, shape, address=None, data=None, compact=None, chunks=None):
         self.dtype, self.shape = np.dtype(dtype), tuple(int(s) for s in shape)
         self.address, self.data, self.compact = address, data, compact
+        self.chunks = chunks  # (B-tree address, chunk shape) of chunked storage
 
 
 def _u(b, off, n):
@@ -221,14 +235,25 @@ class _Parser:
                 elif body[1] == 0:
                     n = _u(body, 2, 2)
                     layout = ("compact", bytes(body[4:4 + n]))
+                elif body[1] == 2:
+                    # chunked: dimensionality (rank + 1: the last "dimension"
+                    # is the element size), v1 B-tree address, chunk sizes
+                    nd = body[2]
+                    dims = tuple(_u(body, 11 + 4 * i, 4) for i in range(nd))
+                    layout = ("chunked", _u(body, 3, 8), dims[:-1])
                 else:
-                    raise H5FormatError("layout class %d (chunked data)" % body[1])
-            elif t in (0x0B, 0x0C) or t > 0x18:
+                    raise H5FormatError("layout class %d" % body[1])
+            elif t == 0x0B:
+                if body[1] != 0:  # number of filters (compression is outside the subset)
+                    raise H5FormatError("filtered (compressed) dataset")
+            elif t == 0x0C or t > 0x18:
                 raise H5FormatError("object header message 0x%x" % t)
         if dtype is None or shape is None or layout is None:
             raise H5FormatError("object at %d is neither a group nor a dataset" % addr)
         if layout[0] == "compact":
             ds = _Dataset(dtype, shape, compact=layout[1])
+        elif layout[0] == "chunked":
+            ds = _Dataset(dtype, shape, chunks=(layout[1], layout[2]))
         else:
             ds = _Dataset(dtype, shape, address=None if layout[1] == UNDEF else layout[1])
         ds.hdr = addr
@@ -273,14 +298,80 @@ class _Parser:
             out[names[noff:end].decode()] = (obj, addr + 8 + 40 * e + 24)
 
     def data(self, ds):
-        n = int(np.prod(ds.shape, dtype=np.int64)) * ds.dtype.itemsize
-        if ds.compact is not None:
-            raw = ds.compact[:n]
+        vlen = ds.dtype == VLEN_STR
+        est = VLEN_BYTES if vlen else ds.dtype.itemsize
+        sdt = np.dtype("V%d" % VLEN_BYTES) if vlen else ds.dtype  # storage elements
+        n = int(np.prod(ds.shape, dtype=np.int64)) * est
+        if ds.chunks is not None:
+            out = np.zeros(ds.shape, sdt)
+            if ds.chunks[0] != UNDEF:
+                self._chunks(ds.chunks[0], ds.chunks[1], est, sdt, out)
+        elif ds.compact is not None:
+            out = np.frombuffer(ds.compact[:n], sdt).reshape(ds.shape).copy()
         elif ds.address is None:
-            return np.zeros(ds.shape, ds.dtype)
+            out = np.zeros(ds.shape, sdt)
         else:
-            raw = self.read(ds.address, n)
-        return np.frombuffer(raw, ds.dtype).reshape(ds.shape).copy()
+            out = np.frombuffer(self.read(ds.address, n), sdt).reshape(ds.shape).copy()
+        return self._vlen_strings(out) if vlen else out
+
+    def _chunks(self, addr, cshape, est, sdt, out):
+        """Copy the chunks under v1 B-tree node addr (node type 1: raw data
+        chunks; keys: chunk size, filter mask, chunk offsets) into out."""
+        rank = len(cshape)
+        h = self.read(addr, 24)
+        if h[:4] != b"TREE" or h[4] != 1:
+            raise H5FormatError("chunk B-tree node")
+        level, used = h[5], _u(h, 6, 2)
+        ksize = 8 + 8 * (rank + 1)
+        body = self.read(addr + 24, used * (ksize + 8) + ksize)
+        nbytes = int(np.prod(cshape, dtype=np.int64)) * est
+        for i in range(used):
+            k = i * (ksize + 8)
+            child = _u(body, k + ksize, 8)
+            if level > 0:
+                self._chunks(child, cshape, est, sdt, out)
+                continue
+            if _u(body, k, 4) != nbytes or _u(body, k + 4, 4) != 0:
+                raise H5FormatError("filtered or partial chunk")
+            off = [_u(body, k + 8 + 8 * d, 8) for d in range(rank)]
+            chunk = np.frombuffer(self.read(child, nbytes), sdt).reshape(cshape)
+            dst = tuple(slice(o, min(o + c, s)) for o, c, s in zip(off, cshape, out.shape))
+            src = tuple(slice(0, x.stop - x.start) for x in dst)
+            out[dst] = chunk[src]
+
+    def _vlen_strings(self, raw):
+        """Variable-length strings: each element (length, global heap
+        collection address, object index) names an object of a collection."""
+        b = raw.tobytes()
+        out = np.empty(raw.shape, dtype=object)
+        flat = out.reshape(-1)
+        cache = getattr(self, "_gcol", None)
+        if cache is None:
+            cache = self._gcol = {}
+        for i in range(flat.size):
+            length, coll, idx = _u(b, 16 * i, 4), _u(b, 16 * i + 4, 8), _u(b, 16 * i + 12, 4)
+            if coll == 0 or coll == UNDEF:
+                flat[i] = b""
+                continue
+            if coll not in cache:
+                cache[coll] = self._collection(coll)
+            flat[i] = cache[coll][idx][:length]
+        return out[()] if out.ndim == 0 else out
+
+    def _collection(self, addr):
+        h = self.read(addr, 16)
+        if h[:4] != b"GCOL":
+            raise H5FormatError("global heap collection signature")
+        size = _u(h, 8, 8)
+        b = self.read(addr, size)
+        objs, p = {}, 16
+        while p + 16 <= size:
+            idx, osize = _u(b, p, 2), _u(b, p + 8, 8)
+            if idx == 0:
+                break
+            objs[idx] = bytes(b[p + 16:p + 16 + osize])
+            p += 16 + _pad8(osize)
+        return objs
 
 
 # ---------------------------------------------------------------- writing
