@@ -104,6 +104,15 @@ struct Cfg {
     // threads of the LDS-tile row passes (1024: two blocks of 512 per CU,
     // four waves per SIMD, instead of two blocks of 256)
     static constexpr int NT_ROW = N >= 1024 ? 512 : (N >= 256 ? 256 : (N >= 64 ? 128 : 64));
+    // LDS row stride of the LDS-tile cross-product pass (6 RP components per
+    // row): odd, spreading strided lanes over the banks, unless that slot
+    // costs a resident workgroup per CU (1024^3 FP32: three 48 KB
+    // workgroups instead of two of 56 KB, 14.4 -> 12.0 ms per launch)
+    static constexpr int ls_row() {
+        const long c = 6 * RP, lds = 160 * 1024, row = long(N) * long(sizeof(C2<T>));
+        return lds / (row * c) > lds / (row * (c + 1)) ? int(c) : int(c + 1);
+    }
+    static constexpr int LS_ROW = ls_row();
     static_assert(NT <= 1024, "tile threads");
 };
 
@@ -457,7 +466,7 @@ __global__ void __launch_bounds__((Cfg<T, N>::NT_ROW))
 k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restrict__ tw, T npoints, size_t bstride) {
     using CF = Cfg<T, N>;
     // LDS row stride: odd, so lanes striding by rows hit distinct banks
-    constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = 6 * RP + 1;
+    constexpr int RP = CF::RP, NT = CF::NT_ROW, NXH = CF::NXH, LS = CF::LS_ROW;
     constexpr size_t PS = plane_size(N), YS = row_stride(N);
     __shared__ C2<T> lds[N * LS];
     // box blockIdx.y of a batch (complex planes and real velocity move together)

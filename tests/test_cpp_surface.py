@@ -40,8 +40,12 @@ def _driver():
 
 
 @pytest.mark.gpu
-def test_driver_matches_python_and_oracle(tmp_path, oracle):
+def test_driver_matches_python_and_oracle(tmp_path, oracle, monkeypatch):
     import bfps_amd as B
+    # one tracer kernel form in both processes: the timed choice between
+    # forms could differ between them where the forms run about as fast,
+    # and the forms agree to rounding, not bit for bit
+    monkeypatch.setenv("BFPSH_TRACER_KERNEL", "brick")
     from bfps_amd import ic
     n, np_, nsteps, nb, sm, niter, it0 = 32, 2000, 4, 2, 1, 6, 0
     nu, famp, fk0, fk1, dt = (2.0 * 2 / 32) ** (4. / 3), 0.5, 2.0, 4.0, 0.5 / 32

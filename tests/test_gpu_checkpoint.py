@@ -194,7 +194,7 @@ def test_reference_test_flow_through_launch_jobs(B, tmp_path):
         assert int(f["iteration"]) == 64 and f["statistics/moments/velocity"].shape == (65, 10, 4)
 
 
-def test_cpp_job_reproduces_the_reference_test(B, tmp_path):
+def test_cpp_job_reproduces_the_reference_test(B, tmp_path, monkeypatch):
     """The reference's test flow with the job itself in C++
     (tests/cpp/nsve_particles_job.cpp: direct_numerical_simulation, NSVE and
     NSVEparticles call for call over include/bfps_hip.hpp, files through
@@ -208,6 +208,9 @@ def test_cpp_job_reproduces_the_reference_test(B, tmp_path):
     from bfps_amd import build as hb
     from bfps_amd import dns, h5io
     say("case: C++ NSVEparticles job, 2 jobs x 32 iterations from the golden file")
+    # one tracer kernel form in both processes (bit-for-bit comparison below;
+    # the forms agree to rounding only)
+    monkeypatch.setenv("BFPSH_TRACER_KERNEL", "brick")
     job = os.path.join(HERE, "cpp", "nsve_particles_job")
     if not os.path.exists(job):
         hb.build_driver()
