@@ -590,6 +590,7 @@ def main():
                        "particle_interps_per_s": np_ / (ms64 / 1e3),
                        "hbm_frac_step": (step_bytes(n, 8, np_ > 0) + tb) / (ms64 / 1e3) / HBM_PEAK,
                        "roofline": roofline(r64["profile"], n, 8, np_, wl["nb"], False, s64),
+                       "tracer_form": tracer_form(r64["profile"]),
                        "kernel_ms": {k: v[0] / v[1] for k, v in merge_profile(r64["profile"]).items()}}
     if args.accuracy > 0 and not args.no_fp64:
         out["accuracy"] = accuracy(B, ic, wl, args.accuracy)

@@ -20,6 +20,9 @@
 // rounding.
 #include <hipcub/hipcub.hpp>
 
+#include <cstdio>
+#include <cstdlib>
+
 #include "ctx.hpp"
 
 namespace {
@@ -932,10 +935,12 @@ struct ColCfg {
     // y steps per work item (n / KY for an unsplit column)
     static constexpr int MAXS = 256;
     // weights [PB][WQ], table, results [PB] (doubles), cells [PB][3] (ints),
-    // step ranges [2][MAXS] (uint32)
+    // step ranges [2][MAXS] (uint32), ahead of the ring: at LDS offsets an
+    // instruction's offset field carries, and the ring 1 KB aligned
     static constexpr size_t EXTRA = (size_t(PB) * WQ + TABD + PB) * 8 + size_t(PB) * 3 * 4 + size_t(2) * MAXS * 4;
+    static constexpr size_t HEAD = (EXTRA + 1023) / 1024 * 1024;
     static constexpr size_t LDS_MAX = 160 * 1024;
-    static constexpr size_t lds_of(int ky) { return size_t(2 * ky + W - 1) * SLOT_BYTES + EXTRA; }
+    static constexpr size_t lds_of(int ky) { return size_t(2 * ky + W - 1) * SLOT_BYTES + HEAD; }
     static constexpr int KY = lds_of(8) <= LDS_MAX ? 8 : (lds_of(4) <= LDS_MAX ? 4 : 0);
     static constexpr int R = 2 * KY + W - 1;
     static constexpr size_t LDS = lds_of(KY);
@@ -1005,12 +1010,12 @@ k_column(const T* __restrict__ f, const double* __restrict__ crec, double* __res
     // highest polynomial degree of an (n, m) table: 2n+1 (m = 0) or 5 (m = 2)
     constexpr int DM = W - 1 > 5 ? W - 1 : 5;
     extern __shared__ double smem[];
-    unsigned char* ring = reinterpret_cast<unsigned char*>(smem);  // R x SLOT_BYTES
-    double* wgt = smem + size_t(R) * CC::SLOT_BYTES / 8;            // [PB][WQ]
+    double* wgt = smem;                                             // [PB][WQ]
     double* tabl = wgt + PB * WQ;                                   // the weight table
     double* res = tabl + CC::TABD;                                  // [PB]
-    int* lcell = reinterpret_cast<int*>(res + PB);                  // [PB][3]
+    int* lcell = reinterpret_cast<int*>(res + PB);                  // [PB][3]: origin, planes to the ring end
     uint32_t* rng = reinterpret_cast<uint32_t*>(lcell + PB * 3);    // [2][MAXS]: step ranges of the work item
+    unsigned char* ring = reinterpret_cast<unsigned char*>(smem) + CC::HEAD;  // R x SLOT_BYTES
 
     const int n = pg.n;
     const int G = gridDim.x;
@@ -1175,10 +1180,16 @@ k_column(const T* __restrict__ f, const double* __restrict__ crec, double* __res
                         static_assert(NT / PB >= W, "one weight index per thread");
                         const int q = qq, j = tid / PB;
                         if (q < cnt && j == 0) {
+                            // the tracer's stencil origin in the window of
+                            // its first plane's ring slot, and the number of
+                            // its planes before the ring's end
                             const uint32_t pc = uint32_t(__double_as_longlong(myr.w));
-                            lcell[q * 3 + 0] = int(pc & 1023u) - it.cx0;
-                            lcell[q * 3 + 1] = int((pc >> 10) & 1023u) - it.s0 * KY;
-                            lcell[q * 3 + 2] = int((pc >> 20) & 1023u) - it.cz0;
+                            const int lc0 = int(pc & 1023u) - it.cx0;
+                            const int lc1 = int((pc >> 10) & 1023u) - it.s0 * KY;
+                            const int lc2 = int((pc >> 20) & 1023u) - it.cz0;
+                            const int sl0 = lc1 % R;
+                            lcell[q * 3 + 0] = XS + lc0 + lc2 * ZS + sl0 * SLOTV;
+                            lcell[q * 3 + 1] = R - sl0;
                         }
                         if (q < cnt && j < W) {
                             const double* c = tabl + j * (deg + 1);
@@ -1199,26 +1210,23 @@ k_column(const T* __restrict__ f, const double* __restrict__ crec, double* __res
                         }
                     }
                     __syncthreads();  // weights and cells of the batch
-                    const T* rt = reinterpret_cast<const T*>(ring);
+                    // this lane's column x = origin + jx (lanes past the
+                    // stencil read column W - 1 with weight 0) and z rows
+                    // origin + rz NI + i (consecutive rows: one base
+                    // address serves ds_read2 pairs)
+                    const T* rt = reinterpret_cast<const T*>(ring) + (jx < W ? jx : W - 1) + rz * NI * ZS;
                     for (int qb = grp; qb < cnt; qb += GROUPS) {  // whole lane groups iterate together
                         const double* wq = wgt + qb * WQ;
                         const double wxo = jx < W ? wq[jx] : 0.0;
                         double wzr[NI];
 #pragma unroll
                         for (int i = 0; i < NI; i++) wzr[i] = wq[2 * WS + rz * NI + i];
-                        const int lc0 = lcell[qb * 3 + 0], lc1 = lcell[qb * 3 + 1], lc2 = lcell[qb * 3 + 2];
-                        // this lane's column x = lc0 + jx (lanes past the
-                        // stencil read column W - 1 with weight 0), z rows
-                        // lc2 + rz NI + i (consecutive rows: one base
-                        // address serves ds_read2 pairs), y planes lc1 + jy
-                        // in ring slots
-                        const T* tcol = rt + (XS + lc0 + (jx < W ? jx : W - 1)) + (lc2 + rz * NI) * ZS;
-                        // plane jy sits in slot (lc1 + jy) mod R: from pa
-                        // before the ring's end, from pb = pa - R slots after
-                        const int sl0 = lc1 % R;
-                        const T* pa = tcol + sl0 * SLOTV;
+                        // plane jy of the stencil sits in ring slot
+                        // (sl0 + jy) mod R: from pa before the ring's end,
+                        // from pb = pa - R slots after it
+                        const T* pa = rt + lcell[qb * 3 + 0];
                         const T* pb = pa - R * SLOTV;
-                        const int wrap = R - sl0;
+                        const int wrap = lcell[qb * 3 + 1];
                         double acc = 0.0;
                         // software pipeline: plane jy + PD's LDS reads are
                         // issued before plane jy's conversions and FMAs
@@ -1520,14 +1528,58 @@ int sample_form(bfpsh_ctx* ctx, int form, const void* f, int nc, double* d_out) 
     return BFPSH_OK;
 }
 
-// The kernel form of a box's velocity loop and samples: the forced one if it
-// applies; else, once per tracer set, the fastest applicable form timed on
-// a 3-component sample of f when it beats the first applicable one by more
-// than 10 % (plan rigor MEASURE or PATIENT, as the pass autotuner), or the
-// first applicable of column walk, bricks, global gather (ESTIMATE).  The
-// forms sum in different orders: results agree to rounding, not bit for bit.
+// one velocity loop of kernel form `form` (applicable): interpolate the
+// 3-plane real field u (ctx->ru in a step) into the head slot of the history
+// ring and move the tracers
 template <typename T, int NB>
-int tracer_form(bfpsh_ctx* ctx, const void* f, int* form) {
+int loop_form(bfpsh_ctx* ctx, int form, const void* u, double dt, int order) {
+    if constexpr (ColCfg<T, NB>::ok) {
+        if (form == FORM_COLUMN) {
+            ColGeom g;
+            col_geom<T, NB>(ctx, g);
+            int r = sort_cols<T, NB>(ctx, g);
+            if (r) return r;
+            r = launch_col<T, NB, 3, 0>(ctx, g, u, ctx->d_rhs, "interp_ab_col");
+            if (r) return r;
+            LAUNCH("ab_moves", (k_ab_moves<<<unsigned((ctx->np + 255) / 256), 256, 0, ctx->stream>>>(
+                                   ctx->d_pos, ctx->d_rhs, ctx->np, ctx->nsteps, ctx->head, order, dt)));
+            return BFPSH_OK;
+        }
+    }
+    if constexpr (BrickCfg<T, NB>::ok) {
+        if (form == FORM_BRICK) {
+            int r = sort_bricks<T, NB>(ctx);
+            if (r) return r;
+            return launch_brick<T, NB, 3, 0>(ctx, u, nullptr, dt, order, "interp_ab");
+        }
+    }
+    if (ctx->sort_age >= ctx->sort_every) {
+        int r = sort_tracers(ctx);
+        if (r) return r;
+    }
+    const size_t lds = size_t(2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
+    LAUNCH("interp_ab_gather", (k_interp_ab<T, NB><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
+                                   (const T*)u, ctx->d_pos, ctx->d_rhs, ctx->d_perm + ctx->np, ctx->np,
+                                   ctx->nsteps, ctx->head, order, ctx->add_head ? 1 : 0, dt, ctx->d_tab, ctx->deg,
+                                   pgeom_d(ctx))));
+    return BFPSH_OK;
+}
+
+// The kernel form of a box's velocity loop and samples: the forced one if it
+// applies; else, once per tracer set, the form whose velocity loop (sort,
+// interpolation, Adams-Bashforth move of the highest order) runs fastest,
+// timed on a copy of the tracer state with dt = 0, when it beats the first
+// applicable one by more than 10 % (plan rigor MEASURE or PATIENT, as the
+// pass autotuner), or the first applicable of column walk, bricks, global
+// gather (ESTIMATE).  The trials interpolate ctx->ru, or the field f of nc
+// planes about to be sampled when ctx->ru is not allocated yet; with
+// neither (a 1-plane sample first) this call takes the first applicable
+// form and the choice waits for a later call.  The forms sum in different
+// orders: results agree to rounding, not bit for bit.  BFPSH_TRACER_VERBOSE
+// set in the environment prints the trials to stderr.
+template <typename T, int NB>
+int tracer_form(bfpsh_ctx* ctx, const void* f, int nc, int* form) {
+    static const char* const names[4] = {"auto", "bricks", "global gather", "column walk"};
     if (ctx->tracer_kernel != FORM_AUTO && form_ok<T, NB>(ctx, ctx->tracer_kernel)) {
         *form = ctx->tracer_kernel;
         return BFPSH_OK;
@@ -1540,25 +1592,57 @@ int tracer_form(bfpsh_ctx* ctx, const void* f, int* form) {
     for (int c : {int(FORM_COLUMN), int(FORM_BRICK), int(FORM_GATHER)})
         if (form_ok<T, NB>(ctx, c)) cand[ncand++] = c;
     int best = cand[0];
+    const void* u = ctx->ru ? ctx->ru : (nc >= 3 ? f : nullptr);
+    if (ncand > 1 && ctx->plan_rigor != BFPSH_ESTIMATE && ctx->tracer_kernel == FORM_AUTO && !u) {
+        *form = best;
+        return BFPSH_OK;
+    }
     if (ncand > 1 && ctx->plan_rigor != BFPSH_ESTIMATE && ctx->tracer_kernel == FORM_AUTO) {
-        double* tmp = nullptr;
+        // the trials move a copy of the positions with a zeroed history; the
+        // copy is refreshed before every launch sequence, so that each one
+        // locates the tracers from the real positions whatever the previous
+        // one left (its dt = 0 move adds 0 x the interpolated values)
+        const size_t pos_bytes = size_t(ctx->np) * 3 * sizeof(double);
+        const size_t rhs_bytes = size_t(ctx->nsteps) * pos_bytes;
+        double *spos = nullptr, *srhs = nullptr;
         hipEvent_t a = nullptr, b = nullptr;
-        HIPCHK(hipMalloc((void**)&tmp, size_t(ctx->np) * 3 * sizeof(double)));
+        HIPCHK(hipMalloc((void**)&spos, pos_bytes));
+        if (hipMalloc((void**)&srhs, rhs_bytes) != hipSuccess) {
+            (void)hipFree(spos);
+            return fail(ctx, BFPSH_EHIP, "tracer kernel choice: device memory for the trial state");
+        }
+        HIPCHK(hipMemsetAsync(srhs, 0, rhs_bytes, ctx->stream));
         HIPCHK(hipEventCreate(&a));
         HIPCHK(hipEventCreate(&b));
-        const bool prof = ctx->prof;
+        double* const pos = ctx->d_pos;
+        double* const rhs = ctx->d_rhs;
+        const bool prof = ctx->prof, add_head = ctx->add_head;
         ctx->prof = false;  // the trials stay out of the kernel profile
+        ctx->d_pos = spos;
+        ctx->d_rhs = srhs;
+        ctx->add_head = false;
+        auto fresh = [&]() -> int {
+            if (hipMemcpyAsync(spos, pos, pos_bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+                return BFPSH_EHIP;
+            ctx->sort_age = 1 << 30;  // every form sorts, as in a step
+            return BFPSH_OK;
+        };
         double tbest = 1e300, tfirst = 0.0;
         int r = BFPSH_OK;
         for (int i = 0; i < ncand && r == BFPSH_OK; i++) {
-            r = sample_form<T, NB>(ctx, cand[i], f, 3, tmp);  // warm-up
+            r = fresh();
+            if (r == BFPSH_OK) r = loop_form<T, NB>(ctx, cand[i], u, 0.0, ctx->nsteps);  // warm-up
+            if (r == BFPSH_OK) r = fresh();
             if (r == BFPSH_OK && hipEventRecord(a, ctx->stream) != hipSuccess) r = BFPSH_EHIP;
-            if (r == BFPSH_OK) r = sample_form<T, NB>(ctx, cand[i], f, 3, tmp);
+            if (r == BFPSH_OK) r = loop_form<T, NB>(ctx, cand[i], u, 0.0, ctx->nsteps);
             float ms = 0.f;
             if (r == BFPSH_OK && (hipEventRecord(b, ctx->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
                                   hipEventElapsedTime(&ms, a, b) != hipSuccess))
                 r = BFPSH_EHIP;
             if (i == 0) tfirst = ms;
+            if (r == BFPSH_OK && std::getenv("BFPSH_TRACER_VERBOSE"))
+                std::fprintf(stderr, "bfpsh: tracer form trial (n = %d, %s tracers, %lld of them): %s %.3f ms\n",
+                             NB, sizeof(T) == 8 ? "FP64" : "FP32", (long long)ctx->np, names[cand[i]], ms);
             if (r == BFPSH_OK && ms < tbest) {
                 tbest = ms;
                 best = cand[i];
@@ -1567,10 +1651,18 @@ int tracer_form(bfpsh_ctx* ctx, const void* f, int* form) {
         // another form only when clearly faster: near ties keep the
         // first applicable form, so that runs choose alike
         if (tbest > 0.9 * tfirst) best = cand[0];
+        if (r == BFPSH_OK && std::getenv("BFPSH_TRACER_VERBOSE"))
+            std::fprintf(stderr, "bfpsh: tracer form chosen: %s\n", names[best]);
+        // the trials' launches are done before the copies are freed
+        if (r == BFPSH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) r = BFPSH_EHIP;
+        ctx->d_pos = pos;
+        ctx->d_rhs = rhs;
+        ctx->add_head = add_head;
         ctx->prof = prof;
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
-        (void)hipFree(tmp);
+        (void)hipFree(spos);
+        (void)hipFree(srhs);
         if (r) return r == BFPSH_EHIP ? fail(ctx, r, "tracer kernel choice: HIP error") : r;
         ctx->sort_age = 1 << 30;  // the trials' sorts leave another order behind
     }
@@ -1582,46 +1674,15 @@ int tracer_form(bfpsh_ctx* ctx, const void* f, int* form) {
 template <typename T, int NB>
 int loop_nb(bfpsh_ctx* ctx, double dt, int order) {
     int form = FORM_GATHER;
-    {
-        int r = tracer_form<T, NB>(ctx, ctx->ru, &form);
-        if (r) return r;
-    }
-    if constexpr (ColCfg<T, NB>::ok) {
-        if (form == FORM_COLUMN) {
-            ColGeom g;
-            col_geom<T, NB>(ctx, g);
-            int r = sort_cols<T, NB>(ctx, g);
-            if (r) return r;
-            r = launch_col<T, NB, 3, 0>(ctx, g, ctx->ru, ctx->d_rhs, "interp_ab_col");
-            if (r) return r;
-            LAUNCH("ab_moves", (k_ab_moves<<<unsigned((ctx->np + 255) / 256), 256, 0, ctx->stream>>>(
-                                   ctx->d_pos, ctx->d_rhs, ctx->np, ctx->nsteps, ctx->head, order, dt)));
-            return BFPSH_OK;
-        }
-    }
-    if constexpr (BrickCfg<T, NB>::ok) {
-        if (form == FORM_BRICK) {
-            int r = sort_bricks<T, NB>(ctx);
-            if (r) return r;
-            return launch_brick<T, NB, 3, 0>(ctx, ctx->ru, nullptr, dt, order, "interp_ab");
-        }
-    }
-    if (ctx->sort_age >= ctx->sort_every) {
-        int r = sort_tracers(ctx);
-        if (r) return r;
-    }
-    const size_t lds = size_t(2 * NB + 2) * (ctx->deg + 1) * sizeof(double);
-    LAUNCH("interp_ab_gather", (k_interp_ab<T, NB><<<blocks_for(ctx->np, Lanes<NB>::LP), kThreads, lds, ctx->stream>>>(
-                                   (const T*)ctx->ru, ctx->d_pos, ctx->d_rhs, ctx->d_perm + ctx->np, ctx->np,
-                                   ctx->nsteps, ctx->head, order, ctx->add_head ? 1 : 0, dt, ctx->d_tab, ctx->deg,
-                                   pgeom_d(ctx))));
-    return BFPSH_OK;
+    int r = tracer_form<T, NB>(ctx, ctx->ru, 3, &form);
+    if (r) return r;
+    return loop_form<T, NB>(ctx, form, ctx->ru, dt, order);
 }
 
 template <typename T, int NB>
 int sample_nb(bfpsh_ctx* ctx, const void* f, int nc, double* d_out) {
     int form = FORM_GATHER;
-    int r = tracer_form<T, NB>(ctx, f, &form);
+    int r = tracer_form<T, NB>(ctx, f, nc, &form);
     if (r) return r;
     return sample_form<T, NB>(ctx, form, f, nc, d_out);
 }

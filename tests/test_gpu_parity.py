@@ -431,6 +431,52 @@ def test_brick_first_batches_vs_gather(B, monkeypatch, precision, n, nb, m):
     assert d < tol and r < tol
 
 
+@pytest.mark.parametrize("precision,n,nb,m", [("single", 64, 2, 1), ("double", 64, 4, 1), ("double", 32, 6, 2)])
+def test_timed_form_choice_leaves_the_state(B, monkeypatch, precision, n, nb, m):
+    """At MEASURE rigor the first tracer call times the velocity loop of
+    every applicable kernel form on a copy of the tracer state.  Here that
+    first call is a gradient sample (the real velocity buffer is not
+    allocated yet, so the trials interpolate the gradient's first three
+    planes); then 3 steps and both samples.  The same run with the chosen
+    form forced (no trials) must give the same positions, history and
+    samples bit for bit: the trials leave the real state as it was."""
+    from bfps_amd import ic
+    say("case: timed form choice %s n=%d nb=%d m=%d" % (precision, n, nb, m))
+    pos = ic.uniform_positions(20000, seed=91 + nb)
+    dt = 0.5 / n
+
+    def run(kernel):
+        if kernel:
+            monkeypatch.setenv("BFPSH_TRACER_KERNEL", kernel)
+        fs = B.vorticity_equation("t", n, precision=precision, plan_rigor="FFTW_MEASURE")
+        fs.nu, fs.forcing_type = 0.01, "linear"
+        fs.io_checkpoint(read=True, data=ic.solenoidal_vorticity(n, seed=5, k0=3.0, dtype=fs.cdtype))
+        ps = B.particles_system(fs, len(pos), nb, m, 4, step_idx=1)
+        ps.put(pos)
+        monkeypatch.delenv("BFPSH_TRACER_KERNEL", raising=False)
+        fs.profile(True)
+        fs.gradient_rspace()
+        g0 = ps.sample("gradient")
+        for _ in range(3):
+            B.NSVEparticles_step(fs, dt)
+        fs.velocity_rspace()
+        v = ps.sample("velocity")
+        fs.gradient_rspace()
+        g = ps.sample("gradient")
+        prof = fs.profile_report()
+        gp, grhs, _, gsi = ps.get()
+        fs.close()
+        form = "column" if "interp_ab_col" in prof else ("gather" if "interp_ab_gather" in prof else "brick")
+        return form, (g0, gp, grhs, v, g), gsi
+
+    form, auto, si = run(None)
+    forced_form, forced, fsi = run(form)
+    say("  chosen: %s" % form)
+    assert forced_form == form and si == fsi == 4
+    for a, b in zip(auto, forced):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("n,nb,m", [(64, 1, 1), (128, 2, 2), (128, 3, 0), (64, 3, 1)])
 def test_tracers_fp64_columns(B, oracle, monkeypatch, n, nb, m):
     """FP64 boxes of 64^3 and more with n <= 3 run the column walk too (FP64
