@@ -71,7 +71,14 @@ int bfpsh_create(bfpsh_ctx** out, int nx, int ny, int nz, double dkx, double dky
  * MEASURE and PATIENT take the wisdom's shape when a search of at least
  * their rigor chose it, else time every compiled shape (3 launches each on
  * one scratch plane for MEASURE, 10 for PATIENT) and keep the fastest in
- * the wisdom.  Boxes below 256^3 have one shape. */
+ * the wisdom.  Boxes below 256^3 have one shape.  At MEASURE and PATIENT
+ * the first tracer loop or sample of a tracer set also times the velocity
+ * loop of each applicable tracer kernel form (column walk, bricks, global
+ * gather) on a device copy of the tracer state, which it allocates for the
+ * time it runs, and keeps the fastest when it is more than 10 % faster
+ * than the first applicable one (BFPSH_TRACER_KERNEL=column|brick|gather
+ * in the environment forces a form, BFPSH_TRACER_VERBOSE prints the
+ * trials). */
 enum { BFPSH_ESTIMATE = 0, BFPSH_MEASURE = 1, BFPSH_PATIENT = 2 };
 /* strided pass kinds: inverse and forward transforms along kz (z) and ky
  * (y), the vorticity step's fused first pass (Biot-Savart + inverse z) and
