@@ -37,6 +37,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 LDS_PEAK = 128 * 256 * 2.4e9  # B/s: 128 B/clk per CU, 256 CUs, 2.4 GHz
+FP64_PEAK = 78.6e12  # MI355X FP64 vector, FLOP/s
 
 WORKLOADS = {
     "512": dict(n=512, nu=0.0012, forcing="linear", famp=0.5, fk0=2.0, fk1=4.0, np=2 ** 22, nb=6, m=2, k0=4.0,
@@ -243,6 +244,15 @@ def roofline(prof, n, es, np_, nb, traffic_ok, steps):
         out["form"] = form
         out["lds"] = {"bytes_per_launch": lds_bytes, "achieved": lds_bytes / avg_s / 1e9, "peak": LDS_PEAK / 1e9,
                       "unit": "GB/s", "frac": lds_bytes / avg_s / LDS_PEAK}
+        # FP64 arithmetic of the reference's sum
+        # (particles_field_computer.hpp:142-150): per stencil point the
+        # weight product bz by bx (2 flops) and, per component, value x
+        # weight accumulated (2 flops); the kernels do it factorized (fewer
+        # flops) but convert every value to FP64 first, and their bound is
+        # FP64 VALU issue
+        flops = np_ * w ** 3 * (2 + 3 * 2)
+        out["fp64"] = {"flops_per_launch": flops, "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK / 1e12,
+                       "unit": "TFLOP/s", "frac": flops / avg_s / FP64_PEAK}
     return out
 
 
