@@ -223,9 +223,12 @@ k_inv_z_bs(const C2<T>* __restrict__ w, C2<T>* __restrict__ out, const C2<T>* __
             }
             C2<T> u = mk<T>(T(0), T(0));
             if (k2 <= g.kM2 && k2 > 0) {
-                if (cu == 0) u = mk<T>(T(-(kyv * a2.y - kz * a1.y) / k2), T((kyv * a2.x - kz * a1.x) / k2));
-                if (cu == 1) u = mk<T>(T(-(kz * a0.y - kx * a2.y) / k2), T((kz * a0.x - kx * a2.x) / k2));
-                if (cu == 2) u = mk<T>(T(-(kx * a1.y - kyv * a0.y) / k2), T((kx * a1.x - kyv * a0.x) / k2));
+                // one FP64 division per element and field, then products
+                // (within an ulp of the quotients before the rounding to T)
+                const double ik2 = 1.0 / k2;
+                if (cu == 0) u = mk<T>(T(-(kyv * a2.y - kz * a1.y) * ik2), T((kyv * a2.x - kz * a1.x) * ik2));
+                if (cu == 1) u = mk<T>(T(-(kz * a0.y - kx * a2.y) * ik2), T((kz * a0.x - kx * a2.x) * ik2));
+                if (cu == 2) u = mk<T>(T(-(kx * a1.y - kyv * a0.y) * ik2), T((kx * a1.x - kyv * a0.x) * ik2));
             }
             if (MODE == 2) {
                 const int j = f / 3;
@@ -472,6 +475,9 @@ k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restri
     // box blockIdx.y of a batch (complex planes and real velocity move together)
     data += blockIdx.y * bstride;
     if (ru) ru += blockIdx.y * 2 * bstride;
+    // the 1/npoints normalization as a product (exact for the power-of-two
+    // sizes, within an ulp of the quotient otherwise)
+    const T inp = T(1) / npoints;
     const int tid = threadIdx.x;
     const int q0 = blockIdx.x * RP;  // first row pair; rows (y, z) flattened: row = y*N + z
     {
@@ -510,7 +516,7 @@ k_rows_cross(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __restri
             const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
             const T re = u[c1].x * w[c2].x - u[c2].x * w[c1].x;
             const T im = u[c1].y * w[c2].y - u[c2].y * w[c1].y;
-            lds[nn * LS + c * RP + p] = mk<T>(re / npoints, im / npoints);
+            lds[nn * LS + c * RP + p] = mk<T>(re * inp, im * inp);
         }
     }
     __syncthreads();
@@ -609,6 +615,7 @@ k_rows_cross_w(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __rest
     __shared__ C2<T> lds[RPB * 3 * RW::SL];
     data += blockIdx.y * bstride;
     if (ru) ru += blockIdx.y * 2 * bstride;
+    const T inp = T(1) / npoints;  // normalization as a product, as k_rows_cross
     const int tid = threadIdx.x;
     const int rp = tid / G, g = tid % G;
     const int partner = (tid & 63) - g + (G - g) % G;  // lane holding index N - k
@@ -686,7 +693,7 @@ k_rows_cross_w(C2<T>* __restrict__ data, T* __restrict__ ru, const C2<T>* __rest
             const C2<T> u1 = v[c1][eo], u2 = v[c2][eo], w1 = v[3 + c1][eo], w2 = v[3 + c2][eo];
             const T re = u1.x * w2.x - u2.x * w1.x;
             const T im = u1.y * w2.y - u2.y * w1.y;
-            f[c][e] = mk<T>(re / npoints, im / npoints);
+            f[c][e] = mk<T>(re * inp, im * inp);
         }
     }
     rows_w_fft<T, N, -1, 3>(f, reg, tw, g);
@@ -826,6 +833,7 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
 #pragma unroll
     for (int c = 0; c < 3; c++) F::run(res[c], lds, tw, grp, b);
     const double kx = kxi * g.dkx, kzv = kwave(kz, N, g.dkz);
+    const double ikM2 = 1.0 / g.kM2;
 #pragma unroll
     for (int i = 0; i < E; i++) {
         const int ky = F::out_row(grp, i);
@@ -835,7 +843,7 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
         // dealias
         C2<T> a[3];
         if (g.smooth) {
-            const double r = k2 / g.kM2;
+            const double r = k2 * ikM2;
             const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4, r16 = r8 * r8;
             const double flt = exp(-36.0 * (r16 * r2));
 #pragma unroll
@@ -867,8 +875,9 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
         }
         // force_divfree
         if (k2 > 0) {
-            const T t0 = T((kx * cu[0].x + kyv * cu[1].x + kzv * cu[2].x) / k2);
-            const T t1 = T((kx * cu[0].y + kyv * cu[1].y + kzv * cu[2].y) / k2);
+            const double ik2 = 1.0 / k2;  // one division, then products
+            const T t0 = T((kx * cu[0].x + kyv * cu[1].x + kzv * cu[2].x) * ik2);
+            const T t1 = T((kx * cu[0].y + kyv * cu[1].y + kzv * cu[2].y) * ik2);
             cu[0] = mk<T>(T(cu[0].x - t0 * kx), T(cu[0].y - t1 * kx));
             cu[1] = mk<T>(T(cu[1].x - t0 * kyv), T(cu[1].y - t1 * kyv));
             cu[2] = mk<T>(T(cu[2].x - t0 * kzv), T(cu[2].y - t1 * kzv));
@@ -905,12 +914,13 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
             C2<T> nv[3];
             if (k2 <= g.kM2) {
                 const double f0 = exp(-sp.nu * k2 * dt * 0.5);
+                const double f3 = f0 * (1.0 / 3);  // products instead of six divisions by 3
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     const C2<T> w0 = v0[c * PS + off];
                     const C2<T> w2 = vp[c * PS + off];
-                    nv[c] = mk<T>(T((w0.x * f0 + 2 * (w2.x + dt * cu[c].x)) * f0 / 3),
-                                  T((w0.y * f0 + 2 * (w2.y + dt * cu[c].y)) * f0 / 3));
+                    nv[c] = mk<T>(T((w0.x * f0 + 2 * (w2.x + dt * cu[c].x)) * f3),
+                                  T((w0.y * f0 + 2 * (w2.y + dt * cu[c].y)) * f3));
                 }
             } else {
 #pragma unroll
@@ -918,8 +928,9 @@ k_fwd_y_update(const C2<T>* __restrict__ data, const C2<T>* v0, const C2<T>* __r
             }
             // final force_divfree of the new vorticity
             if (k2 > 0) {
-                const T t0 = T((kx * nv[0].x + kyv * nv[1].x + kzv * nv[2].x) / k2);
-                const T t1 = T((kx * nv[0].y + kyv * nv[1].y + kzv * nv[2].y) / k2);
+                const double ik2 = 1.0 / k2;
+                const T t0 = T((kx * nv[0].x + kyv * nv[1].x + kzv * nv[2].x) * ik2);
+                const T t1 = T((kx * nv[0].y + kyv * nv[1].y + kzv * nv[2].y) * ik2);
                 nv[0] = mk<T>(T(nv[0].x - t0 * kx), T(nv[0].y - t1 * kx));
                 nv[1] = mk<T>(T(nv[1].x - t0 * kyv), T(nv[1].y - t1 * kyv));
                 nv[2] = mk<T>(T(nv[2].x - t0 * kzv), T(nv[2].y - t1 * kzv));
@@ -1026,8 +1037,9 @@ __global__ void k_lowpass_divfree(C2<T>* v, Geom g) {
     for (int c = 0; c < 3; c++) a[c] = v[c * ps + idx];
     if (k2 >= g.kM2) a[0] = a[1] = a[2] = mk<T>(T(0), T(0));
     if (k2 > 0) {
-        const T t0 = T((kx * a[0].x + kyv * a[1].x + kzv * a[2].x) / k2);
-        const T t1 = T((kx * a[0].y + kyv * a[1].y + kzv * a[2].y) / k2);
+        const double ik2 = 1.0 / k2;  // as the step's projections
+        const T t0 = T((kx * a[0].x + kyv * a[1].x + kzv * a[2].x) * ik2);
+        const T t1 = T((kx * a[0].y + kyv * a[1].y + kzv * a[2].y) * ik2);
         a[0] = mk<T>(T(a[0].x - t0 * kx), T(a[0].y - t1 * kx));
         a[1] = mk<T>(T(a[1].x - t0 * kyv), T(a[1].y - t1 * kyv));
         a[2] = mk<T>(T(a[2].x - t0 * kzv), T(a[2].y - t1 * kzv));
@@ -1051,9 +1063,10 @@ __global__ void k_biot_savart(const C2<T>* __restrict__ w, C2<T>* __restrict__ u
     const double k2 = kx * kx + kyv * kyv + kzv * kzv;
     const C2<T> a0 = w[idx], a1 = w[ps + idx], a2 = w[2 * ps + idx];
     if (k2 <= g.kM2 && k2 > 0) {
-        u[idx] = mk<T>(T(-(kyv * a2.y - kzv * a1.y) / k2), T((kyv * a2.x - kzv * a1.x) / k2));
-        u[ps + idx] = mk<T>(T(-(kzv * a0.y - kx * a2.y) / k2), T((kzv * a0.x - kx * a2.x) / k2));
-        u[2 * ps + idx] = mk<T>(T(-(kx * a1.y - kyv * a0.y) / k2), T((kx * a1.x - kyv * a0.x) / k2));
+        const double ik2 = 1.0 / k2;  // as the step's fused Biot-Savart pass
+        u[idx] = mk<T>(T(-(kyv * a2.y - kzv * a1.y) * ik2), T((kyv * a2.x - kzv * a1.x) * ik2));
+        u[ps + idx] = mk<T>(T(-(kzv * a0.y - kx * a2.y) * ik2), T((kzv * a0.x - kx * a2.x) * ik2));
+        u[2 * ps + idx] = mk<T>(T(-(kx * a1.y - kyv * a0.y) * ik2), T((kx * a1.x - kyv * a0.x) * ik2));
     } else {
         u[idx] = u[ps + idx] = u[2 * ps + idx] = mk<T>(T(0), T(0));
     }
